@@ -1,0 +1,269 @@
+"""ctypes binding of the C ABI in include/bpg.h (libbpg.so).
+
+This is the host-side mirror used by the tests, smoke() and bench.py.  It
+loads the in-tree library and fails loudly when it is missing: there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_float, c_int, c_long, c_uint, c_ulonglong, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().with_name("libbpg.so")
+
+TTEST, FTEST, GROUP_MEAN = 0, 1, 2
+VOXEL, CLUSTER_EXTENT, CLUSTER_MASS = 0, 1, 2
+TWOSAMPLE, CORRELATION, MEAN_DESIGN = 0, 1, 2
+PREC_FP32, PREC_SPLIT_BF16 = 0, 1
+PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16}
+
+OK = 0
+ERRORS = {-1: "BPG_ERR_ARG", -2: "BPG_ERR_HIP", -3: "BPG_ERR_NOMEM", -4: "BPG_ERR_DESIGN",
+          -5: "BPG_ERR_UNSUPPORTED", -6: "BPG_ERR_NODEVICE"}
+
+# every symbol include/bpg.h declares
+EXPORTS = [
+    "bpg_create", "bpg_destroy", "bpg_last_error", "bpg_device_name",
+    "bpg_plan_create", "bpg_plan_destroy", "bpg_plan_num_voxels",
+    "bpg_plan_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
+    "bpg_plan_pvalues", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
+    "bpg_gen_perm_correlation", "bpg_detect_designs", "bpg_max_permutations",
+    "bpg_model_precompute",
+]
+
+
+class StudyDesc(ctypes.Structure):
+    _fields_ = [("test", c_int), ("inference", c_int), ("precision", c_int),
+                ("connectivity", c_int), ("W", c_int), ("H", c_int), ("D", c_int),
+                ("N", c_int), ("p", c_int), ("nc", c_int), ("Y", c_void_p),
+                ("mask", c_void_p), ("X", c_void_p), ("C", c_void_p), ("cdt", c_float)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("phase_ms", c_double), ("transform_ms", c_double), ("operand_ms", c_double),
+                ("contract_ms", c_double), ("cluster_ms", c_double),
+                ("contract_launches", c_int), ("perm_voxels", c_long)]
+
+
+class BpgError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libbpg.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                           "python broccoli_amd/build.py")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp = c_void_p
+    sig = {
+        "bpg_create": (c_int, [c_int, POINTER(c_void_p)]),
+        "bpg_destroy": (None, [vp]),
+        "bpg_last_error": (ctypes.c_char_p, [vp]),
+        "bpg_device_name": (ctypes.c_char_p, [vp]),
+        "bpg_plan_create": (c_int, [vp, POINTER(StudyDesc), POINTER(c_void_p)]),
+        "bpg_plan_destroy": (None, [vp]),
+        "bpg_plan_num_voxels": (c_long, [vp]),
+        "bpg_plan_permutation_phase": (c_int, [vp, c_int, vp, vp, c_long, vp, vp]),
+        "bpg_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
+        "bpg_plan_observed_maps": (c_int, [vp, vp]),
+        "bpg_plan_pvalues": (c_int, [vp, vp, vp, vp, vp]),
+        "bpg_gen_sign_matrix": (c_int, [c_int, c_long, c_int, c_ulonglong, vp]),
+        "bpg_gen_perm_two_sample": (c_int, [c_int, c_int, c_int, c_long, c_int, c_ulonglong, vp]),
+        "bpg_gen_perm_correlation": (c_int, [c_int, c_long, c_int, c_ulonglong, vp]),
+        "bpg_detect_designs": (c_int, [c_int, c_int, c_int, vp, vp, c_int, vp, vp, vp]),
+        "bpg_max_permutations": (c_double, [c_int, c_int, c_int, c_int]),
+        "bpg_model_precompute": (c_int, [c_int, c_int, c_int, c_int, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def _check(rc: int, ctx=None):
+    if rc != OK:
+        msg = ""
+        if ctx is not None:
+            m = load().bpg_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise BpgError(rc, msg)
+
+
+# ---------------------------------------------------------------- generators
+
+def gen_sign_matrix(N: int, K: int, compat: bool = True, seed: int = 0) -> np.ndarray:
+    out = np.zeros((K, N), np.float32)
+    _check(load().bpg_gen_sign_matrix(N, K, int(compat), seed, _ptr(out)))
+    return out
+
+
+def gen_perm_two_sample(N: int, n1: int, K: int, compat: bool = True, seed: int = 0) -> np.ndarray:
+    out = np.zeros((K, N), np.uint16)
+    _check(load().bpg_gen_perm_two_sample(N, n1, N - n1, K, int(compat), seed, _ptr(out)))
+    return out
+
+
+def gen_perm_correlation(N: int, K: int, compat: bool = True, seed: int = 0) -> np.ndarray:
+    out = np.zeros((K, N), np.uint16)
+    _check(load().bpg_gen_perm_correlation(N, K, int(compat), seed, _ptr(out)))
+    return out
+
+
+def detect_designs(X: np.ndarray, C: np.ndarray, ftest: bool):
+    """X: (N, p) design, C: (nc, p) contrasts -> (designs, n1, n2)."""
+    N, p = X.shape
+    nc = C.shape[0]
+    Xc = np.asfortranarray(X, np.float32).ravel(order="F").copy()
+    Cc = np.ascontiguousarray(C, np.float32).ravel()
+    d = np.zeros(nc, np.int32)
+    n1 = np.zeros(nc, np.int32)
+    n2 = np.zeros(nc, np.int32)
+    _check(load().bpg_detect_designs(N, p, nc, _ptr(Xc), _ptr(Cc), int(ftest), _ptr(d), _ptr(n1), _ptr(n2)))
+    g = 1 if ftest else nc
+    return d[:g], n1[:g], n2[:g]
+
+
+def max_permutations(test: int, design: int, N: int, n2: int) -> float:
+    return load().bpg_max_permutations(test, design, N, n2)
+
+
+def model_precompute(X: np.ndarray, C: np.ndarray, ftest: bool):
+    N, p = X.shape
+    nc = C.shape[0]
+    Xc = np.asfortranarray(X, np.float32).ravel(order="F").copy()
+    Cc = np.ascontiguousarray(C, np.float32).ravel()
+    xtxxt = np.zeros(N * p, np.float32)
+    ctxtxc = np.zeros(nc * nc if ftest else nc, np.float32)
+    _check(load().bpg_model_precompute(N, p, nc, int(ftest), _ptr(Xc), _ptr(Cc), _ptr(xtxxt), _ptr(ctxtxc)))
+    return xtxxt, ctxtxc
+
+
+# ---------------------------------------------------------------- device API
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = c_void_p()
+        rc = self._lib.bpg_create(device, ctypes.byref(h))
+        if rc != OK:
+            raise BpgError(rc, f"bpg_create(device={device})")
+        self.handle = h
+
+    @property
+    def device_name(self) -> str:
+        return self._lib.bpg_device_name(self.handle).decode()
+
+    def close(self):
+        if self.handle:
+            self._lib.bpg_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Plan:
+    """A device-resident study (bpg_plan).
+
+    study: dict with W, H, D, N, p, nc, test, Y (N, D, H, W) float32, mask
+    (D, H, W) float32, X (N, p), C (nc, p); optional inference, cdt,
+    connectivity.
+    """
+
+    def __init__(self, ctx: Context, study: dict, precision: str = "split_bf16"):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.study = study
+        s = study
+        self._Y = np.ascontiguousarray(s["Y"], np.float32)
+        self._mask = np.ascontiguousarray(s["mask"], np.float32)
+        self._X = np.asfortranarray(s["X"], np.float32).ravel(order="F").copy()
+        self._C = np.ascontiguousarray(s["C"], np.float32).ravel()
+        d = StudyDesc()
+        d.test = s["test"]
+        d.inference = s.get("inference", VOXEL)
+        d.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        d.connectivity = s.get("connectivity", 26)
+        d.W, d.H, d.D = s["W"], s["H"], s["D"]
+        d.N, d.p, d.nc = s["N"], s["p"], s["nc"]
+        d.Y, d.mask, d.X, d.C = _ptr(self._Y), _ptr(self._mask), _ptr(self._X), _ptr(self._C)
+        d.cdt = s.get("cdt", 0.0)
+        self.desc = d
+        h = c_void_p()
+        _check(self._lib.bpg_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        self.nmaps = s["nc"] if s["test"] == TTEST else 1
+        self.G = s["W"] * s["H"] * s["D"]
+
+    @property
+    def num_voxels(self) -> int:
+        return self._lib.bpg_plan_num_voxels(self.handle)
+
+    def permutation_phase(self, map_index: int, perm=None, sign=None, want_stats: bool = False):
+        """Runs all permutations of one statistical map.
+
+        Returns (maxdist float32[K], stats float32[K, V_mask] or None)."""
+        if sign is not None:
+            sign = np.ascontiguousarray(sign, np.float32)
+            K = sign.shape[0]
+        else:
+            perm = np.ascontiguousarray(perm, np.uint16)
+            K = perm.shape[0]
+        maxdist = np.zeros(K, np.float32)
+        stats = np.zeros((K, self.num_voxels), np.float32) if want_stats else None
+        _check(self._lib.bpg_plan_permutation_phase(self.handle, map_index, _ptr(perm), _ptr(sign), K,
+                                                    _ptr(maxdist), _ptr(stats)), self.ctx.handle)
+        return maxdist, stats
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check(self._lib.bpg_plan_last_timing(self.handle, ctypes.byref(t)))
+        return t
+
+    def observed_maps(self) -> np.ndarray:
+        out = np.zeros(self.nmaps * self.G, np.float32)
+        _check(self._lib.bpg_plan_observed_maps(self.handle, _ptr(out)), self.ctx.handle)
+        return out.reshape(self.nmaps, self.G)
+
+    def pvalues(self, maxdists, want_labels: bool = False):
+        Ks = np.array([len(m) for m in maxdists], np.int64)
+        arrs = [np.ascontiguousarray(m, np.float32) for m in maxdists]
+        ptrs = (c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        P = np.zeros(self.nmaps * self.G, np.float32)
+        labels = np.zeros(self.nmaps * self.G, np.uint32) if want_labels else None
+        _check(self._lib.bpg_plan_pvalues(self.handle, _ptr(Ks), ctypes.cast(ptrs, c_void_p), _ptr(P),
+                                          _ptr(labels)), self.ctx.handle)
+        P = P.reshape(self.nmaps, self.G)
+        return (P, labels.reshape(self.nmaps, self.G)) if want_labels else P
+
+    def close(self):
+        if self.handle:
+            self._lib.bpg_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,534 @@
+// bpg_capi.cpp -- context, plan and permutation-phase entry points of the
+// C ABI (include/bpg.h).  Host code: validates, builds the FP64 models,
+// owns device buffers and one HIP stream per plan, and enqueues the
+// kernels of bpg_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bpg_internal.h"
+#include "bpg_kernels.h"
+
+struct bpg_ctx {
+    int device = 0;
+    std::string err;
+    std::string name;
+    std::string arch;
+};
+
+namespace {
+
+// device buffer that only grows
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+        }
+        if (want == 0) want = 1;
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+int pow2ceil(int x) {
+    int r = 1;
+    while (r < x) r <<= 1;
+    return r;
+}
+
+// FP32 operand row count for N subjects: multiples of 4 up to 64, then of 16.
+int f32_rows(int N) { return N <= 64 ? (N + 3) / 4 * 4 : (N + 15) / 16 * 16; }
+
+}  // namespace
+
+namespace bpg {
+int f32_perm_tile(int pfit, int NS);
+}
+
+struct bpg_plan {
+    bpg_ctx* ctx = nullptr;
+    bpg_study_desc d{};
+    int nmaps = 1;
+    std::vector<bpg::MapModel> models;
+    bpg::ObsModel obs;
+    int Vc = 0, Vpad = 0, nwords = 0;
+    size_t G = 0;
+    std::vector<unsigned> comp2grid;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    bpg_timing timing{};
+    bool have_obs = false;
+
+    DBuf<float> Yc;
+    DBuf<unsigned> d_comp2grid;
+    DBuf<int> d_grid2comp;
+    DBuf<double> yy, Qt, Qn, dscale;
+    DBuf<unsigned> Yb, Gb;
+    DBuf<float> Y32, G32;
+    DBuf<unsigned short> perm;
+    DBuf<float> sign;
+    DBuf<unsigned> maxbits;
+    DBuf<float> stats;
+    DBuf<unsigned> bits;
+    DBuf<unsigned> parent, sizes, maxsize, labels;
+    DBuf<double> oX, oXp, oC, osc;
+    DBuf<float> d_obs, sorted, P;
+
+    void release_all() {
+        Yc.release(); d_comp2grid.release(); d_grid2comp.release();
+        yy.release(); Qt.release(); Qn.release(); dscale.release();
+        Yb.release(); Gb.release(); Y32.release(); G32.release();
+        perm.release(); sign.release(); maxbits.release(); stats.release(); bits.release();
+        parent.release(); sizes.release(); maxsize.release(); labels.release();
+        oX.release(); oXp.release(); oC.release(); osc.release();
+        d_obs.release(); sorted.release(); P.release();
+    }
+};
+
+#define BPG_HIP(ctx, call)                                                         \
+    do {                                                                           \
+        const hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) {                                                    \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            return e_ == hipErrorOutOfMemory ? BPG_ERR_NOMEM : BPG_ERR_HIP;        \
+        }                                                                          \
+    } while (0)
+
+#define BPG_FAIL(ctx, code, msg) \
+    do {                         \
+        (ctx)->err = (msg);      \
+        return (code);           \
+    } while (0)
+
+extern "C" int bpg_create(int device, bpg_ctx** out) {
+    if (!out) return BPG_ERR_ARG;
+    *out = nullptr;
+    bpg_ctx* c = new (std::nothrow) bpg_ctx();
+    if (!c) return BPG_ERR_NOMEM;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        delete c;
+        return BPG_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= count) {
+        delete c;
+        return BPG_ERR_NODEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return BPG_ERR_HIP;
+    }
+    c->device = device;
+    c->name = prop.name;
+    c->arch = prop.gcnArchName;
+    if (c->arch.compare(0, 6, "gfx950") != 0) {
+        delete c;
+        return BPG_ERR_UNSUPPORTED;
+    }
+    *out = c;
+    return BPG_OK;
+}
+
+extern "C" void bpg_destroy(bpg_ctx* ctx) { delete ctx; }
+
+extern "C" const char* bpg_last_error(const bpg_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" const char* bpg_device_name(const bpg_ctx* ctx) { return ctx ? ctx->name.c_str() : ""; }
+
+extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_plan** out) {
+    if (!ctx || !desc || !out) return BPG_ERR_ARG;
+    *out = nullptr;
+    const bpg_study_desc& d = *desc;
+    if (d.test != BPG_TTEST && d.test != BPG_FTEST && d.test != BPG_GROUP_MEAN)
+        BPG_FAIL(ctx, BPG_ERR_ARG, "unknown statistical test");
+    if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS)
+        BPG_FAIL(ctx, BPG_ERR_ARG, "unknown inference mode");
+    if (d.inference == BPG_CLUSTER_MASS)
+        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "cluster-mass inference is not built");
+    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16)
+        BPG_FAIL(ctx, BPG_ERR_ARG, "unknown precision");
+    if (d.connectivity != 26 && d.connectivity != 6) BPG_FAIL(ctx, BPG_ERR_ARG, "connectivity must be 26 or 6");
+    if (d.W <= 0 || d.H <= 0 || d.D <= 0) BPG_FAIL(ctx, BPG_ERR_ARG, "grid dimensions must be positive");
+    if ((double)d.W * d.H * d.D * 3.0 >= 4294967295.0) BPG_FAIL(ctx, BPG_ERR_ARG, "grid too large for 32-bit labels");
+    if (!d.Y || !d.mask || !d.X || !d.C) BPG_FAIL(ctx, BPG_ERR_ARG, "null input pointer");
+    const int p = d.test == BPG_GROUP_MEAN ? 1 : d.p;
+    if (p < 1 || p > BPG_MAX_REGRESSORS) BPG_FAIL(ctx, BPG_ERR_ARG, "number of regressors must be 1..25");
+    if (d.nc < 1) BPG_FAIL(ctx, BPG_ERR_ARG, "number of contrasts must be positive");
+    if (d.test == BPG_FTEST && (d.nc > BPG_MAX_FROWS || d.nc > p))
+        BPG_FAIL(ctx, BPG_ERR_ARG, "F contrast must have 1..min(10, p) rows");
+    if (d.N <= p || d.N > BPG_MAX_SUBJECTS) BPG_FAIL(ctx, BPG_ERR_ARG, "need p < N <= 256 subjects");
+    if (d.test == BPG_FTEST && d.N <= p + 1) BPG_FAIL(ctx, BPG_ERR_ARG, "F test needs N > p + 1");
+    if (d.precision == BPG_PREC_SPLIT_BF16 && p > 16)
+        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "split-BF16 form supports p <= 16; use the FP32 form");
+
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    bpg_plan* pl = new (std::nothrow) bpg_plan();
+    if (!pl) return BPG_ERR_NOMEM;
+    pl->ctx = ctx;
+    pl->d = d;
+    pl->d.p = p;
+    if (d.test == BPG_GROUP_MEAN) pl->d.nc = 1;
+    pl->d.Y = pl->d.mask = pl->d.X = pl->d.C = nullptr;  // not retained
+    pl->nmaps = d.test == BPG_TTEST ? d.nc : 1;
+    // models (FP64, host)
+    bpg_study_desc dm = d;
+    dm.p = p;
+    if (d.test == BPG_GROUP_MEAN) dm.nc = 1;
+    pl->models.resize(pl->nmaps);
+    for (int m = 0; m < pl->nmaps; m++) {
+        const int rc = bpg::build_map_model(dm, m, pl->models[m], ctx->err);
+        if (rc) { delete pl; return rc; }
+    }
+    {
+        const int rc = bpg::build_obs_model(dm, pl->obs, ctx->err);
+        if (rc) { delete pl; return rc; }
+    }
+    // compaction: in-mask voxels in linear order (mask == 1.0f, K2:1586)
+    pl->G = (size_t)d.W * d.H * d.D;
+    std::vector<int> grid2comp(pl->G, -1);
+    for (size_t g = 0; g < pl->G; g++)
+        if (d.mask[g] == 1.0f) {
+            grid2comp[g] = (int)pl->comp2grid.size();
+            pl->comp2grid.push_back((unsigned)g);
+        }
+    pl->Vc = (int)pl->comp2grid.size();
+    pl->Vpad = std::max(32, (pl->Vc + 255) / 256 * 256);
+    pl->nwords = pl->Vpad / 32;
+    std::vector<float> yc((size_t)d.N * std::max(pl->Vc, 1));
+    for (int s = 0; s < d.N; s++) {
+        const float* src = d.Y + (size_t)s * pl->G;
+        float* dst = yc.data() + (size_t)s * pl->Vc;
+        for (int v = 0; v < pl->Vc; v++) dst[v] = src[pl->comp2grid[v]];
+    }
+    auto fail = [&](int rc) { pl->release_all(); if (pl->stream) (void)hipStreamDestroy(pl->stream); delete pl; return rc; };
+    if (hipStreamCreate(&pl->stream) != hipSuccess) { ctx->err = "hipStreamCreate failed"; return fail(BPG_ERR_HIP); }
+    for (auto& e : pl->ev)
+        if (hipEventCreate(&e) != hipSuccess) { ctx->err = "hipEventCreate failed"; return fail(BPG_ERR_HIP); }
+    if (pl->Yc.ensure(yc.size()) || pl->d_comp2grid.ensure(std::max(pl->Vc, 1)) ||
+        pl->d_grid2comp.ensure(pl->G)) {
+        ctx->err = "device allocation of the study failed";
+        return fail(BPG_ERR_NOMEM);
+    }
+    if (hipMemcpy(pl->Yc.p, yc.data(), yc.size() * sizeof(float), hipMemcpyHostToDevice) ||
+        hipMemcpy(pl->d_comp2grid.p, pl->comp2grid.data(), pl->Vc * sizeof(unsigned), hipMemcpyHostToDevice) ||
+        hipMemcpy(pl->d_grid2comp.p, grid2comp.data(), pl->G * sizeof(int), hipMemcpyHostToDevice)) {
+        ctx->err = "upload of the study failed";
+        return fail(BPG_ERR_HIP);
+    }
+    *out = pl;
+    return BPG_OK;
+}
+
+extern "C" void bpg_plan_destroy(bpg_plan* pl) {
+    if (!pl) return;
+    (void)hipSetDevice(pl->ctx->device);
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    pl->release_all();
+    for (auto& e : pl->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    delete pl;
+}
+
+extern "C" long bpg_plan_num_voxels(const bpg_plan* pl) { return pl ? pl->Vc : -1; }
+
+namespace {
+
+// (int)(T * 10000.0f) as KM:299, made total (saturating, NaN -> INT_MIN).
+int quantize(float t) {
+    const float x = t * 10000.0f;
+    if (std::isnan(x)) return INT_MIN;
+    if (x >= 2147483648.0f) return INT_MAX;
+    if (x <= -2147483648.0f) return INT_MIN;
+    return (int)x;
+}
+
+// The empty-mask value of LIB:10971: the int buffer starts from the bit
+// pattern of -1e6f and is returned as (float)max / 10000.0f.
+float empty_max() {
+    const float init = -1000000.0f;
+    int m;
+    std::memcpy(&m, &init, sizeof m);
+    return (float)((float)m / 10000.0f);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned short* perm,
+                                          const float* sign, long K, float* maxdist, float* stats) {
+    if (!pl) return BPG_ERR_ARG;
+    bpg_ctx* ctx = pl->ctx;
+    const bpg_study_desc& d = pl->d;
+    if (map < 0 || map >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
+    if (K <= 0 || K > (1L << 26)) BPG_FAIL(ctx, BPG_ERR_ARG, "number of permutations out of range");
+    if (!maxdist) BPG_FAIL(ctx, BPG_ERR_ARG, "null maxdist");
+    const bpg::MapModel& m = pl->models[map];
+    if (m.sign_operand ? !sign : !perm) BPG_FAIL(ctx, BPG_ERR_ARG, "missing permutation / sign matrix");
+    const int N = d.N;
+    if (!m.sign_operand)
+        for (long i = 0; i < K * (long)N; i++)
+            if (perm[i] >= N) BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = pl->stream;
+    const bool bf = d.precision == BPG_PREC_SPLIT_BF16;
+    const bool cluster = d.inference != BPG_VOXEL;
+    const int pfit = m.pfit;
+    const int KK = (N + 15) / 16;
+    const int NS = f32_rows(N);
+    const int PP = pow2ceil(pfit);
+    const int Vc = pl->Vc, Vpad = pl->Vpad;
+    const int n_sub = bf ? (int)((K * PP + 31) / 32) : 0;
+    const int PT = bf ? 1 : bpg::f32_perm_tile(pfit, NS);
+    const long Kpad = bf ? K : (K + PT - 1) / PT * PT;
+
+    // buffers
+    BPG_HIP(ctx, pl->yy.ensure(Vpad));
+    BPG_HIP(ctx, pl->Qt.ensure((size_t)N * pfit));
+    BPG_HIP(ctx, pl->Qn.ensure((size_t)N * std::max(m.q, 1)));
+    BPG_HIP(ctx, pl->dscale.ensure(N));
+    if (bf) {
+        BPG_HIP(ctx, pl->Yb.ensure((size_t)(Vpad / 32) * KK * 2 * 64 * 4));
+        BPG_HIP(ctx, pl->Gb.ensure((size_t)n_sub * KK * 2 * 64 * 4));
+    } else {
+        BPG_HIP(ctx, pl->Y32.ensure((size_t)NS * Vpad));
+        BPG_HIP(ctx, pl->G32.ensure((size_t)Kpad * pfit * NS));
+    }
+    if (m.sign_operand) BPG_HIP(ctx, pl->sign.ensure((size_t)K * N));
+    else BPG_HIP(ctx, pl->perm.ensure((size_t)K * N));
+    BPG_HIP(ctx, pl->maxbits.ensure(K));
+    if (stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
+    if (cluster) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
+
+    BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, m.Qt.data(), m.Qt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    if (m.q > 0)
+        BPG_HIP(ctx, hipMemcpyAsync(pl->Qn.p, m.Qn.data(), m.Qn.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, m.dscale.data(), N * sizeof(double), hipMemcpyHostToDevice, st));
+    {
+        bpg::TransformArgs ta{};
+        ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;
+        ta.N = N; ta.q = m.q; ta.Vc = Vc; ta.Vpad = Vpad;
+        ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : NS;
+        ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf ? nullptr : pl->Y32.p;
+        BPG_HIP(ctx, bpg::launch_transform(ta, st));
+    }
+    BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
+    {
+        if (m.sign_operand)
+            BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p, sign, (size_t)K * N * sizeof(float), hipMemcpyHostToDevice, st));
+        else
+            BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, perm, (size_t)K * N * sizeof(unsigned short), hipMemcpyHostToDevice, st));
+        bpg::OperandArgs oa{};
+        oa.perm = m.sign_operand ? nullptr : pl->perm.p;
+        oa.sign = m.sign_operand ? pl->sign.p : nullptr;
+        oa.Qt = pl->Qt.p;
+        oa.N = N; oa.pfit = pfit; oa.K = (int)K; oa.PP = PP; oa.KK = KK; oa.NS = NS;
+        if (bf) {
+            oa.n_sub = n_sub;
+            oa.Gb = pl->Gb.p;
+            BPG_HIP(ctx, bpg::launch_operand_bf16(oa, st));
+        } else {
+            oa.n_sub = (int)Kpad;
+            oa.G32 = pl->G32.p;
+            BPG_HIP(ctx, bpg::launch_operand_f32(oa, st));
+        }
+    }
+    BPG_HIP(ctx, hipMemsetAsync(pl->maxbits.p, 0, K * sizeof(unsigned), st));
+    BPG_HIP(ctx, hipEventRecord(pl->ev[2], st));
+    int launches = 0;
+    {
+        bpg::ContractArgs ca{};
+        ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p;
+        ca.yy = pl->yy.p; ca.maxbits = pl->maxbits.p;
+        ca.stats = stats ? pl->stats.p : nullptr;
+        ca.bits = cluster ? pl->bits.p : nullptr;
+        ca.nwords = pl->nwords;
+        ca.K = (int)K; ca.Vc = Vc; ca.Vpad = Vpad; ca.N = N; ca.pfit = pfit; ca.r = m.r; ca.NS = NS;
+        ca.ftest = d.test == BPG_FTEST;
+        ca.sqrt_dof = (float)std::sqrt((double)m.dof);
+        ca.f_scale = (float)((double)m.dof / (double)m.r);
+        ca.cdt = d.cdt;
+        if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, KK, PP, m.sign_operand, st, &launches));
+        else BPG_HIP(ctx, bpg::launch_contract_f32(ca, NS, st, &launches));
+    }
+    BPG_HIP(ctx, hipEventRecord(pl->ev[3], st));
+    std::vector<unsigned> host(K);
+    if (cluster) {
+        // batches of maps, each with a grid-sized parent / size array
+        const size_t budget = (size_t)2 << 30;
+        long B = (long)std::max<size_t>(1, budget / (pl->G * 2 * sizeof(unsigned)));
+        B = std::min<long>(B, K);
+        BPG_HIP(ctx, pl->parent.ensure((size_t)B * pl->G));
+        BPG_HIP(ctx, pl->sizes.ensure((size_t)B * pl->G));
+        BPG_HIP(ctx, pl->maxsize.ensure(K));
+        BPG_HIP(ctx, hipMemsetAsync(pl->maxsize.p, 0, K * sizeof(unsigned), st));
+        for (long k0 = 0; k0 < K; k0 += B) {
+            bpg::CclArgs cc{};
+            cc.bits = pl->bits.p + (size_t)k0 * pl->nwords;
+            cc.comp2grid = pl->d_comp2grid.p;
+            cc.grid2comp = pl->d_grid2comp.p;
+            cc.parent = pl->parent.p;
+            cc.sizes = pl->sizes.p;
+            cc.maxsize = pl->maxsize.p + k0;
+            cc.labels = nullptr;
+            cc.W = d.W; cc.H = d.H; cc.D = d.D; cc.Vc = Vc; cc.nwords = pl->nwords;
+            cc.nmaps = (int)std::min<long>(B, K - k0);
+            cc.connectivity = d.connectivity;
+            BPG_HIP(ctx, bpg::launch_ccl(cc, st));
+        }
+        BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
+        BPG_HIP(ctx, hipMemcpyAsync(host.data(), pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    } else {
+        BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
+        BPG_HIP(ctx, hipMemcpyAsync(host.data(), pl->maxbits.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    }
+    if (stats)
+        BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
+    BPG_HIP(ctx, hipEventRecord(pl->ev[5], st));
+    BPG_HIP(ctx, hipStreamSynchronize(st));
+    for (long k = 0; k < K; k++) {
+        if (cluster) {
+            maxdist[k] = (float)host[k];  // CalculateLargestCluster, LIB:18453-18456
+        } else if (Vc == 0 || host[k] == 0u) {
+            maxdist[k] = empty_max();
+        } else {
+            // trunc(max * 1e4) / 1e4 (KM:299, LIB:10988); trunc is monotone,
+            // so it commutes with the max.
+            maxdist[k] = (float)((float)quantize(bpg::decode_max(host[k])) / 10000.0f);
+        }
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    bpg_timing& tm = pl->timing;
+    tm.phase_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    tm.transform_ms = elapsed(pl->ev[0], pl->ev[1]);
+    tm.operand_ms = elapsed(pl->ev[1], pl->ev[2]);
+    tm.contract_ms = elapsed(pl->ev[2], pl->ev[3]);
+    tm.cluster_ms = cluster ? elapsed(pl->ev[3], pl->ev[4]) : 0.0;
+    tm.contract_launches = launches;
+    tm.perm_voxels = K * (long)Vc;
+    return BPG_OK;
+}
+
+extern "C" int bpg_plan_last_timing(const bpg_plan* pl, bpg_timing* out) {
+    if (!pl || !out) return BPG_ERR_ARG;
+    *out = pl->timing;
+    return BPG_OK;
+}
+
+extern "C" int bpg_plan_observed_maps(bpg_plan* pl, float* maps) {
+    if (!pl || !maps) return BPG_ERR_ARG;
+    bpg_ctx* ctx = pl->ctx;
+    const bpg::ObsModel& o = pl->obs;
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = pl->stream;
+    const int Vc = pl->Vc, N = pl->d.N;
+    BPG_HIP(ctx, pl->oX.ensure(o.X.size()));
+    BPG_HIP(ctx, pl->oXp.ensure(o.Xp.size()));
+    BPG_HIP(ctx, pl->oC.ensure(o.C.size()));
+    BPG_HIP(ctx, pl->osc.ensure(o.sc.size()));
+    BPG_HIP(ctx, pl->d_obs.ensure((size_t)pl->nmaps * std::max(Vc, 1)));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->oX.p, o.X.data(), o.X.size() * 8, hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->oXp.p, o.Xp.data(), o.Xp.size() * 8, hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->oC.p, o.C.data(), o.C.size() * 8, hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->osc.p, o.sc.data(), o.sc.size() * 8, hipMemcpyHostToDevice, st));
+    bpg::ObsArgs a{};
+    a.Yc = pl->Yc.p; a.X = pl->oX.p; a.Xp = pl->oXp.p; a.C = pl->oC.p; a.sc = pl->osc.p;
+    a.N = N; a.p = o.p; a.nc = o.nc; a.Vc = Vc; a.ftest = o.ftest; a.nmaps = pl->nmaps;
+    a.out = pl->d_obs.p;
+    BPG_HIP(ctx, bpg::launch_observed(a, st));
+    std::vector<float> h((size_t)pl->nmaps * Vc);
+    if (Vc > 0)
+        BPG_HIP(ctx, hipMemcpyAsync(h.data(), pl->d_obs.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    BPG_HIP(ctx, hipStreamSynchronize(st));
+    std::fill(maps, maps + (size_t)pl->nmaps * pl->G, 0.0f);
+    for (int c = 0; c < pl->nmaps; c++)
+        for (int v = 0; v < Vc; v++) maps[(size_t)c * pl->G + pl->comp2grid[v]] = h[(size_t)c * Vc + v];
+    pl->have_obs = true;
+    return BPG_OK;
+}
+
+extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const* maxdist,
+                                float* pvalues, unsigned* labels) {
+    if (!pl || !K || !maxdist || !pvalues) return BPG_ERR_ARG;
+    bpg_ctx* ctx = pl->ctx;
+    if (!pl->have_obs) BPG_FAIL(ctx, BPG_ERR_ARG, "observed maps have not been computed");
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = pl->stream;
+    const bpg_study_desc& d = pl->d;
+    const int Vc = pl->Vc;
+    const bool cluster = d.inference != BPG_VOXEL;
+    std::fill(pvalues, pvalues + (size_t)pl->nmaps * pl->G, 0.0f);
+    if (labels) std::fill(labels, labels + (size_t)pl->nmaps * pl->G, (unsigned)(pl->G * 3));
+    if (Vc == 0) return BPG_OK;
+    BPG_HIP(ctx, pl->P.ensure(Vc));
+    if (cluster) {
+        BPG_HIP(ctx, pl->bits.ensure(pl->nwords));
+        BPG_HIP(ctx, pl->parent.ensure(pl->G));
+        BPG_HIP(ctx, pl->sizes.ensure(pl->G));
+        BPG_HIP(ctx, pl->labels.ensure(pl->G));
+        BPG_HIP(ctx, pl->maxsize.ensure(1));
+    }
+    std::vector<float> h(Vc);
+    for (int c = 0; c < pl->nmaps; c++) {
+        if (K[c] <= 0 || !maxdist[c]) BPG_FAIL(ctx, BPG_ERR_ARG, "empty null distribution");
+        std::vector<float> srt(maxdist[c], maxdist[c] + K[c]);
+        std::sort(srt.begin(), srt.end());
+        BPG_HIP(ctx, pl->sorted.ensure(K[c]));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->sorted.p, srt.data(), K[c] * sizeof(float), hipMemcpyHostToDevice, st));
+        const float* obs = pl->d_obs.p + (size_t)c * Vc;
+        if (cluster) {
+            // ClusterizeOpenCL on the observed map (LIB:16822, LIB:18301-18401)
+            BPG_HIP(ctx, bpg::launch_threshold_bits(obs, Vc, d.cdt, pl->bits.p, pl->nwords, st));
+            BPG_HIP(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(pl->labels.p), (int)(pl->G * 3), pl->G, st));
+            BPG_HIP(ctx, hipMemsetAsync(pl->maxsize.p, 0, sizeof(unsigned), st));
+            bpg::CclArgs cc{};
+            cc.bits = pl->bits.p; cc.comp2grid = pl->d_comp2grid.p; cc.grid2comp = pl->d_grid2comp.p;
+            cc.parent = pl->parent.p; cc.sizes = pl->sizes.p; cc.maxsize = pl->maxsize.p;
+            cc.labels = pl->labels.p;
+            cc.W = d.W; cc.H = d.H; cc.D = d.D; cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = 1;
+            cc.connectivity = d.connectivity;
+            BPG_HIP(ctx, bpg::launch_ccl(cc, st));
+        }
+        bpg::PvalArgs pa{};
+        pa.obs = obs; pa.sorted_max = pl->sorted.p;
+        pa.labels = cluster ? pl->labels.p : nullptr;
+        pa.sizes = cluster ? pl->sizes.p : nullptr;
+        pa.comp2grid = pl->d_comp2grid.p;
+        pa.K = (int)K[c]; pa.Vc = Vc; pa.inference = d.inference; pa.cdt = d.cdt; pa.P = pl->P.p;
+        BPG_HIP(ctx, bpg::launch_pvalues(pa, st));
+        BPG_HIP(ctx, hipMemcpyAsync(h.data(), pl->P.p, Vc * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (cluster && labels)
+            BPG_HIP(ctx, hipMemcpyAsync(labels + (size_t)c * pl->G, pl->labels.p, pl->G * sizeof(unsigned),
+                                        hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipStreamSynchronize(st));
+        for (int v = 0; v < Vc; v++) pvalues[(size_t)c * pl->G + pl->comp2grid[v]] = h[v];
+    }
+    return BPG_OK;
+}

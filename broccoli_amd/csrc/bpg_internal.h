@@ -1,0 +1,49 @@
+// bpg_internal.h -- shared declarations of the bpg library (not installed).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bpg.h"
+
+namespace bpg {
+
+// FP64 model of one statistical map (one t contrast, the F contrast, or the
+// group mean).  The permuted fit of the reference (K2:1562-1669: OLS of the
+// residualised data on the row-permuted design) is rewritten with an
+// orthonormal basis Qt of col(X) whose first r columns span the contrast
+// space X (X'X)^-1 C':
+//   z   = Qt' P' y*                 (p values per voxel and permutation)
+//   SSE = |y*|^2 - |z|^2
+//   t   = z_0 * sqrt(dof / SSE)                  (= c b / sqrt(s2 c(X'X)^-1 c'))
+//   F   = (z_0^2 + .. + z_{r-1}^2) * dof / (r SSE)
+// Qn spans the nuisance space X null(C) projected out before the loop
+// (LIB:16654-16727).  For the group mean the operand rows are scaled by
+// Qt(:,0) and the permutation operand holds the exact signs.
+struct MapModel {
+    int r = 1;      // contrast rows
+    int q = 0;      // nuisance dimension (0: no residualisation)
+    int pfit = 1;   // regressors of the permuted fit
+    int dof = 1;    // N - pfit
+    std::vector<double> Qt;     // N x pfit, column-major
+    std::vector<double> Qn;     // N x q, column-major
+    std::vector<double> dscale; // N, operand row scale
+    bool sign_operand = false;  // group mean: the operand is the sign matrix
+};
+
+// Observed-map model (K1:51-128, K1:1456-1673) in FP64.
+struct ObsModel {
+    int p = 1, nc = 1, nmaps = 1;
+    bool ftest = false;
+    std::vector<double> X;    // N x p column-major
+    std::vector<double> Xp;   // p x N: Xp[r + s*p] = pinv(X)(r,s)
+    std::vector<double> C;    // C[r + c*p]
+    std::vector<double> sc;   // t: c(X'X)^-1c' per contrast; F: (C(X'X)^-1C')^-1
+};
+
+int householder_qr(int m, int n, const double* A, double* Q, double* R);
+int build_map_model(const bpg_study_desc& d, int map, MapModel& out, std::string& err);
+int build_obs_model(const bpg_study_desc& d, ObsModel& out, std::string& err);
+
+}  // namespace bpg

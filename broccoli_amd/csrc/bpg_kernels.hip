@@ -1,0 +1,745 @@
+// bpg_kernels.hip -- gfx950 (CDNA4) kernels of the permutation-test GLM.
+//
+// The reference evaluates one permutation per launch, one voxel per
+// work-item, re-reading the data twice per permutation
+// (code/Kernels/kernelStatistics2.cpp:1562-1669).  Here the permuted fit of
+// every voxel for a whole tile of permutations is one contraction
+//     Z = G Y*      G: (K * regressors) x N   permutation operand
+//                   Y*: N x V                  residualised data (in HBM once)
+// followed, in registers, by the statistic, the per-permutation running
+// maximum (max-T) and an optional supra-threshold bitmask for cluster
+// inference.  Two arithmetic forms:
+//   - split-BF16 MFMA: G = Ghi + Glo, Y = Yhi + Ylo (bf16 each), products
+//     Ghi*Yhi + Ghi*Ylo + Glo*Yhi on v_mfma_f32_32x32x16_bf16 with FP32
+//     accumulation (the Glo term is dropped when G holds exact +-1 signs);
+//   - plain FP32 on the vector ALUs.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "bpg.h"
+#include "bpg_kernels.h"
+
+namespace bpg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// float -> bf16 bits, round to nearest even
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// order-preserving float <-> uint encoding for atomicMax
+__device__ __forceinline__ unsigned enc_max(float f) {
+    unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+float decode_max(unsigned e) {
+    unsigned u = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
+    float f;
+    std::memcpy(&f, &u, sizeof f);
+    return f;
+}
+
+// Blocks are dealt round-robin to the 8 XCDs.  Renumber them so that the
+// blocks resident on one XCD walk consecutive logical ids, i.e. share the
+// same voxel chunk (and its lines in that XCD's L2).
+__device__ __forceinline__ int xcd_logical_block(int per_xcd) {
+    const int b = blockIdx.x;
+    return (b & 7) * per_xcd + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// nuisance residualisation + operand split (once per statistical map)
+// ---------------------------------------------------------------------------
+// y* = y - Qn (Qn' y)  (the projector I - W2 pinv(W2) of LIB:16684-16689,
+// applied in FP64), yy = |y*|^2 in FP64, operand row s = dscale_s * y*_s.
+// One thread per voxel.
+__global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.Vpad) return;
+    const int N = a.N;
+    const bool in = v < a.Vc;
+    double w[24];
+#pragma unroll
+    for (int j = 0; j < 24; j++) w[j] = 0.0;
+    if (in && a.q > 0) {
+        for (int s = 0; s < N; s++) {
+            const double ys = (double)a.Yc[(size_t)s * a.Vc + v];
+#pragma unroll
+            for (int j = 0; j < 24; j++)
+                if (j < a.q) w[j] += a.Qn[s + (size_t)j * N] * ys;
+        }
+    }
+    double yy = 0.0;
+    const int vt = v >> 5, col = v & 31;
+    unsigned short* Yb16 = reinterpret_cast<unsigned short*>(a.Yb);
+    const int srows = (a.KK * 16 > a.NS) ? a.KK * 16 : a.NS;
+    for (int s = 0; s < srows; s++) {
+        double o = 0.0;
+        if (in && s < N) {
+            double ys = (double)a.Yc[(size_t)s * a.Vc + v];
+#pragma unroll
+            for (int j = 0; j < 24; j++)
+                if (j < a.q) ys -= a.Qn[s + (size_t)j * N] * w[j];
+            yy += ys * ys;
+            o = a.dscale[s] * ys;
+        }
+        if (a.Y32 && s < a.NS) a.Y32[(size_t)s * a.Vpad + v] = (float)o;
+        if (a.Yb && s < a.KK * 16) {
+            const int kk = s >> 4, h = (s >> 3) & 1, e = s & 7;
+            const int lane = h * 32 + col;
+            const size_t base = (((size_t)vt * a.KK + kk) * 2) * 64 + lane;
+            const unsigned short hi = f2bf((float)o);
+            const unsigned short lo = f2bf((float)(o - (double)bf2f(hi)));
+            Yb16[base * 8 + e] = hi;
+            Yb16[(base + 64) * 8 + e] = lo;
+        }
+    }
+    a.yy[v] = in ? yy : 1.0;
+}
+
+hipError_t launch_transform(const TransformArgs& a, hipStream_t s) {
+    if (a.q > 24) return hipErrorInvalidValue;
+    const int blocks = (a.Vpad + 255) / 256;
+    hipLaunchKernelGGL(k_transform, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// permutation operand
+// ---------------------------------------------------------------------------
+// Row i of a 32-row MFMA subtile lands in lane half h = (i>>2)&1, slot
+// j = 4*(i>>3) + (i&3) of the 32x32 accumulator.  Slots are grouped so one
+// lane holds PP consecutive slots per permutation: permutation
+// h*(16/PP) + j/PP, regressor j%PP.  Each lane then owns whole permutations
+// of one voxel and the statistic needs no cross-lane traffic.
+__global__ __launch_bounds__(256) void k_operand_bf16(OperandArgs a) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)a.n_sub * a.KK * 2 * 64;
+    if (t >= total) return;
+    const int lane = (int)(t & 63);
+    const int part = (int)((t >> 6) & 1);
+    const size_t rest = t >> 7;
+    const int kk = (int)(rest % a.KK);
+    const int st = (int)(rest / a.KK);
+    const int i = lane & 31, hl = lane >> 5;
+    const int j = 4 * (i >> 3) + (i & 3);
+    const int hh = (i >> 2) & 1;
+    const int per = 16 / a.PP;
+    const int k = st * (32 / a.PP) + hh * per + j / a.PP;
+    const int reg = j % a.PP;
+    const int s0 = kk * 16 + hl * 8;
+    unsigned short out[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int s = s0 + e;
+        double val = 0.0;
+        if (k < a.K && reg < a.pfit && s < a.N) {
+            if (a.sign) val = (double)a.sign[(size_t)k * a.N + s];
+            else val = a.Qt[(size_t)a.perm[(size_t)k * a.N + s] + (size_t)reg * a.N];
+        }
+        const unsigned short hi = f2bf((float)val);
+        out[e] = part == 0 ? hi : f2bf((float)(val - (double)bf2f(hi)));
+    }
+    uint4 pk;
+    pk.x = out[0] | ((unsigned)out[1] << 16);
+    pk.y = out[2] | ((unsigned)out[3] << 16);
+    pk.z = out[4] | ((unsigned)out[5] << 16);
+    pk.w = out[6] | ((unsigned)out[7] << 16);
+    reinterpret_cast<uint4*>(a.Gb)[t] = pk;
+}
+
+hipError_t launch_operand_bf16(const OperandArgs& a, hipStream_t s) {
+    const size_t total = (size_t)a.n_sub * a.KK * 2 * 64;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(k_operand_bf16, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// G32[(k*pfit + i)*NS + s]; rows are padded to the launch's permutation
+// tile with zeros (the buffer is sized for that, K here is the padded count
+// in a.n_sub).
+__global__ __launch_bounds__(256) void k_operand_f32(OperandArgs a) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)a.n_sub * a.pfit * a.NS;
+    if (t >= total) return;
+    const int s = (int)(t % a.NS);
+    const size_t row = t / a.NS;
+    const int i = (int)(row % a.pfit);
+    const int k = (int)(row / a.pfit);
+    float val = 0.0f;
+    if (k < a.K && s < a.N) {
+        if (a.sign) val = a.sign[(size_t)k * a.N + s];
+        else val = (float)a.Qt[(size_t)a.perm[(size_t)k * a.N + s] + (size_t)i * a.N];
+    }
+    a.G32[t] = val;
+}
+
+hipError_t launch_operand_f32(const OperandArgs& a, hipStream_t s) {
+    const size_t total = (size_t)a.n_sub * a.pfit * a.NS;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(k_operand_f32, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// statistic epilogue (shared by both forms)
+// ---------------------------------------------------------------------------
+// SSE = |y*|^2 - |z|^2 with the subtraction in FP64 (|y*|^2 is exact in
+// FP64; the difference is then rounded once).
+__device__ __forceinline__ float stat_t(float z0, float ss, double yyv, float sqrt_dof) {
+    const float sse = (float)(yyv - (double)ss);
+    return z0 * sqrt_dof * rsqrtf(sse);
+}
+__device__ __forceinline__ float stat_f(float num, float ss, double yyv, float f_scale) {
+    const float sse = (float)(yyv - (double)ss);
+    return num * f_scale / sse;
+}
+
+// ---------------------------------------------------------------------------
+// split-BF16 MFMA contraction
+// ---------------------------------------------------------------------------
+// Block = 4 waves.  Each wave keeps R subtiles (32 permutation rows each) of
+// the operand resident in registers (hi and lo parts, KK k-steps of 16
+// subjects) and streams 32-voxel tiles of the data operand.  Per tile and
+// subtile: 2 or 3 MFMAs per k-step, then the statistic for the 16/PP
+// permutations each lane owns.  Running maxima stay in registers until the
+// end, then a 32-lane butterfly and one atomic per permutation per wave.
+struct Bf16Launch {
+    int n_mgroups, n_vchunks, vt_per_chunk, per_xcd, n_sub, n_vt;
+};
+
+template <int KK, int R, int PP, bool SIGN, bool FT>
+__global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int lb = xcd_logical_block(L.per_xcd);
+    if (lb >= L.n_mgroups * L.n_vchunks) return;
+    const int mg = lb % L.n_mgroups;
+    const int vc = lb / L.n_mgroups;
+    const int sub0 = (mg * 4 + wave) * R;
+    constexpr int PER = 16 / PP;  // permutations per lane per subtile
+
+    const bf16x8* G = reinterpret_cast<const bf16x8*>(a.Gb);
+    const bf16x8* Y = reinterpret_cast<const bf16x8*>(a.Yb);
+
+    bf16x8 ahi[R][KK], alo[R][KK];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int st = sub0 + r;
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) {
+            bf16x8 h = {}, l = {};
+            if (st < L.n_sub) {
+                const size_t base = (((size_t)st * KK + kk) * 2) * 64 + lane;
+                h = G[base];
+                if (!SIGN) l = G[base + 64];
+            }
+            ahi[r][kk] = h;
+            alo[r][kk] = l;
+        }
+    }
+    float runmax[R][PER];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int P = 0; P < PER; P++) runmax[r][P] = -INFINITY;
+
+    const int hl = lane >> 5;
+    const int vt0 = vc * L.vt_per_chunk;
+    const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
+    for (int vt = vt0; vt < vt1; vt++) {
+        bf16x8 bhi[KK], blo[KK];
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) {
+            const size_t base = (((size_t)vt * KK + kk) * 2) * 64 + lane;
+            bhi[kk] = Y[base];
+            blo[kk] = Y[base + 64];
+        }
+        const int v = vt * 32 + (lane & 31);
+        const double yyv = a.yy[v];
+        const bool valid = v < a.Vc;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            f32x16 acc;
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < KK; kk++) {
+                if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], bhi[kk], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], blo[kk], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bhi[kk], acc, 0, 0, 0);
+            }
+            const int kbase = (sub0 + r) * (32 / PP) + hl * PER;
+#pragma unroll
+            for (int P = 0; P < PER; P++) {
+                float ss = 0.0f, num = 0.0f;
+#pragma unroll
+                for (int g = 0; g < PP; g++) {
+                    const float z = acc[P * PP + g];
+                    ss = fmaf(z, z, ss);
+                    if (FT) {
+                        if (g < a.r) num = fmaf(z, z, num);
+                    }
+                }
+                float st = FT ? stat_f(num, ss, yyv, a.f_scale) : stat_t(acc[P * PP], ss, yyv, a.sqrt_dof);
+                if (!valid) st = -INFINITY;
+                runmax[r][P] = fmaxf(runmax[r][P], st);
+                const int k = kbase + P;
+                if (a.stats && valid && k < a.K) a.stats[(size_t)k * a.Vc + v] = st;
+                if (a.bits) {
+                    const unsigned long long bal = __ballot(valid && st > a.cdt);
+                    if ((lane & 31) == 0 && k < a.K)
+                        a.bits[(size_t)k * a.nwords + vt] = hl ? (unsigned)(bal >> 32) : (unsigned)bal;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int P = 0; P < PER; P++) {
+            float m = runmax[r][P];
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 8, 64));
+            m = fmaxf(m, __shfl_xor(m, 4, 64));
+            m = fmaxf(m, __shfl_xor(m, 2, 64));
+            m = fmaxf(m, __shfl_xor(m, 1, 64));
+            const int k = (sub0 + r) * (32 / PP) + hl * PER + P;
+            if ((lane & 31) == 0 && k < a.K && m > -INFINITY) atomicMax(&a.maxbits[k], enc_max(m));
+        }
+}
+
+namespace {
+
+template <int KK, int R, int PP, bool SIGN, bool FT>
+hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
+    const int total = L.n_mgroups * L.n_vchunks;
+    const int blocks = L.per_xcd * 8;
+    (void)total;
+    hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT>), dim3(blocks), dim3(256), 0, s, a, L);
+    return hipGetLastError();
+}
+
+template <int KK, int PP>
+hipError_t dispatch_bf16_kind(const ContractArgs& a, Bf16Launch L, bool sign, hipStream_t s) {
+    // resident operand subtiles per wave: keep hi+lo fragments within ~128 VGPRs
+    constexpr int R = (KK <= 2) ? 4 : (KK <= 4 ? 2 : 1);
+    L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
+    // voxel chunks: enough blocks to cover the 256 CUs several times
+    int target_blocks = 2048;
+    int nvc = (target_blocks + L.n_mgroups - 1) / L.n_mgroups;
+    if (nvc > L.n_vt) nvc = L.n_vt;
+    if (nvc < 1) nvc = 1;
+    L.vt_per_chunk = (L.n_vt + nvc - 1) / nvc;
+    L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
+    L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
+    if (sign) {
+        if (a.ftest || PP != 1) return hipErrorInvalidValue;
+        if constexpr (PP == 1) return run_bf16<KK, R, 1, true, false>(a, L, s);
+    }
+    return a.ftest ? run_bf16<KK, R, PP, false, true>(a, L, s) : run_bf16<KK, R, PP, false, false>(a, L, s);
+}
+
+template <int KK>
+hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s) {
+    switch (PP) {
+        case 1: return dispatch_bf16_kind<KK, 1>(a, L, sign, s);
+        case 2: return dispatch_bf16_kind<KK, 2>(a, L, sign, s);
+        case 4: return dispatch_bf16_kind<KK, 4>(a, L, sign, s);
+        case 8: return dispatch_bf16_kind<KK, 8>(a, L, sign, s);
+        case 16: return dispatch_bf16_kind<KK, 16>(a, L, sign, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign_operand,
+                                hipStream_t s, int* launches) {
+    Bf16Launch L{};
+    L.n_sub = (a.K * PP + 31) / 32;
+    L.n_vt = a.Vpad / 32;
+    if (launches) *launches = 1;
+    switch (KK) {
+        case 1: return dispatch_bf16_pp<1>(a, L, PP, sign_operand, s);
+        case 2: return dispatch_bf16_pp<2>(a, L, PP, sign_operand, s);
+        case 3: return dispatch_bf16_pp<3>(a, L, PP, sign_operand, s);
+        case 4: return dispatch_bf16_pp<4>(a, L, PP, sign_operand, s);
+        case 5: return dispatch_bf16_pp<5>(a, L, PP, sign_operand, s);
+        case 6: return dispatch_bf16_pp<6>(a, L, PP, sign_operand, s);
+        case 7: return dispatch_bf16_pp<7>(a, L, PP, sign_operand, s);
+        case 8: return dispatch_bf16_pp<8>(a, L, PP, sign_operand, s);
+        case 9: return dispatch_bf16_pp<9>(a, L, PP, sign_operand, s);
+        case 10: return dispatch_bf16_pp<10>(a, L, PP, sign_operand, s);
+        case 11: return dispatch_bf16_pp<11>(a, L, PP, sign_operand, s);
+        case 12: return dispatch_bf16_pp<12>(a, L, PP, sign_operand, s);
+        case 13: return dispatch_bf16_pp<13>(a, L, PP, sign_operand, s);
+        case 14: return dispatch_bf16_pp<14>(a, L, PP, sign_operand, s);
+        case 15: return dispatch_bf16_pp<15>(a, L, PP, sign_operand, s);
+        case 16: return dispatch_bf16_pp<16>(a, L, PP, sign_operand, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// plain FP32 contraction (vector ALUs)
+// ---------------------------------------------------------------------------
+// Block = 256 threads, one voxel per thread per tile, a tile of PTT = 64/P
+// permutations.  The block's operand rows sit in LDS as [s][k][i] so the
+// PTT*P values of one subject are read as broadcasts; each thread streams its
+// voxel's data one subject at a time and keeps PTT*P FP32 accumulators.
+struct F32Launch {
+    int n_pg, n_vchunks, vox_per_chunk, per_xcd;
+};
+
+template <int P>
+struct F32Tile {
+    static constexpr int PTT = (64 / P) < 1 ? 1 : (64 / P);
+    static constexpr int ROW = PTT * P;                   // floats per subject
+    static constexpr int ROW4 = (ROW + 3) / 4 * 4;        // padded to float4
+};
+
+template <int P, bool FT>
+__global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch L) {
+    using T = F32Tile<P>;
+    constexpr int PTT = T::PTT;
+    extern __shared__ float sG[];  // [N][ROW4]
+    __shared__ unsigned sMax[PTT];
+    const int tid = threadIdx.x;
+    const int lb = xcd_logical_block(L.per_xcd);
+    if (lb >= L.n_pg * L.n_vchunks) return;
+    const int pg = lb % L.n_pg;
+    const int vc = lb / L.n_pg;
+    const int k0 = pg * PTT;
+    const int N = a.N;
+    // G32 is [k][i][NS]; gather this block's rows into [s][k][i]
+    for (int t = tid; t < N * T::ROW4; t += 256) {
+        const int s = t / T::ROW4, ki = t % T::ROW4;
+        float val = 0.0f;
+        if (ki < T::ROW) {
+            const int k = ki / P, i = ki % P;
+            val = a.G32[((size_t)(k0 + k) * P + i) * a.NS + s];
+        }
+        sG[t] = val;
+    }
+    if (tid < PTT) sMax[tid] = 0u;
+    __syncthreads();
+    const int v0 = vc * L.vox_per_chunk;
+    const int v1 = min(v0 + L.vox_per_chunk, a.Vpad);
+    const int lane = tid & 63;
+    for (int vb = v0; vb < v1; vb += 256) {
+        const int v = vb + tid;
+        const bool inb = v < a.Vpad;
+        const bool valid = v < a.Vc;
+        float z[T::ROW];
+#pragma unroll
+        for (int q = 0; q < T::ROW; q++) z[q] = 0.0f;
+        const float* yp = a.Y32 + (inb ? v : 0);
+        for (int s = 0; s < N; s++) {
+            const float ys = inb ? yp[(size_t)s * a.Vpad] : 0.0f;
+            const float4* g4 = reinterpret_cast<const float4*>(sG + s * T::ROW4);
+#pragma unroll
+            for (int q4 = 0; q4 < T::ROW4 / 4; q4++) {
+                const float4 g = g4[q4];
+                if (4 * q4 + 0 < T::ROW) z[4 * q4 + 0] = fmaf(g.x, ys, z[4 * q4 + 0]);
+                if (4 * q4 + 1 < T::ROW) z[4 * q4 + 1] = fmaf(g.y, ys, z[4 * q4 + 1]);
+                if (4 * q4 + 2 < T::ROW) z[4 * q4 + 2] = fmaf(g.z, ys, z[4 * q4 + 2]);
+                if (4 * q4 + 3 < T::ROW) z[4 * q4 + 3] = fmaf(g.w, ys, z[4 * q4 + 3]);
+            }
+        }
+        const double yyv = inb ? a.yy[v] : 1.0;
+#pragma unroll
+        for (int k = 0; k < PTT; k++) {
+            float ss = 0.0f, num = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                const float zi = z[k * P + i];
+                ss = fmaf(zi, zi, ss);
+                if (FT) {
+                    if (i < a.r) num = fmaf(zi, zi, num);
+                }
+            }
+            float st = FT ? stat_f(num, ss, yyv, a.f_scale) : stat_t(z[k * P], ss, yyv, a.sqrt_dof);
+            if (!valid) st = -INFINITY;
+            const int kk = k0 + k;
+            if (a.stats && valid && kk < a.K) a.stats[(size_t)kk * a.Vc + v] = st;
+            if (a.bits) {
+                const unsigned long long bal = __ballot(valid && st > a.cdt);
+                if (lane == 0 && kk < a.K) {
+                    const int w0 = (vb + (tid & ~63)) >> 5;
+                    if (w0 < a.nwords) a.bits[(size_t)kk * a.nwords + w0] = (unsigned)bal;
+                    if (w0 + 1 < a.nwords) a.bits[(size_t)kk * a.nwords + w0 + 1] = (unsigned)(bal >> 32);
+                }
+            }
+            float m = st;
+            m = fmaxf(m, __shfl_xor(m, 32, 64));
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 8, 64));
+            m = fmaxf(m, __shfl_xor(m, 4, 64));
+            m = fmaxf(m, __shfl_xor(m, 2, 64));
+            m = fmaxf(m, __shfl_xor(m, 1, 64));
+            if (lane == 0 && m > -INFINITY) atomicMax(&sMax[k], enc_max(m));
+        }
+    }
+    __syncthreads();
+    if (tid < PTT && k0 + tid < a.K && sMax[tid] != 0u) atomicMax(&a.maxbits[k0 + tid], sMax[tid]);
+}
+
+namespace {
+
+template <int P, bool FT>
+hipError_t run_f32(const ContractArgs& a, hipStream_t s) {
+    using T = F32Tile<P>;
+    F32Launch L{};
+    L.n_pg = (a.K + T::PTT - 1) / T::PTT;
+    const int nvsteps = (a.Vpad + 255) / 256;
+    int nvc = (2048 + L.n_pg - 1) / L.n_pg;
+    if (nvc > nvsteps) nvc = nvsteps;
+    if (nvc < 1) nvc = 1;
+    const int steps_per = (nvsteps + nvc - 1) / nvc;
+    L.vox_per_chunk = steps_per * 256;
+    L.n_vchunks = (nvsteps + steps_per - 1) / steps_per;
+    L.per_xcd = (L.n_pg * L.n_vchunks + 7) / 8;
+    const size_t shmem = (size_t)a.N * T::ROW4 * sizeof(float);
+    hipLaunchKernelGGL((k_contract_f32<P, FT>), dim3(L.per_xcd * 8), dim3(256), shmem, s, a, L);
+    return hipGetLastError();
+}
+
+template <bool FT>
+hipError_t dispatch_f32(const ContractArgs& a, hipStream_t s) {
+    switch (a.pfit) {
+#define BPG_P(n) case n: return run_f32<n, FT>(a, s);
+        BPG_P(1) BPG_P(2) BPG_P(3) BPG_P(4) BPG_P(5) BPG_P(6) BPG_P(7) BPG_P(8) BPG_P(9)
+        BPG_P(10) BPG_P(11) BPG_P(12) BPG_P(13) BPG_P(14) BPG_P(15) BPG_P(16) BPG_P(17)
+        BPG_P(18) BPG_P(19) BPG_P(20) BPG_P(21) BPG_P(22) BPG_P(23) BPG_P(24) BPG_P(25)
+#undef BPG_P
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+// permutations per block of the FP32 form
+int f32_perm_tile(int pfit, int /*NS*/) { return (64 / pfit) < 1 ? 1 : (64 / pfit); }
+
+hipError_t launch_contract_f32(const ContractArgs& a, int NS, hipStream_t s, int* launches) {
+    (void)NS;
+    if (launches) *launches = 1;
+    return a.ftest ? dispatch_f32<true>(a, s) : dispatch_f32<false>(a, s);
+}
+
+// ---------------------------------------------------------------------------
+// observed maps (FP64), CalculateBetaWeightsGLM + CalculateStatisticalMaps
+// GLM{T,F}Test: beta = pinv(X) y, e = y - X beta; t: sigma^2 = sum e^2 /
+// (N - p); F: sigma^2 = sum (e - mean e)^2 / (N - p - 1).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_observed(ObsArgs a) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.Vc) return;
+    const int N = a.N, p = a.p;
+    double beta[BPG_MAX_REGRESSORS];
+#pragma unroll
+    for (int r = 0; r < BPG_MAX_REGRESSORS; r++) beta[r] = 0.0;
+    for (int s = 0; s < N; s++) {
+        const double ys = (double)a.Yc[(size_t)s * a.Vc + v];
+#pragma unroll
+        for (int r = 0; r < BPG_MAX_REGRESSORS; r++)
+            if (r < p) beta[r] += a.Xp[r + (size_t)s * p] * ys;
+    }
+    double se = 0.0, se2 = 0.0;
+    for (int s = 0; s < N; s++) {
+        double e = (double)a.Yc[(size_t)s * a.Vc + v];
+#pragma unroll
+        for (int r = 0; r < BPG_MAX_REGRESSORS; r++)
+            if (r < p) e -= a.X[s + (size_t)r * N] * beta[r];
+        se += e;
+        se2 += e * e;
+    }
+    if (!a.ftest) {
+        const double var = se2 / (double)(N - p);
+        for (int c = 0; c < a.nmaps; c++) {
+            double cv = 0.0;
+            for (int r = 0; r < p; r++) cv += a.C[r + (size_t)c * p] * beta[r];
+            a.out[(size_t)c * a.Vc + v] = (float)(cv / sqrt(var * a.sc[c]));
+        }
+    } else {
+        const double mean = se / (double)N;
+        const double var = (se2 - (double)N * mean * mean) / (double)(N - p - 1);
+        double cb[BPG_MAX_FROWS];
+        for (int c = 0; c < a.nc; c++) {
+            double acc = 0.0;
+            for (int r = 0; r < p; r++) acc += a.C[r + (size_t)c * p] * beta[r];
+            cb[c] = acc;
+        }
+        double q = 0.0;
+        for (int c = 0; c < a.nc; c++)
+            for (int cc = 0; cc < a.nc; cc++) q += cb[c] * a.sc[cc + (size_t)c * a.nc] * cb[cc];
+        a.out[v] = (float)(q / var / (double)a.nc);
+    }
+}
+
+hipError_t launch_observed(const ObsArgs& a, hipStream_t s) {
+    if (a.Vc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_observed, dim3((a.Vc + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// p-values: #{k : T > m_k} / K by binary search in the sorted null
+// distribution (KC:516-652 count the same set one by one).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pvalues(PvalArgs a) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.Vc) return;
+    float T = a.obs[v];
+    if (a.inference != BPG_VOXEL) {
+        if (!(T > a.cdt)) { a.P[v] = 0.0f; return; }
+        const unsigned g = a.comp2grid[v];
+        T = (float)a.sizes[a.labels[g]];
+        if (a.inference == BPG_CLUSTER_MASS) T = T / 10000.0f;
+    }
+    int lo = 0, hi = a.K;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a.sorted_max[mid] < T) lo = mid + 1;
+        else hi = mid;
+    }
+    a.P[v] = (float)lo / (float)a.K;
+}
+
+hipError_t launch_pvalues(const PvalArgs& a, hipStream_t s) {
+    if (a.Vc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pvalues, dim3((a.Vc + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// connected components (union-find on grid indices, min-index roots)
+// ---------------------------------------------------------------------------
+// The reference propagates the minimum label among neighbours until nothing
+// changes (KC:42-401); its fixpoint labels every component with its minimum
+// linear index.  Here each supra-threshold voxel is linked to its
+// lower-index neighbours with atomicMin unions, so every tree is rooted at
+// its minimum index and the labels are the same.
+__device__ __forceinline__ unsigned uf_find(const unsigned* p, unsigned x) {
+    unsigned px = p[x];
+    while (px != x) {
+        x = px;
+        px = p[x];
+    }
+    return x;
+}
+
+__device__ __forceinline__ void uf_union(unsigned* p, unsigned a, unsigned b) {
+    for (;;) {
+        a = uf_find(p, a);
+        b = uf_find(p, b);
+        if (a == b) return;
+        if (a < b) {
+            const unsigned t = a;
+            a = b;
+            b = t;
+        }
+        const unsigned old = atomicMin(&p[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)a.nmaps * a.nwords) return;
+    const int map = (int)(t / a.nwords);
+    const int w = (int)(t % a.nwords);
+    const size_t G = (size_t)a.W * a.H * a.D;
+    unsigned* parent = a.parent + (size_t)map * G;
+    unsigned* sizes = a.sizes + (size_t)map * G;
+    const unsigned* bits = a.bits + (size_t)map * a.nwords;
+    unsigned word = bits[w];
+    while (word) {
+        const int b = __ffs(word) - 1;
+        word &= word - 1;
+        const int c = w * 32 + b;
+        if (c >= a.Vc) break;
+        const unsigned g = a.comp2grid[c];
+        if (stage == 0) {
+            parent[g] = g;
+            sizes[g] = 0u;
+        } else if (stage == 1) {
+            const int x = (int)(g % a.W);
+            const int y = (int)((g / a.W) % a.H);
+            const int z = (int)(g / ((size_t)a.W * a.H));
+            for (int dz = -1; dz <= 0; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        // lower-index neighbours only: each pair is linked once
+                        if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
+                        const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+                        if (a.connectivity == 6 && nz != 1) continue;
+                        const int xx = x + dx, yy = y + dy, zz = z + dz;
+                        if (xx < 0 || yy < 0 || zz < 0 || xx >= a.W || yy >= a.H) continue;
+                        const unsigned ng = (unsigned)(xx + (size_t)yy * a.W + (size_t)zz * a.W * a.H);
+                        const int nc = a.grid2comp[ng];
+                        if (nc < 0) continue;
+                        if (!((bits[nc >> 5] >> (nc & 31)) & 1u)) continue;
+                        uf_union(parent, g, ng);
+                    }
+        } else if (stage == 2) {
+            const unsigned root = uf_find(parent, g);
+            atomicAdd(&sizes[root], 1u);
+            if (a.labels && map == 0) a.labels[g] = root;
+        } else {
+            if (parent[g] == g) atomicMax(&a.maxsize[map], sizes[g]);
+        }
+    }
+}
+
+hipError_t launch_ccl(const CclArgs& a, hipStream_t s) {
+    const size_t total = (size_t)a.nmaps * a.nwords;
+    if (total == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    for (int stage = 0; stage < 4; stage++) {
+        hipLaunchKernelGGL(k_ccl, dim3(blocks), dim3(256), 0, s, a, stage);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+__global__ __launch_bounds__(256) void k_threshold_bits(const float* obs, int Vc, float cdt,
+                                                        unsigned* bits, int nwords) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = v < Vc && obs[v] > cdt;
+    const unsigned long long bal = __ballot(act);
+    const int lane = threadIdx.x & 63;
+    const int w0 = (v - lane) >> 5;
+    if (lane == 0) {
+        if (w0 < nwords) bits[w0] = (unsigned)bal;
+        if (w0 + 1 < nwords) bits[w0 + 1] = (unsigned)(bal >> 32);
+    }
+}
+
+hipError_t launch_threshold_bits(const float* obs, int Vc, float cdt, unsigned* bits, int nwords,
+                                 hipStream_t s) {
+    const int n = nwords * 32;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_threshold_bits, dim3((n + 255) / 256), dim3(256), 0, s, obs, Vc, cdt, bits,
+                       nwords);
+    return hipGetLastError();
+}
+
+}  // namespace bpg

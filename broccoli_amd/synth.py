@@ -1,0 +1,127 @@
+"""Synthetic second-level studies with the shapes of the benchmark workloads.
+
+Grids are (D, H, W) arrays flattened as x + y*W + z*W*H; data are
+(N, D, H, W), i.e. Y[x + y*W + z*W*H + s*W*H*D] as the reference expects.
+Randomness comes from numpy's PCG64 (numpy.random.default_rng) seeded with
+20260000 + workload number, with independent child streams for data,
+covariates and permutations.  A `scale` < 1 shrinks the grid and the mask
+(same shapes, fewer voxels) for quick parity runs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi
+
+GRID_2MM = (91, 109, 91)
+GRID_1MM = (182, 218, 182)
+
+
+def ellipsoid_mask(W, H, D, center, semi) -> np.ndarray:
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    r = ((x - center[0]) / semi[0]) ** 2 + ((y - center[1]) / semi[1]) ** 2 + ((z - center[2]) / semi[2]) ** 2
+    return (r <= 1.0).astype(np.float32)
+
+
+def sphere(W, H, D, center, radius) -> np.ndarray:
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    return ((x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2) <= radius ** 2
+
+
+def smooth_noise(rng, N, W, H, D, fwhm=3.0) -> np.ndarray:
+    """Gaussian-smoothed white noise (separable, truncated at 4 sigma), scaled
+    by the analytic factor that gives unit variance away from the borders."""
+    from scipy.ndimage import correlate1d
+
+    sigma = fwhm / 2.3548
+    rad = int(np.ceil(4 * sigma))
+    k = np.exp(-0.5 * (np.arange(-rad, rad + 1) / sigma) ** 2)
+    k /= k.sum()
+    norm = np.sqrt(np.sum(k ** 2)) ** 3
+    out = np.empty((N, D, H, W), np.float32)
+    for s in range(N):
+        v = rng.standard_normal((D, H, W))
+        for ax in range(3):
+            v = correlate1d(v, k, axis=ax, mode="constant")
+        out[s] = (v / norm).astype(np.float32)
+    return out
+
+
+def _scaled(grid, scale):
+    return tuple(max(8, int(round(g * scale))) for g in grid)
+
+
+def study(workload: int, scale: float = 1.0, seed: int | None = None) -> dict:
+    """Synthetic study for workloads 1, 2, 3 and 5 (see the module doc)."""
+    seed = 20260000 + workload if seed is None else seed
+    ss = np.random.SeedSequence(seed)
+    r_data, r_cov, r_perm = (np.random.default_rng(c) for c in ss.spawn(3))
+    if workload == 5:
+        W, H, D = _scaled(GRID_1MM, scale)
+        f = W / GRID_1MM[0]
+        center, semi = (90.5 * f, 108.5 * f, 90.5 * f), (72 * f, 88 * f, 68.8 * f)
+    else:
+        W, H, D = _scaled(GRID_2MM, scale)
+        f = W / GRID_2MM[0]
+        center, semi = (45 * f, 54 * f, 45 * f), (36 * f, 44 * f, 34.4 * f)
+    mask = ellipsoid_mask(W, H, D, center, semi)
+    st = dict(W=W, H=H, D=D, mask=mask, workload=workload, seed=seed, perm_rng=r_perm)
+    if workload == 1:
+        # one-sample t by sign flipping, N=20, intercept only
+        N = 20
+        Y = r_data.standard_normal((N, D, H, W), dtype=np.float32)
+        Y[:, sphere(W, H, D, (60 * f, 54 * f, 45 * f), 5 * f)] += 0.5
+        st.update(N=N, p=1, nc=1, test=capi.GROUP_MEAN, X=np.ones((N, 1), np.float32),
+                  C=np.ones((1, 1), np.float32), Y=Y, K=1000)
+    elif workload == 2:
+        # two-sample GLM with covariates, N=60 (30/30), t contrast [0 1 0 0]
+        N = 60
+        g = np.r_[np.ones(30), np.zeros(30)]
+        age = r_cov.uniform(20, 70, N)
+        age -= age.mean()
+        sex = (r_cov.random(N) < 0.5).astype(np.float64)
+        X = np.c_[np.ones(N), g, age, sex].astype(np.float32)
+        Y = r_data.standard_normal((N, D, H, W), dtype=np.float32)
+        eff = sphere(W, H, D, (30 * f, 54 * f, 45 * f), 5 * f) | sphere(W, H, D, (60 * f, 54 * f, 45 * f), 5 * f)
+        Y[:30][:, eff] += 0.4
+        st.update(N=N, p=4, nc=1, test=capi.TTEST, X=X, C=np.array([[0, 1, 0, 0]], np.float32), Y=Y, K=10000)
+    elif workload == 3:
+        # F test over 3 group dummies (4 groups x 25), 2 covariates, smooth noise
+        N = 100
+        grp = np.repeat(np.arange(4), 25)
+        dum = np.stack([(grp == j).astype(np.float64) for j in (1, 2, 3)], 1)
+        cov = r_cov.standard_normal((N, 2))
+        X = np.c_[np.ones(N), dum, cov].astype(np.float32)
+        C = np.zeros((3, 6), np.float32)
+        C[0, 1] = C[1, 2] = C[2, 3] = 1
+        Y = smooth_noise(r_data, N, W, H, D)
+        st.update(N=N, p=6, nc=3, test=capi.FTEST, X=X, C=C, Y=Y, K=10000,
+                  inference=capi.CLUSTER_EXTENT, cdt=4.0)
+    elif workload == 5:
+        N = 200
+        grp = np.repeat(np.arange(4), 50)
+        dum = np.stack([(grp == j).astype(np.float64) for j in (1, 2, 3)], 1)
+        cov = r_cov.standard_normal((N, 4))
+        X = np.c_[np.ones(N), dum, cov].astype(np.float32)
+        C = np.zeros((1, 8), np.float32)
+        C[0, 4] = 1
+        Y = r_data.standard_normal((N, D, H, W), dtype=np.float32)
+        st.update(N=N, p=8, nc=1, test=capi.TTEST, X=X, C=C, Y=Y, K=10000)
+    else:
+        raise ValueError(f"unknown workload {workload}")
+    st["V"] = int(mask.sum())
+    return st
+
+
+def permutations(st: dict, K: int, compat: bool = False, seed: int | None = None):
+    """Permutation (t, F) or sign (group mean) matrix for a study, from the
+    library's generators, with the design detection of the CLI."""
+    if seed is None:
+        seed = int(st["perm_rng"].integers(1, 2**62))
+    N = st["N"]
+    if st["test"] == capi.GROUP_MEAN:
+        return dict(sign=capi.gen_sign_matrix(N, K, compat=compat, seed=seed))
+    designs, n1, n2 = capi.detect_designs(st["X"], st["C"], st["test"] == capi.FTEST)
+    if designs[0] == capi.TWOSAMPLE:
+        return dict(perm=capi.gen_perm_two_sample(N, int(n1[0]), K, compat=compat, seed=seed))
+    return dict(perm=capi.gen_perm_correlation(N, K, compat=compat, seed=seed))

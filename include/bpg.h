@@ -1,0 +1,168 @@
+/*
+ * bpg.h -- C ABI of the MI355X second-level permutation-test GLM.
+ *
+ * This is the thin layer between the BROCCOLI_LIB drop-in class
+ * (include/broccoli_lib.h) and the CDNA4 HIP kernels.  Plain pointers and
+ * sizes only; every function returns BPG_OK (0) or a negative status and
+ * never throws.  All host buffers are caller-owned; device memory is owned by
+ * the context / plan objects.
+ *
+ * Reference paths (BROCCOLI repository):
+ *   LIB = code/BROCCOLI_LIB/broccoli_lib.cpp, HDR = .../broccoli_lib.h
+ *   RGL = code/Bash_Wrapper/RandomiseGroupLevel.cpp
+ *   CONST = code/BROCCOLI_LIB/broccoli_constants.h
+ *
+ * Layouts are the reference's (HDR setters, RGL:1148, RGL:1184-1192):
+ *   Y[x + y*W + z*W*H + s*W*H*D]   mask[x + y*W + z*W*H] (in-mask iff 1.0f)
+ *   X[s + r*N]                     C[r + c*p]
+ *   perm[s + k*N] (0-based uint16) sign[s + k*N] (+-1.0f)
+ *   maps / p-values [x + y*W + z*W*H + c*W*H*D]
+ */
+#ifndef BPG_H
+#define BPG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BPG_OK 0
+#define BPG_ERR_ARG (-1)         /* invalid argument / shape                 */
+#define BPG_ERR_HIP (-2)         /* HIP runtime failure (see bpg_last_error) */
+#define BPG_ERR_NOMEM (-3)       /* host or device allocation failed         */
+#define BPG_ERR_DESIGN (-4)      /* rank-deficient design or contrast        */
+#define BPG_ERR_UNSUPPORTED (-5) /* valid input outside the built kernels    */
+#define BPG_ERR_NODEVICE (-6)    /* no HIP device with that index            */
+
+/* statistical tests, CONST:78-80 */
+#define BPG_TTEST 0
+#define BPG_FTEST 1
+#define BPG_GROUP_MEAN 2
+/* inference modes, CONST:51-53 */
+#define BPG_VOXEL 0
+#define BPG_CLUSTER_EXTENT 1
+#define BPG_CLUSTER_MASS 2
+/* group designs, CONST:82-83 (+ the CLI's mean design, RGL:972-976) */
+#define BPG_TWOSAMPLE 0
+#define BPG_CORRELATION 1
+#define BPG_MEAN_DESIGN 2
+/* arithmetic of the permutation contraction */
+#define BPG_PREC_FP32 0       /* FP32 VALU contraction                         */
+#define BPG_PREC_SPLIT_BF16 1 /* hi/lo BF16 MFMA with FP32 accumulation        */
+
+/* limits of the reference (RGL:630-636 p <= 25, RGL:726-732 F rows <= 10) */
+#define BPG_MAX_REGRESSORS 25
+#define BPG_MAX_FROWS 10
+#define BPG_MAX_SUBJECTS 256
+
+typedef struct bpg_ctx bpg_ctx;
+typedef struct bpg_plan bpg_plan;
+
+typedef struct {
+    int test;         /* BPG_TTEST / BPG_FTEST / BPG_GROUP_MEAN               */
+    int inference;    /* BPG_VOXEL / BPG_CLUSTER_EXTENT / BPG_CLUSTER_MASS    */
+    int precision;    /* BPG_PREC_*                                           */
+    int connectivity; /* 26 (the reference's) or 6                            */
+    int W, H, D;      /* grid                                                 */
+    int N;            /* subjects                                             */
+    int p;            /* regressors (group mean: 1)                           */
+    int nc;           /* t: contrasts; F: rows of the F contrast; mean: 1     */
+    const float* Y;    /* host, N*W*H*D                                       */
+    const float* mask; /* host, W*H*D                                         */
+    const float* X;    /* host, N*p  (group mean: the all-ones column)        */
+    const float* C;    /* host, p*nc (group mean: {1})                        */
+    float cdt;         /* cluster-defining threshold                          */
+} bpg_study_desc;
+
+/* Per-phase timings of the last bpg_plan_permutation_phase call, from HIP
+ * events on the plan's stream. */
+typedef struct {
+    double phase_ms;      /* first permutation launch -> max-T on host        */
+    double transform_ms;  /* nuisance residualisation + operand split         */
+    double operand_ms;    /* permutation operand build                        */
+    double contract_ms;   /* fused contraction + statistic + max-T kernel     */
+    double cluster_ms;    /* cluster labelling (cluster inference only)       */
+    int contract_launches;
+    long perm_voxels;     /* K * V_mask of the phase                          */
+} bpg_timing;
+
+/* ---- context: replaces BROCCOLI_LIB(platform, device, ...) +
+ *      OpenCLInitiate (LIB:151-158, LIB:1181-1420) and ~BROCCOLI_LIB
+ *      (LIB:160-163).  device = HIP ordinal. */
+int bpg_create(int device, bpg_ctx** out);
+void bpg_destroy(bpg_ctx* ctx);
+/* GetOpenCLError / GetOpenCLInitializationError (HDR:412-413) */
+const char* bpg_last_error(const bpg_ctx* ctx);
+/* GetOpenCLDeviceName / GetOpenCLPlatformName (HDR:391-394) */
+const char* bpg_device_name(const bpg_ctx* ctx);
+
+/* ---- plan: device-resident study.  Replaces the buffer creation and
+ *      uploads of Perform*SecondLevelPermutationWrapper (LIB:13148-13188,
+ *      LIB:13228-13261) and SetupPermutationTestSecondLevel (LIB:16128).
+ *      Only in-mask voxels are uploaded (compacted, subject-major). */
+int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_plan** out);
+void bpg_plan_destroy(bpg_plan* plan);
+long bpg_plan_num_voxels(const bpg_plan* plan); /* V_mask */
+
+/* ---- the permutation phase of one statistical map: nuisance
+ *      residualisation (LIB:16654-16727 + TransformData K1:1684), then all K
+ *      permutations (LIB:16733-16763: the per-permutation statistic kernels
+ *      K2:1458 / K2:1562 / K4:1943 and CalculateMaxAtomic KM:282, or
+ *      ClusterizeOpenCLPermutation LIB:18404) in fused launches.
+ *      map: contrast index (t) or 0.  perm (t, F) or sign (group mean): host,
+ *      [s + k*N].  maxdist: host, K floats, the null distribution as the
+ *      reference stores it (quantised max-T, cluster size, or mass / 1e4).
+ *      stats (optional, may be NULL): host, K*V_mask floats, the permuted
+ *      statistic of every in-mask voxel (compact order), for parity checks. */
+int bpg_plan_permutation_phase(bpg_plan* plan, int map, const unsigned short* perm,
+                               const float* sign, long K, float* maxdist, float* stats);
+int bpg_plan_last_timing(const bpg_plan* plan, bpg_timing* out);
+
+/* ---- observed statistical maps on the original data and design
+ *      (CalculateStatisticalMapsGLM{T,F}TestSecondLevel, LIB:15858-15950;
+ *      K1:51, K1:1456, K1:1560).  maps: host grid, nmaps*W*H*D. */
+int bpg_plan_observed_maps(bpg_plan* plan, float* maps);
+
+/* ---- permutation p-values (CalculatePermutationPValues, LIB:16794-16873,
+ *      KC:516-652), on the observed maps of the last bpg_plan_observed_maps
+ *      call.  K[c], maxdist[c][k] per map.  pvalues: host grid,
+ *      nmaps*W*H*D.  labels (optional): host grid, uint32 cluster labels of
+ *      the observed maps (min linear index of the component, 3*W*H*D
+ *      outside clusters), nmaps*W*H*D. */
+int bpg_plan_pvalues(bpg_plan* plan, const long* K, const float* const* maxdist,
+                     float* pvalues, unsigned* labels);
+
+/* ---- host-side pieces of the path -------------------------------------- */
+
+/* Generators (LIB:17069-17122, LIB:16925-17011, LIB:17014-17066).
+ * compat != 0: the reference's stream -- the process-global C rand(),
+ *   libstdc++'s std::random_shuffle order, uniqueness rejection; row 0 is
+ *   the identity / all +1.  Same output as the reference for the same rand()
+ *   state.
+ * compat == 0: a seeded SplitMix64 stream with Fisher-Yates shuffles and the
+ *   same uniqueness rule, independent of global state.
+ * Returns BPG_ERR_ARG when K exceeds the number of distinct vectors. */
+int bpg_gen_sign_matrix(int N, long K, int compat, unsigned long long seed, float* sign);
+int bpg_gen_perm_two_sample(int N, int n1, int n2, long K, int compat,
+                            unsigned long long seed, unsigned short* perm);
+int bpg_gen_perm_correlation(int N, long K, int compat, unsigned long long seed,
+                             unsigned short* perm);
+
+/* Design-type detection (RGL:908-1044): designs[c] in BPG_TWOSAMPLE /
+ * BPG_CORRELATION / BPG_MEAN_DESIGN, n1/n2 group sizes for two-sample
+ * designs.  F test: one entry from X*C'. */
+int bpg_detect_designs(int N, int p, int nc, const float* X, const float* C,
+                       int ftest, int* designs, int* n1, int* n2);
+/* Largest permutation count (RGL:38-42, RGL:1048-1134). */
+double bpg_max_permutations(int test, int design, int N, int n2);
+/* FP64 model precompute of the CLI (RGL:1137-1194), cast to float:
+ * xtxxt[s + r*N] = pinv(X)(r,s); ctxtxc[c] (t) or [c + cc*nc] (F). */
+int bpg_model_precompute(int N, int p, int nc, int ftest, const float* X,
+                         const float* C, float* xtxxt, float* ctxtxc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

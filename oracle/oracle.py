@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
+checker) and bench.py's cpu_baseline leg.  The product package never imports
+it.  The functions restate the reference path (see bpg_oracle.c for the
+reference lines of each).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_long, c_uint, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libbroccoli_ref.so"
+
+REF32, F64 = 0, 1
+
+
+def build(ref: bool = False) -> None:
+    """Compiles liboracle.so (and the reference harness when asked and the
+    reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    if ref and Path("/root/reference/code/BROCCOLI_LIB/broccoli_lib.cpp").exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+
+
+class Study(ctypes.Structure):
+    _fields_ = [("test", c_int), ("inference", c_int), ("W", c_int), ("H", c_int), ("D", c_int),
+                ("N", c_int), ("p", c_int), ("nc", c_int), ("Y", c_void_p), ("mask", c_void_p),
+                ("X", c_void_p), ("xtxxt", c_void_p), ("C", c_void_p), ("ctxtxc", c_void_p),
+                ("K", c_void_p), ("perm", c_void_p), ("sign", c_void_p), ("cdt", c_float),
+                ("connectivity", c_int), ("nthreads", c_int)]
+
+
+_lib = None
+_ref = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB))
+        _lib.orc_srand.argtypes = [c_uint]
+        _lib.orc_max_quantized.restype = c_float
+        _lib.orc_max_quantized.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p]
+        _lib.orc_quantize.argtypes = [c_float]
+        _lib.orc_ccl.restype = c_uint
+        _lib.orc_ccl.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_float, c_int, c_void_p, c_void_p]
+        _lib.orc_stirling_factorial.restype = c_double
+        _lib.orc_max_permutations.restype = c_long
+        _lib.orc_pvalues.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_float, c_long, c_void_p, c_void_p]
+    return _lib
+
+
+def ref_lib():
+    """The reference's own CPU routines (oracle/_ref), or None when not built."""
+    global _ref
+    if _ref is None and REF_LIB.exists():
+        _ref = ctypes.CDLL(str(REF_LIB))
+        _ref.ref_clusterize.argtypes = [c_int, c_int, c_int, c_void_p, c_float, c_void_p, c_void_p,
+                                        c_void_p, c_void_p]
+    return _ref
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+# ------------------------------------------------------------- generators
+
+def gen_sign_matrix(N, K, seed=1):
+    out = np.zeros((K, N), np.float32)
+    lib().orc_srand(seed)
+    lib().orc_gen_sign_matrix(N, c_long(K), P(out))
+    return out
+
+
+def gen_perm_two_sample(N, n1, K, seed=1):
+    out = np.zeros((K, N), np.uint16)
+    lib().orc_srand(seed)
+    lib().orc_gen_perm_two_sample(N, n1, N - n1, c_long(K), P(out))
+    return out
+
+
+def gen_perm_correlation(N, K, seed=1):
+    out = np.zeros((K, N), np.uint16)
+    lib().orc_srand(seed)
+    lib().orc_gen_perm_correlation(N, c_long(K), P(out))
+    return out
+
+
+def detect_designs(X, C, ftest):
+    N, p = X.shape
+    nc = C.shape[0]
+    Xf = np.asfortranarray(X, np.float32).ravel(order="F").copy()
+    Cc = np.ascontiguousarray(C, np.float32).ravel()
+    d, n1, n2 = (np.zeros(nc, np.int32) for _ in range(3))
+    lib().orc_detect_designs(N, p, nc, P(Xf), P(Cc), int(ftest), P(d), P(n1), P(n2))
+    g = 1 if ftest else nc
+    return d[:g], n1[:g], n2[:g]
+
+
+def model_precompute(X, C, ftest):
+    N, p = X.shape
+    nc = C.shape[0]
+    Xf = np.asfortranarray(X, np.float32).ravel(order="F").copy()
+    Cc = np.ascontiguousarray(C, np.float32).ravel()
+    xtxxt = np.zeros(N * p, np.float32)
+    ctxtxc = np.zeros(nc * nc if ftest else nc, np.float32)
+    rc = lib().orc_model_precompute(N, p, nc, int(ftest), P(Xf), P(Cc), P(xtxxt), P(ctxtxc))
+    assert rc == 0
+    return xtxxt, ctxtxc
+
+
+def pinv64(X):
+    N, p = X.shape
+    Xf = np.asfortranarray(X, np.float64).ravel(order="F").copy()
+    out = np.zeros(N * p)
+    lib().orc_pinv64(N, p, P(Xf), P(out))
+    return out.reshape(p, N)  # out[s + r*N] -> [r][s]
+
+
+def nuisance_projector64(X, Crows):
+    N, p = X.shape
+    r = Crows.shape[0]
+    Xf = np.asfortranarray(X, np.float64).ravel(order="F").copy()
+    Cr = np.ascontiguousarray(Crows, np.float64).ravel()
+    R = np.zeros(N * N)
+    lib().orc_nuisance_projector64(N, p, P(Xf), r, P(Cr), P(R))
+    return R.reshape(N, N).T  # column-major -> [i][j]
+
+
+def ccl(stat, mask, thr, connectivity=26):
+    D, H, W = mask.shape
+    lab = np.zeros(mask.size, np.uint32)
+    sz = np.zeros(mask.size, np.uint32)
+    st = np.ascontiguousarray(stat, np.float32)
+    mk = np.ascontiguousarray(mask, np.float32)
+    best = lib().orc_ccl(W, H, D, P(st), P(mk), c_float(thr), connectivity, P(lab), P(sz))
+    return lab, sz, int(best)
+
+
+def max_quantized(stat, mask):
+    D, H, W = mask.shape
+    return lib().orc_max_quantized(W, H, D, P(np.ascontiguousarray(stat, np.float32)),
+                                   P(np.ascontiguousarray(mask, np.float32)))
+
+
+# ------------------------------------------------------------- studies
+
+class _Bound:
+    """Keeps the arrays of an orc_study alive."""
+
+    def __init__(self, st: dict, perms: dict, K, nthreads=0):
+        N, p, nc = st["N"], st["p"], st["nc"]
+        ftest = st["test"] == 1
+        self.Y = np.ascontiguousarray(st["Y"], np.float32)
+        self.mask = np.ascontiguousarray(st["mask"], np.float32)
+        self.X = np.asfortranarray(st["X"], np.float32).ravel(order="F").copy()
+        self.C = np.ascontiguousarray(st["C"], np.float32).ravel()
+        if "xtxxt" in st:
+            self.xtxxt, self.ctxtxc = st["xtxxt"], st["ctxtxc"]
+        else:
+            self.xtxxt, self.ctxtxc = model_precompute(st["X"], st["C"], ftest)
+        nmaps = nc if st["test"] == 0 else 1
+        Ks = K if isinstance(K, (list, tuple, np.ndarray)) else [K] * nmaps
+        self.K = np.array(Ks, np.int64)
+        self.perm_arrs = []
+        self.sign = None
+        if st["test"] == 2:
+            self.sign = np.ascontiguousarray(perms["sign"], np.float32)
+            self.permp = None
+        else:
+            pl = perms["perm"] if isinstance(perms["perm"], list) else [perms["perm"]] * nmaps
+            self.perm_arrs = [np.ascontiguousarray(a, np.uint16) for a in pl]
+            self.permp = (c_void_p * nmaps)(*[a.ctypes.data for a in self.perm_arrs])
+        s = Study()
+        s.test, s.inference = st["test"], st.get("inference", 0)
+        s.W, s.H, s.D, s.N, s.p, s.nc = st["W"], st["H"], st["D"], N, p, nc
+        s.Y, s.mask, s.X, s.xtxxt = P(self.Y), P(self.mask), P(self.X), P(self.xtxxt)
+        s.C, s.ctxtxc, s.K = P(self.C), P(self.ctxtxc), P(self.K)
+        s.perm = ctypes.cast(self.permp, c_void_p) if self.permp is not None else None
+        s.sign = P(self.sign)
+        s.cdt = st.get("cdt", 0.0)
+        s.connectivity = st.get("connectivity", 26)
+        s.nthreads = nthreads
+        self.s = s
+        self.nmaps = nmaps
+
+
+def perm_stats(st: dict, perms: dict, K: int, c: int = 0, precision: int = F64) -> np.ndarray:
+    """Statistic maps of the first K permutations of map c: (K, D*H*W)."""
+    b = _Bound(st, perms, K)
+    out = np.zeros((K, st["W"] * st["H"] * st["D"]))
+    rc = lib().orc_perm_stats(ctypes.byref(b.s), c, precision, c_long(K), P(out))
+    assert rc == 0
+    return out
+
+
+def randomise(st: dict, perms: dict, K, precision: int = F64):
+    """The whole path: (maps, pvalues, maxdist list)."""
+    b = _Bound(st, perms, K)
+    G = st["W"] * st["H"] * st["D"]
+    maps = np.zeros(b.nmaps * G, np.float32)
+    pv = np.zeros(b.nmaps * G, np.float32)
+    md = [np.zeros(int(k), np.float32) for k in b.K]
+    mdp = (c_void_p * b.nmaps)(*[m.ctypes.data for m in md])
+    rc = lib().orc_randomise(ctypes.byref(b.s), precision, P(maps), P(pv), ctypes.cast(mdp, c_void_p))
+    assert rc == 0
+    return maps.reshape(b.nmaps, G), pv.reshape(b.nmaps, G), md
+
+
+def permutation_phase32(st: dict, perms: dict, K: int, Ystar: np.ndarray, nthreads: int = 0):
+    """Times-able CPU restatement of the reference's K loop (voxel mode) on
+    already-transformed data Ystar (grid layout)."""
+    st2 = dict(st)
+    st2["Y"] = Ystar
+    b = _Bound(st2, perms, K, nthreads)
+    md = np.zeros(K, np.float32)
+    rc = lib().orc_permutation_phase32(ctypes.byref(b.s), 0, c_long(K), P(md))
+    assert rc == 0
+    return md
+
+
+def transformed32(st: dict, c: int = 0) -> np.ndarray:
+    """The FP32 TransformData output of map c (grid layout, float32)."""
+    N = st["N"]
+    ftest = st["test"] == 1
+    p, nc = st["p"], st["nc"]
+    Y = np.ascontiguousarray(st["Y"], np.float32).copy()
+    transform = (st["test"] == 0 and p > 1) or (ftest and p > nc)
+    if transform:
+        rows = st["C"][c:c + 1] if st["test"] == 0 else st["C"]
+        R = nuisance_projector64(st["X"].astype(np.float64), rows.astype(np.float64))
+        Rf = np.ascontiguousarray(R.T, np.float32)  # column-major
+        lib().orc_transform32(st["W"], st["H"], st["D"], N, P(Rf), P(np.ascontiguousarray(st["mask"], np.float32)),
+                              P(Y))
+    return Y
