@@ -1,0 +1,170 @@
+"""Benchmark of the permutation phase (perm*voxels/s, GLM max-T) on one MI355X.
+
+Workload: the two-sample GLM with covariates -- 91x109x91 grid with an
+ellipsoid mask (~228k voxels), N=60 subjects (30/30), p=4 regressors
+(intercept, group, demeaned age, sex), t contrast [0 1 0 0], 10,000 row
+permutations -- on synthetic data of that shape.
+
+One step = the whole permutation phase of the contrast: nuisance
+residualisation and operand split of the HBM-resident data, permutation
+operand build, the fused contraction + t + max-T kernel over all 10,000
+permutations, and the max-T array copied to the host.  The study is
+uploaded once before timing.
+
+Printed JSON line: metric/value for the split-BF16 MFMA form (the headline),
+the plain FP32 form measured in the same run, accuracy against the FP64
+oracle on sampled permutations, the roofline of the dominant kernel (its
+launch time from HIP events on the plan's stream), and the reference
+algorithm restated on the host cores (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from broccoli_amd import capi, synth  # noqa: E402
+
+METRIC = "perm·voxels/s (GLM max-T), split-BF16 MFMA vs plain FP32, t err≤1e-4(1+|t|)"
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+PEAK_FP32_TFLOPS = 157.3
+
+
+def hip_sync():
+    try:
+        ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def run_form(ctx, st, perms, precision, steps, warmup):
+    plan = capi.Plan(ctx, st, precision)
+    for _ in range(warmup):
+        plan.permutation_phase(0, **perms)
+    hip_sync()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        md, _ = plan.permutation_phase(0, **perms)
+        kern.append(plan.timing().contract_ms)
+    hip_sync()
+    dt = (time.perf_counter() - t0) / steps
+    tm = plan.timing()
+    return plan, md, dt, float(np.mean(kern)), tm
+
+
+def accuracy(ctx, st, perms, nsample):
+    """Sampled permutations of the split-BF16 form against the FP64 oracle."""
+    from oracle import oracle
+
+    sub = {k: v[:nsample] for k, v in perms.items() if k in ("perm", "sign")}
+    out = {}
+    for prec in ("split_bf16", "fp32"):
+        with capi.Plan(ctx, st, prec) as plan:
+            md, stats = plan.permutation_phase(0, want_stats=True, **sub)
+        ref = oracle.perm_stats(st, sub, nsample)[:, st["mask"].ravel() == 1.0]
+        err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+        q64 = np.trunc(ref.max(axis=1) * 1e4)
+        qg = np.round(md.astype(np.float64) * 1e4)
+        out[prec] = dict(max_err=float(err.max()), sampled_perms=nsample,
+                         maxT_quanta_differing=int((q64 != qg).sum()))
+    return out
+
+
+def cpu_baseline(st, perms, target_s):
+    """The reference's per-permutation loop restated in C (oracle, ref32
+    precision, OpenMP over the host cores) on a bounded sample of the same
+    workload's permutations."""
+    from oracle import oracle
+
+    ncores = min(len(os.sched_getaffinity(0)), 16)
+    Ystar = oracle.transformed32(st, 0)
+    sub = {k: v for k, v in perms.items() if k in ("perm", "sign")}
+    one = {k: v[:2] for k, v in sub.items()}
+    t0 = time.perf_counter()
+    oracle.permutation_phase32(st, one, 2, Ystar, nthreads=ncores)
+    per = (time.perf_counter() - t0) / 2
+    kc = int(max(2, min(2000, target_s / max(per, 1e-6))))
+    s2 = {k: v[:kc] for k, v in sub.items()}
+    t0 = time.perf_counter()
+    oracle.permutation_phase32(st, s2, kc, Ystar, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    return dict(value=kc * st["V"] / dt, unit="perm·voxels/s", cores=ncores, kind="port",
+                sample=f"{kc} of the 10000 permutations, all {st['V']} voxels, "
+                       f"{dt:.1f} s, one full-grid pass + quantised max per permutation")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--K", type=int, default=10000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the permutation phase is measured on a single "
+              "MI355X (sharding over permutations is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    st = synth.study(2)
+    K = args.K
+    perms = synth.permutations(st, K, seed=20260002)
+    V = st["V"]
+    N, p = st["N"], st["p"]
+    ctx = capi.Context(0)
+    print(f"# study ready: V={V} N={N} p={p} K={K} device={ctx.device_name}", file=sys.stderr, flush=True)
+
+    plan, md, dt, kern_ms, tm = run_form(ctx, st, perms, "split_bf16", args.steps, args.warmup)
+    plan.close()
+    print(f"# split_bf16: {dt*1e3:.3f} ms/step, contraction {kern_ms:.3f} ms", file=sys.stderr, flush=True)
+    value = K * V / dt
+    flops = 2.0 * N * p * K * V
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    res = {
+        "metric": METRIC, "value": value, "unit": "perm·voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "accumulate": "fp32", "data": "synthetic",
+        "config": {"workload": "two-sample GLM t-test with covariates, 91x109x91 grid, ellipsoid mask, "
+                               "N=60 (30/30), p=4, contrast [0 1 0 0], 10000 row permutations, voxel max-T",
+                   "V_mask": V, "N": N, "p": p, "K": K, "precision": "split_bf16"},
+        "phase_breakdown_ms": {"transform": tm.transform_ms, "operand": tm.operand_ms,
+                               "contract": tm.contract_ms},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "k_contract_bf16", "algorithmic_flop_per_launch": flops,
+                     "avg_launch_ms": kern_ms,
+                     "mfma_issue_frac": achieved * 3 * (16 * ((N + 15) // 16) / N) / PEAK_BF16_TFLOPS},
+    }
+    if not args.no_fp32:
+        plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, args.steps // 2), 1)
+        plan.close()
+        print(f"# fp32: {dt32*1e3:.3f} ms/step, contraction {kern32:.3f} ms", file=sys.stderr, flush=True)
+        a32 = flops / (kern32 * 1e-3) / 1e12
+        res["fp32_form"] = {"value": K * V / dt32, "ms_per_step": dt32 * 1e3, "contract_ms": kern32,
+                            "achieved_tflops": a32, "frac_of_fp32_peak": a32 / PEAK_FP32_TFLOPS}
+        res["speedup_vs_fp32"] = dt32 / dt
+        res["maxT_equal_fp32_vs_bf16"] = int(np.sum(md32 == md))
+    if not args.no_accuracy:
+        res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(st, perms, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
