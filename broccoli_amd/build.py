@@ -34,7 +34,12 @@ def hipcc() -> str:
 
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-c", str(src), "-o", str(obj)]
+    extra = []
+    if src.name.startswith("bpg_contract_bf16"):
+        # keep MFMA accumulators in architectural VGPRs: the epilogue reads
+        # them directly instead of through v_accvgpr_read
+        extra = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [hipcc(), *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-x", "c++",
                "-c", str(src), "-o", str(obj)]

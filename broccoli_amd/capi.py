@@ -12,7 +12,10 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().with_name("libbpg.so")
+import os
+
+# BPG_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = Path(os.environ.get("BPG_LIB", Path(__file__).resolve().with_name("libbpg.so")))
 
 TTEST, FTEST, GROUP_MEAN = 0, 1, 2
 VOXEL, CLUSTER_EXTENT, CLUSTER_MASS = 0, 1, 2

@@ -1,0 +1,260 @@
+// bpg_contract_bf16.h -- split-BF16 MFMA contraction with the fused
+// statistic and max-T epilogue (instantiated per KK in
+// bpg_contract_bf16_*.hip so the instantiations compile in parallel).
+//
+// Operand tile (A, per wave, resident): R subtiles of 32 permutation rows x
+// KK k-steps of 16 subjects, hi and lo bf16 parts, in the fragment layout of
+// v_mfma_f32_32x32x16_bf16 (lane l: row l%32, subjects 16*kk + 8*(l/32) +
+// 0..7).  Data tile (B, streamed): 32 voxels x KK k-steps, hi and lo, same
+// layout with voxels as columns.  Accumulator: lane l holds column l%32 and
+// rows 8*b + 4*(l/32) + a at slot 4*b + a; the operand rows are ordered so
+// slots j = P*PP .. P*PP+PP-1 of a lane are the PP regressors of one
+// permutation, so each lane finishes 16/PP whole permutations of one voxel.
+//
+// Epilogue per (permutation, voxel), with z = Qt' P' y* (orthonormal basis,
+// first axes spanning the contrast):
+//   SSE = |y*|^2 - |z|^2           (|y*|^2 held as an FP64-accurate float pair)
+//   t: u = z_0 / sqrt(SSE)          F: u = (z_0^2 + .. + z_{r-1}^2) / SSE
+// The running max keeps u; the positive constant sqrt(N-p) (t) or
+// (N-p)/r (F) is applied once after the max, which is exact: rounding is
+// monotone, so max(fl(c*u)) = fl(c*max(u)).
+#pragma once
+
+#include "bpg_device.h"
+
+#ifndef BPG_CHAIN_MODE
+#define BPG_CHAIN_MODE 0
+#endif
+
+namespace bpg {
+
+template <int KK>
+__device__ __forceinline__ void load_tile(const bf16x8* Y, const double* yy, int vt, int lane,
+                                          bf16x8 (&bh)[KK], bf16x8 (&bl)[KK], double& yv) {
+#pragma unroll
+    for (int kk = 0; kk < KK; kk++) {
+        const size_t base = (((size_t)vt * KK + kk) * 2) * 64 + lane;
+        bh[kk] = Y[base];
+        bl[kk] = Y[base + 64];
+    }
+    yv = yy[vt * 32 + (lane & 31)];
+}
+
+// One voxel tile: for each resident subtile, its MFMA chain and then its
+// statistic (the epilogue of subtile r overlaps the MFMAs of subtile r+1).
+// OUT: also write the statistic of every voxel (stats) and/or the
+// supra-threshold bitmask (bits); the max-T-only form carries no branches.
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT>
+__device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
+                                              const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
+                                              const bf16x8 (&bl)[KK], double yv, int vt, int lane,
+                                              int sub0, float (&runmax)[R][16 / PP]) {
+    constexpr int PER = 16 / PP;
+    const float yhi = (float)yv;  // NaN for padding voxels
+    const float ylo = (float)(yv - (double)yhi);
+    const int hl = lane >> 5;
+    const int v = vt * 32 + (lane & 31);
+    auto epilogue = [&](const f32x16& acc, int r) {
+        float u[PER];
+#pragma unroll
+        for (int P = 0; P < PER; P++) {
+            const float z0 = acc[P * PP];
+            float ss = z0 * z0, num = ss;
+#pragma unroll
+            for (int g = 1; g < PP; g++) {
+                const float z = acc[P * PP + g];
+                ss = fmaf(z, z, ss);
+                if (FT) num = (g < a.r) ? ss : num;
+            }
+            const float sse = (yhi - ss) + ylo;
+            u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
+            runmax[r][P] = fmaxf(runmax[r][P], u[P]);
+        }
+        if (OUT) {
+            const bool valid = v < a.Vc;
+            const float scale = FT ? a.f_scale : a.sqrt_dof;
+            const int kbase = (sub0 + r) * (32 / PP) + hl * PER;
+#pragma unroll
+            for (int P = 0; P < PER; P++) {
+                const float st = u[P] * scale;
+                const int k = kbase + P;
+                if (a.stats && valid && k < a.K) a.stats[(size_t)k * a.Vc + v] = st;
+                if (a.bits) {
+                    const unsigned long long bal = __ballot(valid && st > a.cdt);
+                    if ((lane & 31) == 0 && k < a.K)
+                        a.bits[(size_t)k * a.nwords + vt] = hl ? (unsigned)(bal >> 32) : (unsigned)bal;
+                }
+            }
+        }
+    };
+#if BPG_CHAIN_MODE == 1
+    // the R accumulation chains interleaved, then the R epilogues
+    f32x16 acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[r][e] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KK; kk++)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!SIGN) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], bh[kk], acc[r], 0, 0, 0);
+            acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bl[kk], acc[r], 0, 0, 0);
+            acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bh[kk], acc[r], 0, 0, 0);
+        }
+#pragma unroll
+    for (int r = 0; r < R; r++) epilogue(acc[r], r);
+#else
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        f32x16 acc;
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) {
+            if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], bh[kk], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bl[kk], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bh[kk], acc, 0, 0, 0);
+        }
+        epilogue(acc, r);
+    }
+#endif
+}
+
+// Block = 4 waves, each with R resident operand subtiles; the block walks a
+// chunk of voxel tiles, loading tile t+1 (ping-pong registers, straight from
+// L2; the 4 waves share the lines through the CU's L1) while tile t is
+// multiplied.  At the end, a 32-lane butterfly per permutation and one
+// atomic per permutation per wave.
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT>
+__global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
+    constexpr int PER = 16 / PP;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int lb = xcd_logical_block(L.per_xcd);
+    if (lb >= L.n_mgroups * L.n_vchunks) return;
+    const int mg = lb % L.n_mgroups;
+    const int vc = lb / L.n_mgroups;
+    const int sub0 = (mg * 4 + wave) * R;
+
+    const bf16x8* G = reinterpret_cast<const bf16x8*>(a.Gb);
+    const bf16x8* Y = reinterpret_cast<const bf16x8*>(a.Yb);
+
+    bf16x8 ahi[R][KK], alo[R][KK];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int st = sub0 + r;
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) {
+            bf16x8 h = {}, l = {};
+            if (st < L.n_sub) {
+                const size_t base = (((size_t)st * KK + kk) * 2) * 64 + lane;
+                h = G[base];
+                if (!SIGN) l = G[base + 64];
+            }
+            ahi[r][kk] = h;
+            alo[r][kk] = l;
+        }
+    }
+    float runmax[R][PER];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int P = 0; P < PER; P++) runmax[r][P] = -INFINITY;
+
+    const int vt0 = vc * L.vt_per_chunk;
+    const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
+#if BPG_PINGPONG
+    bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
+    double y0 = 0.0, y1 = 0.0;
+    if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
+    for (int vt = vt0; vt < vt1; vt += 2) {
+        if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
+        contract_tile<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax);
+        if (vt + 1 >= vt1) break;
+        if (vt + 2 < vt1) load_tile<KK>(Y, a.yy, vt + 2, lane, b0h, b0l, y0);
+        contract_tile<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax);
+    }
+#else
+    for (int vt = vt0; vt < vt1; vt++) {
+        bf16x8 bh[KK], bl[KK];
+        double yv;
+        load_tile<KK>(Y, a.yy, vt, lane, bh, bl, yv);
+        contract_tile<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, bh, bl, yv, vt, lane, sub0, runmax);
+    }
+#endif
+    const int hl = lane >> 5;
+    const float scale = FT ? a.f_scale : a.sqrt_dof;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int P = 0; P < PER; P++) {
+            float m = runmax[r][P];
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 8, 64));
+            m = fmaxf(m, __shfl_xor(m, 4, 64));
+            m = fmaxf(m, __shfl_xor(m, 2, 64));
+            m = fmaxf(m, __shfl_xor(m, 1, 64));
+            const int k = (sub0 + r) * (32 / PP) + hl * PER + P;
+            if ((lane & 31) == 0 && k < a.K && m > -INFINITY) atomicMax(&a.maxbits[k], enc_max(m * scale));
+        }
+}
+
+// Tuning knobs: the largest KK that keeps 4 / 2 operand subtiles per wave,
+// and the order of the accumulation chains within a voxel tile (0: each
+// subtile's chain then its epilogue; 1: chains interleaved, then epilogues).
+#ifndef BPG_R_SMALL
+#define BPG_R_SMALL 4  // subtiles per wave for KK <= 4 (N <= 64)
+#endif
+#ifndef BPG_R2_KK
+#define BPG_R2_KK 8
+#endif
+#ifndef BPG_TARGET_BLOCKS
+#define BPG_TARGET_BLOCKS 4096
+#endif
+#ifndef BPG_PINGPONG
+#define BPG_PINGPONG 1
+#endif
+
+template <int KK, int R, int PP, bool SIGN, bool FT>
+hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
+    const int blocks = L.per_xcd * 8;
+    if (a.stats || a.bits)
+        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, true>), dim3(blocks), dim3(256), 0, s, a, L);
+    else
+        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, false>), dim3(blocks), dim3(256), 0, s, a, L);
+    return hipGetLastError();
+}
+
+template <int KK, int PP>
+hipError_t dispatch_bf16_kind(const ContractArgs& a, Bf16Launch L, bool sign, hipStream_t s) {
+    constexpr int R = (KK <= 4) ? BPG_R_SMALL : (KK <= BPG_R2_KK) ? 2 : 1;
+    L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
+    // voxel chunks: enough blocks to cover the 256 CUs several times
+    const int target_blocks = BPG_TARGET_BLOCKS;
+    int nvc = (target_blocks + L.n_mgroups - 1) / L.n_mgroups;
+    if (nvc > L.n_vt) nvc = L.n_vt;
+    if (nvc < 1) nvc = 1;
+    L.vt_per_chunk = (L.n_vt + nvc - 1) / nvc;
+    L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
+    L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
+    if (sign) {
+        if (a.ftest || PP != 1) return hipErrorInvalidValue;
+        if constexpr (PP == 1) return run_bf16<KK, R, 1, true, false>(a, L, s);
+    }
+    return a.ftest ? run_bf16<KK, R, PP, false, true>(a, L, s) : run_bf16<KK, R, PP, false, false>(a, L, s);
+}
+
+template <int KK>
+hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s) {
+    switch (PP) {
+        case 1: return dispatch_bf16_kind<KK, 1>(a, L, sign, s);
+        case 2: return dispatch_bf16_kind<KK, 2>(a, L, sign, s);
+        case 4: return dispatch_bf16_kind<KK, 4>(a, L, sign, s);
+        case 8: return dispatch_bf16_kind<KK, 8>(a, L, sign, s);
+        case 16: return dispatch_bf16_kind<KK, 16>(a, L, sign, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace bpg

@@ -54,6 +54,11 @@ struct ContractArgs {
     float f_scale;         // F: (N - p) / r
     float cdt;
 };
+// Grid of the split-BF16 contraction (see bpg_contract_bf16.h).
+struct Bf16Launch {
+    int n_mgroups, n_vchunks, vt_per_chunk, per_xcd, n_sub, n_vt;
+};
+
 hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign_operand,
                                 hipStream_t s, int* launches);
 hipError_t launch_contract_f32(const ContractArgs& a, int NS, hipStream_t s, int* launches);

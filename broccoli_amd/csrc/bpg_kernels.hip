@@ -21,44 +21,16 @@
 #include <cstring>
 
 #include "bpg.h"
+#include "bpg_device.h"
 #include "bpg_kernels.h"
 
 namespace bpg {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-// ---------------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------------
-
-// float -> bf16 bits, round to nearest even
-__device__ __forceinline__ unsigned short f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-
-// order-preserving float <-> uint encoding for atomicMax
-__device__ __forceinline__ unsigned enc_max(float f) {
-    unsigned u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
 
 float decode_max(unsigned e) {
     unsigned u = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
     float f;
     std::memcpy(&f, &u, sizeof f);
     return f;
-}
-
-// Blocks are dealt round-robin to the 8 XCDs.  Renumber them so that the
-// blocks resident on one XCD walk consecutive logical ids, i.e. share the
-// same voxel chunk (and its lines in that XCD's L2).
-__device__ __forceinline__ int xcd_logical_block(int per_xcd) {
-    const int b = blockIdx.x;
-    return (b & 7) * per_xcd + (b >> 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -108,7 +80,9 @@ __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
             Yb16[(base + 64) * 8 + e] = lo;
         }
     }
-    a.yy[v] = in ? yy : 1.0;
+    // padding voxels carry a NaN norm: their statistic is NaN, which the
+    // running max (fmax) ignores
+    a.yy[v] = in ? yy : __builtin_nan("");
 }
 
 hipError_t launch_transform(const TransformArgs& a, hipStream_t s) {
@@ -200,173 +174,22 @@ hipError_t launch_operand_f32(const OperandArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // SSE = |y*|^2 - |z|^2 with the subtraction in FP64 (|y*|^2 is exact in
 // FP64; the difference is then rounded once).
+// v_rsq_f32 / v_rcp_f32 directly: SSE is a sum of squares of the order of N,
+// never denormal, so the denormal-scaling wrappers of rsqrtf / division are
+// not needed.
 __device__ __forceinline__ float stat_t(float z0, float ss, double yyv, float sqrt_dof) {
     const float sse = (float)(yyv - (double)ss);
-    return z0 * sqrt_dof * rsqrtf(sse);
+    return z0 * sqrt_dof * __builtin_amdgcn_rsqf(sse);
 }
 __device__ __forceinline__ float stat_f(float num, float ss, double yyv, float f_scale) {
     const float sse = (float)(yyv - (double)ss);
-    return num * f_scale / sse;
+    return num * f_scale * __builtin_amdgcn_rcpf(sse);
 }
 
-// ---------------------------------------------------------------------------
-// split-BF16 MFMA contraction
-// ---------------------------------------------------------------------------
-// Block = 4 waves.  Each wave keeps R subtiles (32 permutation rows each) of
-// the operand resident in registers (hi and lo parts, KK k-steps of 16
-// subjects) and streams 32-voxel tiles of the data operand.  Per tile and
-// subtile: 2 or 3 MFMAs per k-step, then the statistic for the 16/PP
-// permutations each lane owns.  Running maxima stay in registers until the
-// end, then a 32-lane butterfly and one atomic per permutation per wave.
-struct Bf16Launch {
-    int n_mgroups, n_vchunks, vt_per_chunk, per_xcd, n_sub, n_vt;
-};
-
-template <int KK, int R, int PP, bool SIGN, bool FT>
-__global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int lb = xcd_logical_block(L.per_xcd);
-    if (lb >= L.n_mgroups * L.n_vchunks) return;
-    const int mg = lb % L.n_mgroups;
-    const int vc = lb / L.n_mgroups;
-    const int sub0 = (mg * 4 + wave) * R;
-    constexpr int PER = 16 / PP;  // permutations per lane per subtile
-
-    const bf16x8* G = reinterpret_cast<const bf16x8*>(a.Gb);
-    const bf16x8* Y = reinterpret_cast<const bf16x8*>(a.Yb);
-
-    bf16x8 ahi[R][KK], alo[R][KK];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int st = sub0 + r;
-#pragma unroll
-        for (int kk = 0; kk < KK; kk++) {
-            bf16x8 h = {}, l = {};
-            if (st < L.n_sub) {
-                const size_t base = (((size_t)st * KK + kk) * 2) * 64 + lane;
-                h = G[base];
-                if (!SIGN) l = G[base + 64];
-            }
-            ahi[r][kk] = h;
-            alo[r][kk] = l;
-        }
-    }
-    float runmax[R][PER];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int P = 0; P < PER; P++) runmax[r][P] = -INFINITY;
-
-    const int hl = lane >> 5;
-    const int vt0 = vc * L.vt_per_chunk;
-    const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
-    for (int vt = vt0; vt < vt1; vt++) {
-        bf16x8 bhi[KK], blo[KK];
-#pragma unroll
-        for (int kk = 0; kk < KK; kk++) {
-            const size_t base = (((size_t)vt * KK + kk) * 2) * 64 + lane;
-            bhi[kk] = Y[base];
-            blo[kk] = Y[base + 64];
-        }
-        const int v = vt * 32 + (lane & 31);
-        const double yyv = a.yy[v];
-        const bool valid = v < a.Vc;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            f32x16 acc;
-#pragma unroll
-            for (int e = 0; e < 16; e++) acc[e] = 0.0f;
-#pragma unroll
-            for (int kk = 0; kk < KK; kk++) {
-                if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], bhi[kk], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], blo[kk], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bhi[kk], acc, 0, 0, 0);
-            }
-            const int kbase = (sub0 + r) * (32 / PP) + hl * PER;
-#pragma unroll
-            for (int P = 0; P < PER; P++) {
-                float ss = 0.0f, num = 0.0f;
-#pragma unroll
-                for (int g = 0; g < PP; g++) {
-                    const float z = acc[P * PP + g];
-                    ss = fmaf(z, z, ss);
-                    if (FT) {
-                        if (g < a.r) num = fmaf(z, z, num);
-                    }
-                }
-                float st = FT ? stat_f(num, ss, yyv, a.f_scale) : stat_t(acc[P * PP], ss, yyv, a.sqrt_dof);
-                if (!valid) st = -INFINITY;
-                runmax[r][P] = fmaxf(runmax[r][P], st);
-                const int k = kbase + P;
-                if (a.stats && valid && k < a.K) a.stats[(size_t)k * a.Vc + v] = st;
-                if (a.bits) {
-                    const unsigned long long bal = __ballot(valid && st > a.cdt);
-                    if ((lane & 31) == 0 && k < a.K)
-                        a.bits[(size_t)k * a.nwords + vt] = hl ? (unsigned)(bal >> 32) : (unsigned)bal;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int P = 0; P < PER; P++) {
-            float m = runmax[r][P];
-            m = fmaxf(m, __shfl_xor(m, 16, 64));
-            m = fmaxf(m, __shfl_xor(m, 8, 64));
-            m = fmaxf(m, __shfl_xor(m, 4, 64));
-            m = fmaxf(m, __shfl_xor(m, 2, 64));
-            m = fmaxf(m, __shfl_xor(m, 1, 64));
-            const int k = (sub0 + r) * (32 / PP) + hl * PER + P;
-            if ((lane & 31) == 0 && k < a.K && m > -INFINITY) atomicMax(&a.maxbits[k], enc_max(m));
-        }
-}
-
-namespace {
-
-template <int KK, int R, int PP, bool SIGN, bool FT>
-hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
-    const int total = L.n_mgroups * L.n_vchunks;
-    const int blocks = L.per_xcd * 8;
-    (void)total;
-    hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT>), dim3(blocks), dim3(256), 0, s, a, L);
-    return hipGetLastError();
-}
-
-template <int KK, int PP>
-hipError_t dispatch_bf16_kind(const ContractArgs& a, Bf16Launch L, bool sign, hipStream_t s) {
-    // resident operand subtiles per wave: keep hi+lo fragments within ~128 VGPRs
-    constexpr int R = (KK <= 2) ? 4 : (KK <= 4 ? 2 : 1);
-    L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
-    // voxel chunks: enough blocks to cover the 256 CUs several times
-    int target_blocks = 2048;
-    int nvc = (target_blocks + L.n_mgroups - 1) / L.n_mgroups;
-    if (nvc > L.n_vt) nvc = L.n_vt;
-    if (nvc < 1) nvc = 1;
-    L.vt_per_chunk = (L.n_vt + nvc - 1) / nvc;
-    L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
-    L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
-    if (sign) {
-        if (a.ftest || PP != 1) return hipErrorInvalidValue;
-        if constexpr (PP == 1) return run_bf16<KK, R, 1, true, false>(a, L, s);
-    }
-    return a.ftest ? run_bf16<KK, R, PP, false, true>(a, L, s) : run_bf16<KK, R, PP, false, false>(a, L, s);
-}
-
+// split-BF16 MFMA contraction: bpg_contract_bf16.h, instantiated per KK in
+// bpg_contract_bf16_*.hip
 template <int KK>
-hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s) {
-    switch (PP) {
-        case 1: return dispatch_bf16_kind<KK, 1>(a, L, sign, s);
-        case 2: return dispatch_bf16_kind<KK, 2>(a, L, sign, s);
-        case 4: return dispatch_bf16_kind<KK, 4>(a, L, sign, s);
-        case 8: return dispatch_bf16_kind<KK, 8>(a, L, sign, s);
-        case 16: return dispatch_bf16_kind<KK, 16>(a, L, sign, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-}  // namespace
+hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s);
 
 hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign_operand,
                                 hipStream_t s, int* launches) {
