@@ -60,9 +60,25 @@ def build(verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+def build_driver() -> Path:
+    """The C++ program that drives the BROCCOLI_LIB drop-in with the
+    reference CLI's call sequence (tests/cpp/dropin_driver.cpp)."""
+    src = ROOT / "tests" / "cpp" / "dropin_driver.cpp"
+    exe = src.with_suffix("")
+    if not src.exists():
+        return exe
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+           f"-L{PKG}", "-lbpg", "-Wl,-rpath,$ORIGIN/../../broccoli_amd", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return exe
 
 
 if __name__ == "__main__":

@@ -113,6 +113,7 @@ def _check(rc: int, ctx=None):
 # ---------------------------------------------------------------- generators
 
 def gen_sign_matrix(N: int, K: int, compat: bool = True, seed: int = 0) -> np.ndarray:
+    """compat: the reference stream (seed != 0 reseeds it like srand(seed))."""
     out = np.zeros((K, N), np.float32)
     _check(load().bpg_gen_sign_matrix(N, K, int(compat), seed, _ptr(out)))
     return out

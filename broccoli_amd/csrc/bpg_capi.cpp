@@ -139,8 +139,8 @@ extern "C" int bpg_create(int device, bpg_ctx** out) {
         return BPG_ERR_HIP;
     }
     c->device = device;
-    c->name = prop.name;
     c->arch = prop.gcnArchName;
+    c->name = prop.name[0] ? std::string(prop.name) : "AMD GPU " + c->arch;
     if (c->arch.compare(0, 6, "gfx950") != 0) {
         delete c;
         return BPG_ERR_UNSUPPORTED;

@@ -270,12 +270,56 @@ struct SplitMix64 {
     unsigned bounded(unsigned n) { return (unsigned)(((next() >> 32) * (unsigned long long)n) >> 32); }
 };
 
-// libstdc++ std::random_shuffle(first, last) with the C rand():
+// glibc's rand() (random_r, TYPE_3: additive feedback r[i] = r[i-31] +
+// r[i-3] over a 34-word history seeded by the Park-Miller recurrence
+// 16807*x mod (2^31-1), 310 warm-up draws, output r[i] >> 1), as a private
+// state.  The reference draws from the process-global rand() without ever
+// seeding it (LIB:16925-17122), i.e. from seed 1; a private copy reproduces
+// that stream whatever else in the process (e.g. a GPU runtime) calls
+// rand().
+class GlibcRand {
+public:
+    GlibcRand() { seed(1); }
+    void seed(unsigned s) {
+        if (s == 0) s = 1;
+        int32_t r[34];
+        r[0] = (int32_t)s;
+        for (int i = 1; i < 31; i++) {
+            const int64_t hi = r[i - 1] / 127773, lo = r[i - 1] % 127773;
+            int64_t w = 16807 * lo - 2836 * hi;
+            if (w < 0) w += 2147483647;
+            r[i] = (int32_t)w;
+        }
+        for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+        for (int i = 0; i < 34; i++) h_[i] = (uint32_t)r[i];
+        pos_ = 34;
+        for (int i = 34; i < 344; i++) step();
+    }
+    int next() { return (int)(step() >> 1); }
+
+private:
+    uint32_t step() {
+        const uint32_t v = h_[(pos_ - 31) % 34] + h_[(pos_ - 3) % 34];
+        h_[pos_ % 34] = v;
+        pos_++;
+        return v;
+    }
+    uint32_t h_[34];
+    uint64_t pos_ = 0;
+};
+
+GlibcRand& compat_stream() {
+    static GlibcRand g;
+    return g;
+}
+
+// libstdc++ std::random_shuffle(first, last) with rand():
 // for i = 1..n-1, j = rand() % (i+1), swap unless i == j.
 template <class T>
 void compat_shuffle(T* a, int n) {
+    GlibcRand& g = compat_stream();
     for (int i = 1; i < n; i++) {
-        const int j = std::rand() % (i + 1);
+        const int j = g.next() % (i + 1);
         if (i != j) std::swap(a[i], a[j]);
     }
 }
@@ -312,11 +356,12 @@ extern "C" int bpg_gen_sign_matrix(int N, long K, int compat, unsigned long long
     seen.insert(key_of(flips.data(), N));
     for (int i = 0; i < N; i++) sign[i] = 1.0f;
     SplitMix64 g{seed};
+    if (compat && seed) compat_stream().seed((unsigned)seed);
     for (long k = 1; k < K; k++) {
         for (;;) {
             if (compat) {
                 // cumulative flips, LIB:17093-17097
-                for (int i = 0; i < N; i++) flips[i] *= (2 * (int)(std::rand() % 2) - 1);
+                for (int i = 0; i < N; i++) flips[i] *= (2 * (int)(compat_stream().next() % 2) - 1);
             } else {
                 for (int i = 0; i < N; i++) flips[i] = (g.next() >> 63) ? 1 : -1;
             }
@@ -338,6 +383,7 @@ extern "C" int bpg_gen_perm_two_sample(int N, int n1, int n2, long K, int compat
     seen.insert(key_of(g.data(), N));
     fill_two_sample(g.data(), N, n1, perm);
     SplitMix64 rng{seed};
+    if (compat && seed) compat_stream().seed((unsigned)seed);
     for (long k = 1; k < K; k++) {
         for (;;) {
             if (compat) compat_shuffle(g.data(), N);
@@ -359,6 +405,7 @@ extern "C" int bpg_gen_perm_correlation(int N, long K, int compat, unsigned long
     seen.insert(key_of(cur.data(), N));
     std::copy(cur.begin(), cur.end(), perm);
     SplitMix64 rng{seed};
+    if (compat && seed) compat_stream().seed((unsigned)seed);
     for (long k = 1; k < K; k++) {
         for (;;) {
             if (compat) compat_shuffle(cur.data(), N);

@@ -137,12 +137,14 @@ int bpg_plan_pvalues(bpg_plan* plan, const long* K, const float* const* maxdist,
 /* ---- host-side pieces of the path -------------------------------------- */
 
 /* Generators (LIB:17069-17122, LIB:16925-17011, LIB:17014-17066).
- * compat != 0: the reference's stream -- the process-global C rand(),
- *   libstdc++'s std::random_shuffle order, uniqueness rejection; row 0 is
- *   the identity / all +1.  Same output as the reference for the same rand()
- *   state.
+ * compat != 0: the reference's stream -- glibc's rand() algorithm restated
+ *   on a library-private state, libstdc++'s std::random_shuffle order,
+ *   uniqueness rejection; row 0 is the identity / all +1.  seed != 0
+ *   reseeds the stream first (as srand(seed)); seed == 0 continues it.  It
+ *   starts as rand() does without srand (seed 1), which is the reference's
+ *   stream: it never seeds rand().
  * compat == 0: a seeded SplitMix64 stream with Fisher-Yates shuffles and the
- *   same uniqueness rule, independent of global state.
+ *   same uniqueness rule.
  * Returns BPG_ERR_ARG when K exceeds the number of distinct vectors. */
 int bpg_gen_sign_matrix(int N, long K, int compat, unsigned long long seed, float* sign);
 int bpg_gen_perm_two_sample(int N, int n1, int n2, long K, int compat,

@@ -1,0 +1,90 @@
+"""The BROCCOLI_LIB drop-in, driven like the reference CLI (tests/cpp/
+dropin_driver.cpp), against the oracle's whole path on the same inputs."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from broccoli_amd import synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = Path(__file__).resolve().parent / "cpp" / "dropin_driver"
+
+
+def write_study(d: Path, st, K, use_file=0, perms=None, precision=1):
+    d.mkdir(parents=True, exist_ok=True)
+    (d / "study.txt").write_text(
+        f"{st['W']} {st['H']} {st['D']} {st['N']} {st['p']} {st['nc']} {st['test']} "
+        f"{st.get('inference', 0)} {K} {st.get('cdt', 0.0)} {use_file} {precision}\n")
+    np.ascontiguousarray(st["Y"], np.float32).tofile(d / "Y.bin")
+    np.ascontiguousarray(st["mask"], np.float32).tofile(d / "mask.bin")
+    np.asfortranarray(st["X"], np.float32).ravel(order="F").tofile(d / "X.bin")
+    np.ascontiguousarray(st["C"], np.float32).ravel().tofile(d / "C.bin")
+    if perms is not None:
+        if "perm" in perms:
+            np.ascontiguousarray(perms["perm"], np.uint16).tofile(d / "perm.bin")
+        else:
+            np.ascontiguousarray(perms["sign"], np.float32).tofile(d / "sign.bin")
+
+
+def run(d: Path):
+    res = subprocess.run([str(DRIVER), str(d)], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    return res.stdout
+
+
+@pytest.mark.parametrize("wl", [1, 2, 3])
+def test_dropin_generated_permutations_match_reference_stream(tmp_path, wl):
+    """Without a permutation file the drop-in generates the reference's
+    rand()/random_shuffle stream (fresh process: default seed)."""
+    st = synth.study(wl, scale=0.2)
+    if wl == 3:
+        st["cdt"] = 2.5
+    K = 48
+    write_study(tmp_path, st, K)
+    out = run(tmp_path)
+    assert "errors 0" in out
+    N = st["N"]
+    G = st["W"] * st["H"] * st["D"]
+    if wl == 1:
+        used = dict(sign=np.fromfile(tmp_path / "sign_used.bin", np.float32).reshape(K, N))
+        assert np.array_equal(used["sign"], oracle.gen_sign_matrix(N, K, seed=1))
+    else:
+        used = dict(perm=np.fromfile(tmp_path / "perm_used.bin", np.uint16).reshape(K, N))
+        d, n1, _ = oracle.detect_designs(st["X"], st["C"], st["test"] == 1)
+        assert d[0] == 0
+        assert np.array_equal(used["perm"], oracle.gen_perm_two_sample(N, int(n1[0]), K, seed=1))
+    maps = np.fromfile(tmp_path / "maps.bin", np.float32).reshape(-1, G)
+    pv = np.fromfile(tmp_path / "pvalues.bin", np.float32).reshape(-1, G)
+    dist = np.fromfile(tmp_path / "dist0.bin", np.float32)
+    maps64, P64, md64 = oracle.randomise(st, used, K, precision=oracle.F64)
+    err = np.abs(maps - maps64) / (1 + np.abs(maps64))
+    assert err.max() <= 1e-6
+    if st.get("inference", 0) == 0:
+        q = np.round(dist.astype(np.float64) * 1e4)
+        q64 = np.round(md64[0].astype(np.float64) * 1e4)
+        assert np.all(np.abs(q - q64) <= 1)
+        m = st["mask"].ravel() == 1.0
+        assert (np.abs(pv[0][m] - P64[0][m]) * K).max() <= (q != q64).sum()
+    else:
+        # cluster sizes: integers, equal unless a voxel sits on the threshold
+        assert np.mean(dist == md64[0]) >= 0.9
+
+
+def test_dropin_permutation_file_and_fp32(tmp_path):
+    st = synth.study(2, scale=0.2)
+    K = 32
+    perms = synth.permutations(st, K, seed=3)
+    write_study(tmp_path, st, K, use_file=1, perms=perms, precision=0)
+    run(tmp_path)
+    used = np.fromfile(tmp_path / "perm_used.bin", np.uint16).reshape(K, st["N"])
+    assert np.array_equal(used, perms["perm"])
+    dist = np.fromfile(tmp_path / "dist0.bin", np.float32)
+    _, _, md64 = oracle.randomise(st, perms, K, precision=oracle.F64)
+    q = np.round(dist.astype(np.float64) * 1e4)
+    q64 = np.round(md64[0].astype(np.float64) * 1e4)
+    assert np.all(np.abs(q - q64) <= 1)
