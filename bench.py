@@ -39,6 +39,16 @@ PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 PEAK_FP32_TFLOPS = 157.3
 
 
+WORKLOADS = {
+    1: "one-sample t by sign flipping, 91x109x91 grid, ellipsoid mask, N=20, p=1, 1000 sign flips, voxel max-T",
+    2: "two-sample GLM t-test with covariates, 91x109x91 grid, ellipsoid mask, N=60 (30/30), p=4, "
+       "contrast [0 1 0 0], 10000 row permutations, voxel max-T",
+    3: "F test over 3 group dummies, 91x109x91 grid, N=100, p=6, smooth noise (FWHM 3), 10000 permutations, "
+       "cluster-extent max (F>4, 26-connectivity)",
+    5: "t test on a covariate, 182x218x182 grid (~1.83M voxels), N=200, p=8, 10000 row permutations, voxel max-T",
+}
+
+
 def hip_sync():
     try:
         ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
@@ -60,6 +70,23 @@ def run_form(ctx, st, perms, precision, steps, warmup):
     dt = (time.perf_counter() - t0) / steps
     tm = plan.timing()
     return plan, md, dt, float(np.mean(kern)), tm
+
+
+def mfma_busy(st, K, kern_ms, clock_hz=2.4e9, simds=1024):
+    """Share of the MFMA pipes' cycles the launch keeps busy: 32x32x16 BF16
+    MFMAs issued (32 cycles each; 2 split products for sign operands, 3
+    otherwise; subjects padded to 16, regressors to a power of two) over
+    the launch time at the peak clock."""
+    N, V = st["N"], st["V"]
+    pfit = 1 if st["test"] == capi.GROUP_MEAN else st["p"]
+    pp = 1
+    while pp < pfit:
+        pp *= 2
+    kk = (N + 15) // 16
+    n_sub = (K * pp + 31) // 32
+    n_vt = (V + 255) // 256 * 8
+    prods = 2 if st["test"] == capi.GROUP_MEAN else 3
+    return n_sub * n_vt * kk * prods * 32 / (kern_ms * 1e-3 * clock_hz * simds)
 
 
 def accuracy(ctx, st, perms, nsample):
@@ -113,15 +140,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 5],
+                    help="2 (default, the metric's workload); 1: sign-flip group mean, K=1000; "
+                         "3: F test with cluster-extent inference; 5: 182x218x182 grid, N=200, p=8")
+    ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = full size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the permutation phase is measured on a single "
               "MI355X (sharding over permutations is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    st = synth.study(2)
-    K = args.K
-    perms = synth.permutations(st, K, seed=20260002)
+    st = synth.study(args.workload, scale=args.scale)
+    K = args.K if args.workload == 2 or args.K != 10000 else st["K"]
+    perms = synth.permutations(st, K, seed=20260000 + args.workload)
+    if args.workload != 2:
+        # extra data points: no CPU baseline or oracle sample at these sizes
+        args.no_cpu = True
+        args.no_accuracy = args.no_accuracy or args.workload in (3, 5)
     V = st["V"]
     N, p = st["N"], st["p"]
     ctx = capi.Context(0)
@@ -138,16 +173,16 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "accumulate": "fp32", "data": "synthetic",
-        "config": {"workload": "two-sample GLM t-test with covariates, 91x109x91 grid, ellipsoid mask, "
-                               "N=60 (30/30), p=4, contrast [0 1 0 0], 10000 row permutations, voxel max-T",
+        "config": {"workload": WORKLOADS[args.workload] if args.scale == 1.0 else
+                   f"{WORKLOADS[args.workload]} (grid scale {args.scale})",
                    "V_mask": V, "N": N, "p": p, "K": K, "precision": "split_bf16"},
         "phase_breakdown_ms": {"transform": tm.transform_ms, "operand": tm.operand_ms,
-                               "contract": tm.contract_ms},
+                               "contract": tm.contract_ms, "cluster": tm.cluster_ms},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
                      "kernel": "k_contract_bf16", "algorithmic_flop_per_launch": flops,
                      "avg_launch_ms": kern_ms,
-                     "mfma_issue_frac": achieved * 3 * (16 * ((N + 15) // 16) / N) / PEAK_BF16_TFLOPS},
+                     "mfma_pipe_busy_est": mfma_busy(st, K, kern_ms)},
     }
     if not args.no_fp32:
         plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, args.steps // 2), 1)

@@ -60,10 +60,6 @@ int f32_rows(int N) { return N <= 64 ? (N + 3) / 4 * 4 : (N + 15) / 16 * 16; }
 
 }  // namespace
 
-namespace bpg {
-int f32_perm_tile(int pfit, int NS);
-}
-
 struct bpg_plan {
     bpg_ctx* ctx = nullptr;
     bpg_study_desc d{};
@@ -86,7 +82,7 @@ struct bpg_plan {
     DBuf<float> Y32, G32;
     DBuf<unsigned short> perm;
     DBuf<float> sign;
-    DBuf<unsigned> maxbits;
+    DBuf<float> partial, maxout;
     DBuf<float> stats;
     DBuf<unsigned> bits;
     DBuf<unsigned> parent, sizes, maxsize, labels;
@@ -97,7 +93,7 @@ struct bpg_plan {
         Yc.release(); d_comp2grid.release(); d_grid2comp.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
-        perm.release(); sign.release(); maxbits.release(); stats.release(); bits.release();
+        perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
         parent.release(); sizes.release(); maxsize.release(); labels.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
         d_obs.release(); sorted.release(); P.release();
@@ -304,8 +300,11 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     const int NS = f32_rows(N);
     const int PP = pow2ceil(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
-    const int n_sub = bf ? (int)((K * PP + 31) / 32) : 0;
-    const int PT = bf ? 1 : bpg::f32_perm_tile(pfit, NS);
+    const bpg::Bf16Launch Lb = bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad) : bpg::Bf16Launch{};
+    const bpg::F32Launch Lf = bf ? bpg::F32Launch{} : bpg::plan_contract_f32((int)K, pfit, Vpad);
+    const int n_sub = Lb.n_sub;
+    const int n_part = bf ? Lb.n_vchunks : Lf.n_vchunks;
+    const int PT = bf ? 1 : bpg::f32_perm_tile(pfit);
     const long Kpad = bf ? K : (K + PT - 1) / PT * PT;
 
     // buffers
@@ -322,7 +321,8 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     }
     if (m.sign_operand) BPG_HIP(ctx, pl->sign.ensure((size_t)K * N));
     else BPG_HIP(ctx, pl->perm.ensure((size_t)K * N));
-    BPG_HIP(ctx, pl->maxbits.ensure(K));
+    BPG_HIP(ctx, pl->partial.ensure((size_t)n_part * K));
+    BPG_HIP(ctx, pl->maxout.ensure(K));
     if (stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
@@ -360,13 +360,12 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
             BPG_HIP(ctx, bpg::launch_operand_f32(oa, st));
         }
     }
-    BPG_HIP(ctx, hipMemsetAsync(pl->maxbits.p, 0, K * sizeof(unsigned), st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[2], st));
     int launches = 0;
     {
         bpg::ContractArgs ca{};
         ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p;
-        ca.yy = pl->yy.p; ca.maxbits = pl->maxbits.p;
+        ca.yy = pl->yy.p; ca.partial = pl->partial.p;
         ca.stats = stats ? pl->stats.p : nullptr;
         ca.bits = cluster ? pl->bits.p : nullptr;
         ca.nwords = pl->nwords;
@@ -375,11 +374,14 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
         ca.sqrt_dof = (float)std::sqrt((double)m.dof);
         ca.f_scale = (float)((double)m.dof / (double)m.r);
         ca.cdt = d.cdt;
-        if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, KK, PP, m.sign_operand, st, &launches));
-        else BPG_HIP(ctx, bpg::launch_contract_f32(ca, NS, st, &launches));
+        if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
+        else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
+        launches = 1;
+        if (!cluster) BPG_HIP(ctx, bpg::launch_reduce_max(pl->partial.p, n_part, (int)K, pl->maxout.p, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[3], st));
     std::vector<unsigned> host(K);
+    std::vector<float> hmax(K);
     if (cluster) {
         // batches of maps, each with a grid-sized parent / size array
         const size_t budget = (size_t)2 << 30;
@@ -407,7 +409,7 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
         BPG_HIP(ctx, hipMemcpyAsync(host.data(), pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     } else {
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
-        BPG_HIP(ctx, hipMemcpyAsync(host.data(), pl->maxbits.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipMemcpyAsync(hmax.data(), pl->maxout.p, K * sizeof(float), hipMemcpyDeviceToHost, st));
     }
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -416,12 +418,12 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     for (long k = 0; k < K; k++) {
         if (cluster) {
             maxdist[k] = (float)host[k];  // CalculateLargestCluster, LIB:18453-18456
-        } else if (Vc == 0 || host[k] == 0u) {
+        } else if (Vc == 0 || !(hmax[k] > -INFINITY)) {
             maxdist[k] = empty_max();
         } else {
             // trunc(max * 1e4) / 1e4 (KM:299, LIB:10988); trunc is monotone,
             // so it commutes with the max.
-            maxdist[k] = (float)((float)quantize(bpg::decode_max(host[k])) / 10000.0f);
+            maxdist[k] = (float)((float)quantize(hmax[k]) / 10000.0f);
         }
     }
     const auto t1 = std::chrono::steady_clock::now();

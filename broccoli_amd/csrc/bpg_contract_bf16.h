@@ -196,25 +196,11 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
             m = fmaxf(m, __shfl_xor(m, 2, 64));
             m = fmaxf(m, __shfl_xor(m, 1, 64));
             const int k = (sub0 + r) * (32 / PP) + hl * PER + P;
-            if ((lane & 31) == 0 && k < a.K && m > -INFINITY) atomicMax(&a.maxbits[k], enc_max(m * scale));
+            // one writer per (voxel chunk, permutation): no atomics
+            if ((lane & 31) == 0 && k < a.K) a.partial[(size_t)vc * a.K + k] = m * scale;
         }
 }
 
-// Tuning knobs: the largest KK that keeps 4 / 2 operand subtiles per wave,
-// and the order of the accumulation chains within a voxel tile (0: each
-// subtile's chain then its epilogue; 1: chains interleaved, then epilogues).
-#ifndef BPG_R_SMALL
-#define BPG_R_SMALL 4  // subtiles per wave for KK <= 4 (N <= 64)
-#endif
-#ifndef BPG_R2_KK
-#define BPG_R2_KK 8
-#endif
-#ifndef BPG_TARGET_BLOCKS
-#define BPG_TARGET_BLOCKS 4096
-#endif
-#ifndef BPG_PINGPONG
-#define BPG_PINGPONG 1
-#endif
 
 template <int KK, int R, int PP, bool SIGN, bool FT>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
@@ -227,17 +213,8 @@ hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
 }
 
 template <int KK, int PP>
-hipError_t dispatch_bf16_kind(const ContractArgs& a, Bf16Launch L, bool sign, hipStream_t s) {
+hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool sign, hipStream_t s) {
     constexpr int R = (KK <= 4) ? BPG_R_SMALL : (KK <= BPG_R2_KK) ? 2 : 1;
-    L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
-    // voxel chunks: enough blocks to cover the 256 CUs several times
-    const int target_blocks = BPG_TARGET_BLOCKS;
-    int nvc = (target_blocks + L.n_mgroups - 1) / L.n_mgroups;
-    if (nvc > L.n_vt) nvc = L.n_vt;
-    if (nvc < 1) nvc = 1;
-    L.vt_per_chunk = (L.n_vt + nvc - 1) / nvc;
-    L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
-    L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
     if (sign) {
         if (a.ftest || PP != 1) return hipErrorInvalidValue;
         if constexpr (PP == 1) return run_bf16<KK, R, 1, true, false>(a, L, s);

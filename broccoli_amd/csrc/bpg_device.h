@@ -28,6 +28,9 @@ __device__ __forceinline__ unsigned enc_max(float f) {
     unsigned u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+__device__ __forceinline__ float dec_max(unsigned e) {
+    return __uint_as_float((e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e);
+}
 
 
 // Blocks are dealt round-robin to the 8 XCDs.  Renumber them so that the

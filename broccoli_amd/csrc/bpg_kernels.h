@@ -3,11 +3,26 @@
 
 #include <hip/hip_runtime.h>
 
-namespace bpg {
+// Tuning knobs of the split-BF16 contraction: operand subtiles kept per
+// wave for KK <= 4 (N <= 64) and the largest KK that keeps two, the number
+// of blocks the voxel range is split over, and ping-pong tile loads.
+#ifndef BPG_R_SMALL
+#define BPG_R_SMALL 4
+#endif
+#ifndef BPG_R2_KK
+#define BPG_R2_KK 8
+#endif
+#ifndef BPG_TARGET_BLOCKS
+#define BPG_TARGET_BLOCKS 4096
+#endif
+#ifndef BPG_MAX_VCHUNKS
+#define BPG_MAX_VCHUNKS 1024
+#endif
+#ifndef BPG_PINGPONG
+#define BPG_PINGPONG 1
+#endif
 
-// Rows of the split-BF16 operand tile: MFMA 32x32x16 subtiles.
-constexpr int kSubRows = 32;
-constexpr int kVoxTile = 32;
+namespace bpg {
 
 struct TransformArgs {
     const float* Yc;       // [N][Vc] compact, subject-major
@@ -43,7 +58,7 @@ struct ContractArgs {
     const float* G32;      // FP32 operand
     const float* Y32;      // FP32 data [NS][Vpad]
     const double* yy;      // [Vpad]
-    unsigned* maxbits;     // [K] order-preserving encoded running max
+    float* partial;        // [n_vchunks][K] per-voxel-chunk maxima of the statistic
     float* stats;          // [K][Vc] or null
     unsigned* bits;        // [K][nwords] supra-threshold bitmask or null
     int nwords;
@@ -54,14 +69,29 @@ struct ContractArgs {
     float f_scale;         // F: (N - p) / r
     float cdt;
 };
-// Grid of the split-BF16 contraction (see bpg_contract_bf16.h).
+// Grid of the split-BF16 contraction (see bpg_contract_bf16.h): blocks of
+// 4 waves x R operand subtiles (32 permutation rows each) over chunks of
+// 32-voxel tiles.
 struct Bf16Launch {
     int n_mgroups, n_vchunks, vt_per_chunk, per_xcd, n_sub, n_vt;
 };
+int bf16_rows_per_wave(int KK);
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad);
+hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
+                                bool sign_operand, hipStream_t s);
 
-hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign_operand,
-                                hipStream_t s, int* launches);
-hipError_t launch_contract_f32(const ContractArgs& a, int NS, hipStream_t s, int* launches);
+// Grid of the FP32 contraction: blocks of PTT permutations over chunks of
+// 256-voxel steps.
+struct F32Launch {
+    int n_pg, n_vchunks, vox_per_chunk, per_xcd;
+};
+int f32_perm_tile(int pfit);
+F32Launch plan_contract_f32(int K, int pfit, int Vpad);
+hipError_t launch_contract_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s);
+
+// out[k] = max over the n_part chunk maxima partial[c*K + k] (NaN-free:
+// chunks without in-mask voxels hold -inf).
+hipError_t launch_reduce_max(const float* partial, int n_part, int K, float* out, hipStream_t s);
 
 struct ObsArgs {
     const float* Yc;       // [N][Vc]
@@ -104,8 +134,5 @@ hipError_t launch_ccl(const CclArgs& a, hipStream_t s);
 // obs > cdt -> compact bitmask [nwords]
 hipError_t launch_threshold_bits(const float* obs, int Vc, float cdt, unsigned* bits, int nwords,
                                  hipStream_t s);
-
-// Host-side decode of the order-preserving max encoding.
-float decode_max(unsigned e);
 
 }  // namespace bpg

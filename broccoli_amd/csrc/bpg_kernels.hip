@@ -26,12 +26,6 @@
 
 namespace bpg {
 
-float decode_max(unsigned e) {
-    unsigned u = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
-    float f;
-    std::memcpy(&f, &u, sizeof f);
-    return f;
-}
 
 // ---------------------------------------------------------------------------
 // nuisance residualisation + operand split (once per statistical map)
@@ -191,12 +185,29 @@ __device__ __forceinline__ float stat_f(float num, float ss, double yyv, float f
 template <int KK>
 hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s);
 
-hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign_operand,
-                                hipStream_t s, int* launches) {
+
+int bf16_rows_per_wave(int KK) { return KK <= 4 ? BPG_R_SMALL : (KK <= BPG_R2_KK ? 2 : 1); }
+
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad) {
     Bf16Launch L{};
-    L.n_sub = (a.K * PP + 31) / 32;
-    L.n_vt = a.Vpad / 32;
-    if (launches) *launches = 1;
+    const int R = bf16_rows_per_wave(KK);
+    L.n_sub = (K * PP + 31) / 32;
+    L.n_vt = Vpad / 32;
+    L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
+    // voxel chunks: enough blocks to cover the 256 CUs several times, at
+    // most BPG_MAX_VCHUNKS partial maxima per permutation
+    int nvc = (BPG_TARGET_BLOCKS + L.n_mgroups - 1) / L.n_mgroups;
+    if (nvc > BPG_MAX_VCHUNKS) nvc = BPG_MAX_VCHUNKS;
+    if (nvc > L.n_vt) nvc = L.n_vt;
+    if (nvc < 1) nvc = 1;
+    L.vt_per_chunk = (L.n_vt + nvc - 1) / nvc;
+    L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
+    L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
+    return L;
+}
+
+hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
+                                bool sign_operand, hipStream_t s) {
     switch (KK) {
         case 1: return dispatch_bf16_pp<1>(a, L, PP, sign_operand, s);
         case 2: return dispatch_bf16_pp<2>(a, L, PP, sign_operand, s);
@@ -218,6 +229,20 @@ hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign
     return hipErrorInvalidValue;
 }
 
+// out[k] = max_c partial[c*K + k]
+__global__ __launch_bounds__(256) void k_reduce_max(const float* partial, int n_part, int K, float* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    float m = -INFINITY;
+    for (int c = 0; c < n_part; c++) m = fmaxf(m, partial[(size_t)c * K + k]);
+    out[k] = m;
+}
+
+hipError_t launch_reduce_max(const float* partial, int n_part, int K, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_max, dim3((K + 255) / 256), dim3(256), 0, s, partial, n_part, K, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // plain FP32 contraction (vector ALUs)
 // ---------------------------------------------------------------------------
@@ -225,10 +250,6 @@ hipError_t launch_contract_bf16(const ContractArgs& a, int KK, int PP, bool sign
 // permutations.  The block's operand rows sit in LDS as [s][k][i] so the
 // PTT*P values of one subject are read as broadcasts; each thread streams its
 // voxel's data one subject at a time and keeps PTT*P FP32 accumulators.
-struct F32Launch {
-    int n_pg, n_vchunks, vox_per_chunk, per_xcd;
-};
-
 template <int P>
 struct F32Tile {
     static constexpr int PTT = (64 / P) < 1 ? 1 : (64 / P);
@@ -319,33 +340,24 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
         }
     }
     __syncthreads();
-    if (tid < PTT && k0 + tid < a.K && sMax[tid] != 0u) atomicMax(&a.maxbits[k0 + tid], sMax[tid]);
+    if (tid < PTT && k0 + tid < a.K)
+        a.partial[(size_t)vc * a.K + k0 + tid] = sMax[tid] ? dec_max(sMax[tid]) : -INFINITY;
 }
 
 namespace {
 
 template <int P, bool FT>
-hipError_t run_f32(const ContractArgs& a, hipStream_t s) {
+hipError_t run_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
     using T = F32Tile<P>;
-    F32Launch L{};
-    L.n_pg = (a.K + T::PTT - 1) / T::PTT;
-    const int nvsteps = (a.Vpad + 255) / 256;
-    int nvc = (2048 + L.n_pg - 1) / L.n_pg;
-    if (nvc > nvsteps) nvc = nvsteps;
-    if (nvc < 1) nvc = 1;
-    const int steps_per = (nvsteps + nvc - 1) / nvc;
-    L.vox_per_chunk = steps_per * 256;
-    L.n_vchunks = (nvsteps + steps_per - 1) / steps_per;
-    L.per_xcd = (L.n_pg * L.n_vchunks + 7) / 8;
     const size_t shmem = (size_t)a.N * T::ROW4 * sizeof(float);
     hipLaunchKernelGGL((k_contract_f32<P, FT>), dim3(L.per_xcd * 8), dim3(256), shmem, s, a, L);
     return hipGetLastError();
 }
 
 template <bool FT>
-hipError_t dispatch_f32(const ContractArgs& a, hipStream_t s) {
+hipError_t dispatch_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
     switch (a.pfit) {
-#define BPG_P(n) case n: return run_f32<n, FT>(a, s);
+#define BPG_P(n) case n: return run_f32<n, FT>(a, L, s);
         BPG_P(1) BPG_P(2) BPG_P(3) BPG_P(4) BPG_P(5) BPG_P(6) BPG_P(7) BPG_P(8) BPG_P(9)
         BPG_P(10) BPG_P(11) BPG_P(12) BPG_P(13) BPG_P(14) BPG_P(15) BPG_P(16) BPG_P(17)
         BPG_P(18) BPG_P(19) BPG_P(20) BPG_P(21) BPG_P(22) BPG_P(23) BPG_P(24) BPG_P(25)
@@ -356,13 +368,27 @@ hipError_t dispatch_f32(const ContractArgs& a, hipStream_t s) {
 
 }  // namespace
 
-// permutations per block of the FP32 form
-int f32_perm_tile(int pfit, int /*NS*/) { return (64 / pfit) < 1 ? 1 : (64 / pfit); }
+// permutations per block of the FP32 form (F32Tile<P>::PTT)
+int f32_perm_tile(int pfit) { return (64 / pfit) < 1 ? 1 : (64 / pfit); }
 
-hipError_t launch_contract_f32(const ContractArgs& a, int NS, hipStream_t s, int* launches) {
-    (void)NS;
-    if (launches) *launches = 1;
-    return a.ftest ? dispatch_f32<true>(a, s) : dispatch_f32<false>(a, s);
+F32Launch plan_contract_f32(int K, int pfit, int Vpad) {
+    F32Launch L{};
+    const int PTT = f32_perm_tile(pfit);
+    L.n_pg = (K + PTT - 1) / PTT;
+    const int nvsteps = (Vpad + 255) / 256;
+    int nvc = (2048 + L.n_pg - 1) / L.n_pg;
+    if (nvc > BPG_MAX_VCHUNKS) nvc = BPG_MAX_VCHUNKS;
+    if (nvc > nvsteps) nvc = nvsteps;
+    if (nvc < 1) nvc = 1;
+    const int steps_per = (nvsteps + nvc - 1) / nvc;
+    L.vox_per_chunk = steps_per * 256;
+    L.n_vchunks = (nvsteps + steps_per - 1) / steps_per;
+    L.per_xcd = (L.n_pg * L.n_vchunks + 7) / 8;
+    return L;
+}
+
+hipError_t launch_contract_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
+    return a.ftest ? dispatch_f32<true>(a, L, s) : dispatch_f32<false>(a, L, s);
 }
 
 // ---------------------------------------------------------------------------
