@@ -159,8 +159,6 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
         BPG_FAIL(ctx, BPG_ERR_ARG, "unknown statistical test");
     if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS)
         BPG_FAIL(ctx, BPG_ERR_ARG, "unknown inference mode");
-    if (d.inference == BPG_CLUSTER_MASS)
-        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "cluster-mass inference is not built");
     if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16)
         BPG_FAIL(ctx, BPG_ERR_ARG, "unknown precision");
     if (d.connectivity != 26 && d.connectivity != 6) BPG_FAIL(ctx, BPG_ERR_ARG, "connectivity must be 26 or 6");
@@ -295,6 +293,10 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     hipStream_t st = pl->stream;
     const bool bf = d.precision == BPG_PREC_SPLIT_BF16;
     const bool cluster = d.inference != BPG_VOXEL;
+    // cluster mass needs the statistic of every supra-threshold voxel: the
+    // contraction then also writes the statistic values
+    const bool mass = d.inference == BPG_CLUSTER_MASS;
+    const bool want_stats = stats != nullptr || mass;
     const int pfit = m.pfit;
     const int KK = (N + 15) / 16;
     const int NS = f32_rows(N);
@@ -323,7 +325,7 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     else BPG_HIP(ctx, pl->perm.ensure((size_t)K * N));
     BPG_HIP(ctx, pl->partial.ensure((size_t)n_part * K));
     BPG_HIP(ctx, pl->maxout.ensure(K));
-    if (stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
+    if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
@@ -366,7 +368,7 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
         bpg::ContractArgs ca{};
         ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p;
         ca.yy = pl->yy.p; ca.partial = pl->partial.p;
-        ca.stats = stats ? pl->stats.p : nullptr;
+        ca.stats = want_stats ? pl->stats.p : nullptr;
         ca.bits = cluster ? pl->bits.p : nullptr;
         ca.nwords = pl->nwords;
         ca.K = (int)K; ca.Vc = Vc; ca.Vpad = Vpad; ca.N = N; ca.pfit = pfit; ca.r = m.r; ca.NS = NS;
@@ -400,6 +402,7 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
             cc.sizes = pl->sizes.p;
             cc.maxsize = pl->maxsize.p + k0;
             cc.labels = nullptr;
+            cc.values = mass ? pl->stats.p + (size_t)k0 * Vc : nullptr;
             cc.W = d.W; cc.H = d.H; cc.D = d.D; cc.Vc = Vc; cc.nwords = pl->nwords;
             cc.nmaps = (int)std::min<long>(B, K - k0);
             cc.connectivity = d.connectivity;
@@ -417,7 +420,8 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     BPG_HIP(ctx, hipStreamSynchronize(st));
     for (long k = 0; k < K; k++) {
         if (cluster) {
-            maxdist[k] = (float)host[k];  // CalculateLargestCluster, LIB:18453-18456
+            // CalculateLargestCluster, LIB:18453-18460: extent, or mass / 1e4
+            maxdist[k] = mass ? (float)host[k] / 10000.0f : (float)host[k];
         } else if (Vc == 0 || !(hmax[k] > -INFINITY)) {
             maxdist[k] = empty_max();
         } else {
@@ -514,6 +518,7 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
             cc.bits = pl->bits.p; cc.comp2grid = pl->d_comp2grid.p; cc.grid2comp = pl->d_grid2comp.p;
             cc.parent = pl->parent.p; cc.sizes = pl->sizes.p; cc.maxsize = pl->maxsize.p;
             cc.labels = pl->labels.p;
+            cc.values = d.inference == BPG_CLUSTER_MASS ? obs : nullptr;
             cc.W = d.W; cc.H = d.H; cc.D = d.D; cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = 1;
             cc.connectivity = d.connectivity;
             BPG_HIP(ctx, bpg::launch_ccl(cc, st));

@@ -118,7 +118,9 @@ hipError_t launch_pvalues(const PvalArgs& a, hipStream_t s);
 
 // Connected components of the supra-threshold voxels of `nmaps` maps, given
 // as compact-order bitmasks [nmaps][nwords].  parent/sizes are grid-sized
-// per map slot ([nmaps][G]); only active voxels are touched.
+// per map slot ([nmaps][G]); only active voxels are touched.  sizes[root]
+// counts voxels, or with `values` sums (uint)(T * 10000.0f) over the
+// component (cluster mass); maxsize[map] is the largest.
 struct CclArgs {
     const unsigned* bits;       // [nmaps][nwords]
     const unsigned* comp2grid;  // [Vc]
@@ -127,6 +129,7 @@ struct CclArgs {
     unsigned* sizes;            // [nmaps][G]
     unsigned* maxsize;          // [nmaps]
     unsigned* labels;           // grid labels of map 0 (or null)
+    const float* values;        // [nmaps][Vc] statistic values (cluster mass) or null
     int W, H, D, Vc, nwords, nmaps, connectivity;
 };
 hipError_t launch_ccl(const CclArgs& a, hipStream_t s);

@@ -549,7 +549,9 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
                     }
         } else if (stage == 2) {
             const unsigned root = uf_find(parent, g);
-            atomicAdd(&sizes[root], 1u);
+            // extent (KC:403-430) or mass (KC:432-458)
+            const unsigned add = a.values ? (unsigned)(a.values[(size_t)map * a.Vc + c] * 10000.0f) : 1u;
+            atomicAdd(&sizes[root], add);
             if (a.labels && map == 0) a.labels[g] = root;
         } else {
             if (parent[g] == g) atomicMax(&a.maxsize[map], sizes[g]);

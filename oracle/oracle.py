@@ -54,6 +54,9 @@ def lib() -> ctypes.CDLL:
         _lib.orc_quantize.argtypes = [c_float]
         _lib.orc_ccl.restype = c_uint
         _lib.orc_ccl.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_float, c_int, c_void_p, c_void_p]
+        _lib.orc_cluster_mass.restype = c_uint
+        _lib.orc_cluster_mass.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_float, c_int, c_void_p,
+                                          c_void_p]
         _lib.orc_stirling_factorial.restype = c_double
         _lib.orc_max_permutations.restype = c_long
         _lib.orc_pvalues.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -147,6 +150,17 @@ def ccl(stat, mask, thr, connectivity=26):
     mk = np.ascontiguousarray(mask, np.float32)
     best = lib().orc_ccl(W, H, D, P(st), P(mk), c_float(thr), connectivity, P(lab), P(sz))
     return lab, sz, int(best)
+
+
+def cluster_mass(stat, mask, thr, labels):
+    """Per-label mass sum of (uint)(T*1e4) over supra-threshold voxels and
+    the largest one (KC:432-458)."""
+    D, H, W = mask.shape
+    ms = np.zeros(mask.size, np.uint32)
+    st = np.ascontiguousarray(stat, np.float32)
+    mk = np.ascontiguousarray(mask, np.float32)
+    best = lib().orc_cluster_mass(W, H, D, P(st), P(mk), c_float(thr), 26, P(labels), P(ms))
+    return ms, int(best)
 
 
 def max_quantized(stat, mask):

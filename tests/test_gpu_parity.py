@@ -158,3 +158,33 @@ def test_group_mean_full_path_matches_oracle(ctx):
     diff = np.abs(P[0][m] - P64[0][m]) * K
     # counts agree except where a quantised max differs (ties are reported)
     assert diff.max() <= (q64 != qg).sum()
+
+
+def test_cluster_mass_null_and_pvalues(ctx):
+    """Cluster mass (KC:432-458): sum of (uint)(T*1e4) over the component;
+    the null is the largest mass / 1e4, bit-exact against the oracle on the
+    GPU's statistic values."""
+    st = synth.study(3, scale=0.25)
+    st["inference"] = capi.CLUSTER_MASS
+    st["cdt"] = 2.5
+    K = 24
+    perms = synth.permutations(st, K, seed=17)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        obs = plan.observed_maps()
+        P, labels = plan.pvalues([md], want_labels=True)
+    for k in range(K):
+        m = scatter(st, stats[k])
+        lab, _, _ = oracle.ccl(m, st["mask"], st["cdt"], 26)
+        _, best = oracle.cluster_mass(m, st["mask"], st["cdt"], lab)
+        assert md[k] == np.float32(best) / np.float32(10000.0), k
+    om = obs[0].reshape(st["mask"].shape)
+    lab, _, _ = oracle.ccl(om, st["mask"], st["cdt"], 26)
+    assert np.array_equal(labels[0], lab)
+    masses, _ = oracle.cluster_mass(om, st["mask"], st["cdt"], lab)
+    D, H, W = st["mask"].shape
+    Pref = np.zeros(W * H * D, np.float32)
+    oracle.lib().orc_pvalues(2, W, H, D, oracle.P(obs[0]), oracle.P(st["mask"]), oracle.P(lab),
+                             oracle.P(masses), st["cdt"], K, oracle.P(md), oracle.P(Pref))
+    assert np.array_equal(P[0], Pref)
+    assert (P[0] > 0).any() or md.max() > 0

@@ -344,6 +344,26 @@ def test_ccl_matches_scipy_label(conn):
     assert len(np.unique(lab[lab != 3 * vol.size])) == n
 
 
+def test_cluster_mass_matches_scipy_label_sums():
+    """Mass of a component = sum of (uint)(T*1e4) over its voxels
+    (KC:432-458), checked on scipy's own labelling with ndimage.sum."""
+    rng = np.random.default_rng(11)
+    vol = (3 * ndimage.gaussian_filter(rng.standard_normal((15, 18, 21)), 1.5)).astype(np.float32)
+    mask = (rng.random(vol.shape) < 0.95).astype(np.float32)
+    thr = 0.3
+    lab, _, _ = oracle.ccl(vol, mask, thr, 26)
+    ms, best = oracle.cluster_mass(vol, mask, thr, lab)
+    act = (vol > thr) & (mask == 1)
+    ref, n = ndimage.label(act, structure=ndimage.generate_binary_structure(3, 3))
+    q = np.where(act, (vol * np.float32(1e4)).astype(np.uint32), 0).astype(np.int64)
+    sums = ndimage.sum(q, ref, index=np.arange(1, n + 1)).astype(np.int64)
+    assert n > 3 and best == sums.max()
+    # every component's mass sits at its label
+    for r in range(1, n + 1):
+        first = np.flatnonzero(ref.ravel() == r)[0]
+        assert ms[lab[first]] == sums[r - 1]
+
+
 def test_ccl_diagonal_contacts_separate_6_and_26():
     vol = np.zeros((5, 5, 5), np.float32)
     vol[0, 0, 0] = vol[1, 1, 1] = vol[2, 2, 2] = 1.0  # corner-to-corner chain
