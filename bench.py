@@ -72,18 +72,15 @@ def run_form(ctx, st, perms, precision, steps, warmup):
     return plan, md, dt, float(np.mean(kern)), tm
 
 
-def mfma_busy(st, K, kern_ms, clock_hz=2.4e9, simds=1024):
+def mfma_busy(st, K, kern_ms, slots, clock_hz=2.4e9, simds=1024):
     """Share of the MFMA pipes' cycles the launch keeps busy: 32x32x16 BF16
     MFMAs issued (32 cycles each; 2 split products for sign operands, 3
-    otherwise; subjects padded to 16, regressors to a power of two) over
-    the launch time at the peak clock."""
+    otherwise; subjects padded to 16, 16 // slots permutations per lane
+    and half-subtile) over the launch time at the peak clock."""
     N, V = st["N"], st["V"]
-    pfit = 1 if st["test"] == capi.GROUP_MEAN else st["p"]
-    pp = 1
-    while pp < pfit:
-        pp *= 2
     kk = (N + 15) // 16
-    n_sub = (K * pp + 31) // 32
+    per_sub = 2 * (16 // slots)
+    n_sub = (K + per_sub - 1) // per_sub
     n_vt = (V + 255) // 256 * 8
     prods = 2 if st["test"] == capi.GROUP_MEAN else 3
     return n_sub * n_vt * kk * prods * 32 / (kern_ms * 1e-3 * clock_hz * simds)
@@ -130,6 +127,20 @@ def cpu_baseline(st, perms, target_s):
                        f"{dt:.1f} s, one full-grid pass + quantised max per permutation")
 
 
+def copy_calibration(mib):
+    """A known-size read + write (torch's elementwise negation: reads and
+    writes mib MiB once each) for scaling the traffic counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(f"# copy calibration: {mib} MiB read + {mib} MiB written per launch", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +155,9 @@ def main():
                     help="2 (default, the metric's workload); 1: sign-flip group mean, K=1000; "
                          "3: F test with cluster-extent inference; 5: 182x218x182 grid, N=200, p=8")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = full size)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, launch a plain elementwise copy of this many MiB three times "
+                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the permutation phase is measured on a single "
@@ -182,7 +196,8 @@ def main():
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
                      "kernel": "k_contract_bf16", "algorithmic_flop_per_launch": flops,
                      "avg_launch_ms": kern_ms,
-                     "mfma_pipe_busy_est": mfma_busy(st, K, kern_ms)},
+                     "mfma_pipe_busy_est": mfma_busy(st, K, kern_ms, tm.slots_per_perm)},
+        "fit_columns": tm.fit_columns,
     }
     if not args.no_fp32:
         plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, args.steps // 2), 1)
@@ -198,6 +213,8 @@ def main():
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(st, perms, args.cpu_seconds)
     ctx.close()
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
     print(json.dumps(res))
 
 

@@ -48,7 +48,8 @@ class StudyDesc(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("phase_ms", c_double), ("transform_ms", c_double), ("operand_ms", c_double),
                 ("contract_ms", c_double), ("cluster_ms", c_double),
-                ("contract_launches", c_int), ("perm_voxels", c_long)]
+                ("contract_launches", c_int), ("perm_voxels", c_long),
+                ("fit_columns", c_int), ("slots_per_perm", c_int)]
 
 
 class BpgError(RuntimeError):

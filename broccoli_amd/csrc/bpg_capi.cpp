@@ -55,6 +55,12 @@ int pow2ceil(int x) {
     return r;
 }
 
+// Accumulator slots per permutation in the split-BF16 layout: a lane holds 16
+// slots, i.e. 16/PP whole permutations.  3 and 5 regressors fit 5 and 3
+// permutations per lane unpadded (15 of 16 slots); other counts round up to
+// a power of two, which fits as many permutations as any value between.
+int bf16_slots(int pfit) { return (pfit == 3 || pfit == 5) ? pfit : pow2ceil(pfit); }
+
 // FP32 operand row count for N subjects: multiples of 4 up to 64, then of 16.
 int f32_rows(int N) { return N <= 64 ? (N + 3) / 4 * 4 : (N + 15) / 16 * 16; }
 
@@ -300,7 +306,7 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     const int pfit = m.pfit;
     const int KK = (N + 15) / 16;
     const int NS = f32_rows(N);
-    const int PP = pow2ceil(pfit);
+    const int PP = bf16_slots(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
     const bpg::Bf16Launch Lb = bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad) : bpg::Bf16Launch{};
     const bpg::F32Launch Lf = bf ? bpg::F32Launch{} : bpg::plan_contract_f32((int)K, pfit, Vpad);
@@ -439,6 +445,8 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     tm.cluster_ms = cluster ? elapsed(pl->ev[3], pl->ev[4]) : 0.0;
     tm.contract_launches = launches;
     tm.perm_voxels = K * (long)Vc;
+    tm.fit_columns = pfit;
+    tm.slots_per_perm = bf ? PP : pfit;
     return BPG_OK;
 }
 

@@ -9,7 +9,8 @@
 // layout with voxels as columns.  Accumulator: lane l holds column l%32 and
 // rows 8*b + 4*(l/32) + a at slot 4*b + a; the operand rows are ordered so
 // slots j = P*PP .. P*PP+PP-1 of a lane are the PP regressors of one
-// permutation, so each lane finishes 16/PP whole permutations of one voxel.
+// permutation, so each lane finishes PER = 16/PP whole permutations of one
+// voxel (PP = 3 or 5 leave slot 15 as a zero row).
 //
 // Epilogue per (permutation, voxel), with z = Qt' P' y* (orthonormal basis,
 // first axes spanning the contrast):
@@ -73,7 +74,7 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
         if (OUT) {
             const bool valid = v < a.Vc;
             const float scale = FT ? a.f_scale : a.sqrt_dof;
-            const int kbase = (sub0 + r) * (32 / PP) + hl * PER;
+            const int kbase = (sub0 + r) * (2 * PER) + hl * PER;
 #pragma unroll
             for (int P = 0; P < PER; P++) {
                 const float st = u[P] * scale;
@@ -121,6 +122,96 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 #endif
 }
 
+#if BPG_CHAIN_MODE == 2
+// Software-pipelined form: the MFMA chain of subtile i is issued together
+// with the epilogue of subtile i-1 (which may belong to the previous voxel
+// tile), and sched_group_barrier interleaves them one MFMA to a few VALU
+// instructions, so the matrix pipe and the VALU work at the same time.
+template <int KK, bool SIGN>
+__device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&al)[KK], const bf16x8 (&bh)[KK],
+                                        const bf16x8 (&bl)[KK]) {
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KK; kk++) {
+        if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[kk], bh[kk], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kk], bl[kk], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kk], bh[kk], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// statistic + running max of one subtile (global subtile index st) of the
+// voxel tile vt; vt < 0 marks the empty item before the first tile
+template <int PP, bool FT, bool OUT>
+__device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, float yhi, float ylo, int vt,
+                                    int lane, int st, float (&rm)[16 / PP]) {
+    constexpr int PER = 16 / PP;
+    float u[PER];
+#pragma unroll
+    for (int P = 0; P < PER; P++) {
+        const float z0 = acc[P * PP];
+        float ss = z0 * z0, num = ss;
+#pragma unroll
+        for (int g = 1; g < PP; g++) {
+            const float z = acc[P * PP + g];
+            ss = fmaf(z, z, ss);
+            if (FT) num = (g < a.r) ? ss : num;
+        }
+        const float sse = (yhi - ss) + ylo;
+        u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
+        rm[P] = fmaxf(rm[P], u[P]);
+    }
+    if (OUT) {
+        const int hl = lane >> 5;
+        const int v = vt * 32 + (lane & 31);
+        const bool valid = vt >= 0 && v < a.Vc;
+        const float scale = FT ? a.f_scale : a.sqrt_dof;
+        const int kbase = st * (2 * PER) + hl * PER;
+#pragma unroll
+        for (int P = 0; P < PER; P++) {
+            const float stv = u[P] * scale;
+            const int k = kbase + P;
+            if (a.stats && valid && k < a.K) a.stats[(size_t)k * a.Vc + v] = stv;
+            if (a.bits) {
+                const unsigned long long bal = __ballot(valid && stv > a.cdt);
+                if ((lane & 31) == 0 && k < a.K && vt >= 0)
+                    a.bits[(size_t)k * a.nwords + vt] = hl ? (unsigned)(bal >> 32) : (unsigned)bal;
+            }
+        }
+    }
+}
+
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT>
+__device__ __forceinline__ void contract_tile_pipe(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
+                                                   const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
+                                                   const bf16x8 (&bl)[KK], double yv, int vt, int lane, int sub0,
+                                                   float (&runmax)[R][16 / PP], f32x16& pend, float& pyhi,
+                                                   float& pylo, int& pvt) {
+    constexpr int NM = KK * (SIGN ? 2 : 3);
+    constexpr int PER = 16 / PP;
+    constexpr int NV = (PER * (PP + 5) + NM - 1) / NM;
+    const float yhi = (float)yv;  // NaN for padding voxels
+    const float ylo = (float)(yv - (double)yhi);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
+        if (r == 0) epi<PP, FT, OUT>(a, pend, pyhi, pylo, pvt, lane, sub0 + R - 1, runmax[R - 1]);
+        else epi<PP, FT, OUT>(a, pend, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1]);
+#pragma unroll
+        for (int i = 0; i < NM; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
+        }
+        pend = acc;
+    }
+    pyhi = yhi;
+    pylo = ylo;
+    pvt = vt;
+}
+#endif
+
 // Block = 4 waves, each with R resident operand subtiles; the block walks a
 // chunk of voxel tiles, loading tile t+1 (ping-pong registers, straight from
 // L2; the 4 waves share the lines through the CU's L1) while tile t is
@@ -164,7 +255,28 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
 
     const int vt0 = vc * L.vt_per_chunk;
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
-#if BPG_PINGPONG
+#if BPG_CHAIN_MODE == 2
+    // pipeline state: the accumulator awaiting its epilogue, with its tile's
+    // norms; the first item is empty (NaN norm: dropped by fmaxf, no output)
+    f32x16 pend;
+#pragma unroll
+    for (int e = 0; e < 16; e++) pend[e] = 0.0f;
+    float pyhi = __builtin_nanf(""), pylo = 0.0f;
+    int pvt = -1;
+    bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
+    double y0 = 0.0, y1 = 0.0;
+    if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
+    for (int vt = vt0; vt < vt1; vt += 2) {
+        if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
+        contract_tile_pipe<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax, pend, pyhi,
+                                                     pylo, pvt);
+        if (vt + 1 >= vt1) break;
+        if (vt + 2 < vt1) load_tile<KK>(Y, a.yy, vt + 2, lane, b0h, b0l, y0);
+        contract_tile_pipe<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax, pend,
+                                                     pyhi, pylo, pvt);
+    }
+    epi<PP, FT, OUT>(a, pend, pyhi, pylo, pvt, lane, sub0 + R - 1, runmax[R - 1]);
+#elif BPG_PINGPONG
     bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
     double y0 = 0.0, y1 = 0.0;
     if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
@@ -195,7 +307,7 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
             m = fmaxf(m, __shfl_xor(m, 4, 64));
             m = fmaxf(m, __shfl_xor(m, 2, 64));
             m = fmaxf(m, __shfl_xor(m, 1, 64));
-            const int k = (sub0 + r) * (32 / PP) + hl * PER + P;
+            const int k = (sub0 + r) * (2 * PER) + hl * PER + P;
             // one writer per (voxel chunk, permutation): no atomics
             if ((lane & 31) == 0 && k < a.K) a.partial[(size_t)vc * a.K + k] = m * scale;
         }
@@ -227,7 +339,9 @@ hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, 
     switch (PP) {
         case 1: return dispatch_bf16_kind<KK, 1>(a, L, sign, s);
         case 2: return dispatch_bf16_kind<KK, 2>(a, L, sign, s);
+        case 3: return dispatch_bf16_kind<KK, 3>(a, L, sign, s);
         case 4: return dispatch_bf16_kind<KK, 4>(a, L, sign, s);
+        case 5: return dispatch_bf16_kind<KK, 5>(a, L, sign, s);
         case 8: return dispatch_bf16_kind<KK, 8>(a, L, sign, s);
         case 16: return dispatch_bf16_kind<KK, 16>(a, L, sign, s);
     }

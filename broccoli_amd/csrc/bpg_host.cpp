@@ -116,6 +116,43 @@ std::vector<double> to_double(const float* a, size_t n) {
     return r;
 }
 
+// The constant vector is the one direction every row permutation leaves in
+// place (P'1 = 1).  When it lies in the nuisance space, the residualised
+// data are orthogonal to it, so its coordinate 1'P'y*/sqrt(N) is zero for
+// every permutation and contributes nothing to |z|^2.  The nuisance columns
+// of Qt (which span the same space as Qn) are then rotated so that one of
+// them is 1/sqrt(N), and that column is dropped from the permuted fit; the
+// degrees of freedom stay N - p.
+void drop_invariant_axis(int N, int p, MapModel& m) {
+    const int r = m.r, nn = p - r;
+    if (nn < 1) return;
+    const double e = 1.0 / std::sqrt((double)N);
+    // coordinates of 1/sqrt(N) in the nuisance columns of Qt
+    std::vector<double> a(nn);
+    double in = 0.0;
+    for (int j = 0; j < nn; j++) {
+        double acc = 0.0;
+        for (int s = 0; s < N; s++) acc += m.Qt[s + (size_t)(r + j) * N] * e;
+        a[j] = acc;
+        in += acc * acc;
+    }
+    if (!(std::fabs(in - 1.0) <= 1e-10)) return;  // not in the nuisance space
+    // orthogonal U with first column +-a: the rotated columns Qt_n U keep
+    // the span, their first one is +-1/sqrt(N)
+    std::vector<double> U((size_t)nn * nn);
+    householder_qr(nn, 1, a.data(), U.data(), nullptr);
+    std::vector<double> Qt((size_t)N * (p - 1), 0.0);
+    std::copy(m.Qt.begin(), m.Qt.begin() + (size_t)N * r, Qt.begin());
+    for (int c = 1; c < nn; c++)
+        for (int s = 0; s < N; s++) {
+            double acc = 0.0;
+            for (int j = 0; j < nn; j++) acc += m.Qt[s + (size_t)(r + j) * N] * U[j + (size_t)c * nn];
+            Qt[s + (size_t)(r + c - 1) * N] = acc;
+        }
+    m.Qt.swap(Qt);
+    m.pfit = p - 1;
+}
+
 }  // namespace
 
 int build_map_model(const bpg_study_desc& d, int map, MapModel& m, std::string& err) {
@@ -196,6 +233,7 @@ int build_map_model(const bpg_study_desc& d, int map, MapModel& m, std::string& 
         std::vector<double> Qn, Rn;
         if (!thin_qr(N, m.q, W2, Qn, Rn)) { err = "nuisance regressors are rank deficient"; return BPG_ERR_DESIGN; }
         m.Qn = Qn;
+        drop_invariant_axis(N, p, m);
     }
     m.dscale.assign(N, 1.0);
     m.sign_operand = false;

@@ -24,8 +24,8 @@ namespace bpg {
 struct MapModel {
     int r = 1;      // contrast rows
     int q = 0;      // nuisance dimension (0: no residualisation)
-    int pfit = 1;   // regressors of the permuted fit
-    int dof = 1;    // N - pfit
+    int pfit = 1;   // columns of Qt in the permuted fit (p, or p - 1 without the constant axis)
+    int dof = 1;    // N - p
     std::vector<double> Qt;     // N x pfit, column-major
     std::vector<double> Qn;     // N x q, column-major
     std::vector<double> dscale; // N, operand row scale

@@ -106,16 +106,17 @@ __global__ __launch_bounds__(256) void k_operand_bf16(OperandArgs a) {
     const int i = lane & 31, hl = lane >> 5;
     const int j = 4 * (i >> 3) + (i & 3);
     const int hh = (i >> 2) & 1;
-    const int per = 16 / a.PP;
-    const int k = st * (32 / a.PP) + hh * per + j / a.PP;
+    const int per = 16 / a.PP;  // whole permutations per lane; slots per*PP..15 stay zero
+    const int k = st * (2 * per) + hh * per + j / a.PP;
     const int reg = j % a.PP;
+    const bool live = j < per * a.PP && k < a.K && reg < a.pfit;
     const int s0 = kk * 16 + hl * 8;
     unsigned short out[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int s = s0 + e;
         double val = 0.0;
-        if (k < a.K && reg < a.pfit && s < a.N) {
+        if (live && s < a.N) {
             if (a.sign) val = (double)a.sign[(size_t)k * a.N + s];
             else val = a.Qt[(size_t)a.perm[(size_t)k * a.N + s] + (size_t)reg * a.N];
         }
@@ -191,7 +192,7 @@ int bf16_rows_per_wave(int KK) { return KK <= 4 ? BPG_R_SMALL : (KK <= BPG_R2_KK
 Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad) {
     Bf16Launch L{};
     const int R = bf16_rows_per_wave(KK);
-    L.n_sub = (K * PP + 31) / 32;
+    L.n_sub = (K + 2 * (16 / PP) - 1) / (2 * (16 / PP));
     L.n_vt = Vpad / 32;
     L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
     // voxel chunks: enough blocks to cover the 256 CUs several times, at
@@ -261,8 +262,11 @@ template <int P, bool FT>
 __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch L) {
     using T = F32Tile<P>;
     constexpr int PTT = T::PTT;
-    extern __shared__ float sG[];  // [N][ROW4]
-    __shared__ unsigned sMax[PTT];
+    // the float4 broadcasts need a 16-byte aligned base: the dynamic array
+    // is declared as float4 and the static one is a whole number of float4
+    extern __shared__ float4 sG4[];  // [N][ROW4] floats
+    float* sG = reinterpret_cast<float*>(sG4);
+    __shared__ unsigned sMax[(PTT + 3) / 4 * 4];
     const int tid = threadIdx.x;
     const int lb = xcd_logical_block(L.per_xcd);
     if (lb >= L.n_pg * L.n_vchunks) return;

@@ -86,6 +86,10 @@ typedef struct {
     double cluster_ms;    /* cluster labelling (cluster inference only)       */
     int contract_launches;
     long perm_voxels;     /* K * V_mask of the phase                          */
+    int fit_columns;      /* columns of the permuted fit: p, or p - 1 when the
+                             constant vector lies in the nuisance space (it is
+                             permutation-invariant and carries no signal)     */
+    int slots_per_perm;   /* split-BF16 accumulator rows per permutation      */
 } bpg_timing;
 
 /* ---- context: replaces BROCCOLI_LIB(platform, device, ...) +

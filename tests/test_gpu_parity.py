@@ -188,3 +188,33 @@ def test_cluster_mass_null_and_pvalues(ctx):
                              oracle.P(masses), st["cdt"], K, oracle.P(md), oracle.P(Pref))
     assert np.array_equal(P[0], Pref)
     assert (P[0] > 0).any() or md.max() > 0
+
+
+@pytest.mark.parametrize("variant,fit_cols", [("intercept_nuisance", 3), ("intercept_contrast", 4),
+                                              ("no_intercept", 3), ("group_only", 1)])
+def test_constant_axis_in_and_out_of_the_nuisance_space(ctx, variant, fit_cols):
+    """The constant vector is dropped from the permuted fit only when it lies
+    in the nuisance space (there it is permutation-invariant and orthogonal
+    to the residualised data); every variant matches the FP64 oracle."""
+    st = synth.study(2, scale=0.15)
+    st["inference"] = capi.VOXEL
+    if variant == "intercept_contrast":
+        st["C"] = np.array([[1, 0, 0, 0]], np.float32)
+    elif variant == "no_intercept":
+        st["X"] = np.ascontiguousarray(st["X"][:, 1:])
+        st["C"] = np.array([[1, 0, 0]], np.float32)
+        st["p"] = 3
+    elif variant == "group_only":
+        st["X"] = np.ascontiguousarray(st["X"][:, :2])
+        st["C"] = np.array([[0, 1]], np.float32)
+        st["p"] = 2
+    K = 40
+    perms = synth.permutations(st, K, seed=23)
+    for prec in ("split_bf16", "fp32"):
+        with capi.Plan(ctx, st, prec) as plan:
+            md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+            assert plan.timing().fit_columns == fit_cols
+        ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+        err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+        assert err.max() <= TOL, (prec, err.max())
+        check_maxT(md, ref, stats)
