@@ -49,6 +49,31 @@ struct DBuf {
     }
 };
 
+// pinned host staging: the per-phase uploads and downloads run as true
+// asynchronous DMA, and the host fills / reads them while kernels run
+template <typename T>
+struct HBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+            n = 0;
+        }
+        if (want == 0) want = 1;
+        const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 int pow2ceil(int x) {
     int r = 1;
     while (r < x) r <<= 1;
@@ -81,8 +106,7 @@ struct bpg_plan {
     bool have_obs = false;
 
     DBuf<float> Yc;
-    DBuf<unsigned> d_comp2grid;
-    DBuf<int> d_grid2comp;
+    DBuf<int> d_nbr;
     DBuf<double> yy, Qt, Qn, dscale;
     DBuf<unsigned> Yb, Gb;
     DBuf<float> Y32, G32;
@@ -94,9 +118,13 @@ struct bpg_plan {
     DBuf<unsigned> parent, sizes, maxsize, labels;
     DBuf<double> oX, oXp, oC, osc;
     DBuf<float> d_obs, sorted, P;
+    HBuf<double> h_model;
+    HBuf<unsigned short> h_perm;
+    HBuf<float> h_sign, h_max;
 
     void release_all() {
-        Yc.release(); d_comp2grid.release(); d_grid2comp.release();
+        h_model.release(); h_perm.release(); h_sign.release(); h_max.release();
+        Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
@@ -224,14 +252,32 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
     if (hipStreamCreate(&pl->stream) != hipSuccess) { ctx->err = "hipStreamCreate failed"; return fail(BPG_ERR_HIP); }
     for (auto& e : pl->ev)
         if (hipEventCreate(&e) != hipSuccess) { ctx->err = "hipEventCreate failed"; return fail(BPG_ERR_HIP); }
-    if (pl->Yc.ensure(yc.size()) || pl->d_comp2grid.ensure(std::max(pl->Vc, 1)) ||
-        pl->d_grid2comp.ensure(pl->G)) {
+    // cluster inference: compact indices of the 13 lower-index 26-neighbours
+    // of every in-mask voxel (face neighbours first), -1 outside the mask
+    std::vector<int> nbr;
+    if (d.inference != BPG_VOXEL) {
+        static const int off[bpg::BPG_NBR][3] = {
+            {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},                                   // faces
+            {-1, -1, 0}, {1, -1, 0},                                              // dz = 0 edges
+            {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1},    // dz = -1
+            {-1, 1, -1}, {0, 1, -1}, {1, 1, -1}};
+        nbr.assign((size_t)std::max(pl->Vc, 1) * bpg::BPG_NBR, -1);
+        for (int c = 0; c < pl->Vc; c++) {
+            const size_t g = pl->comp2grid[c];
+            const int x = (int)(g % d.W), y = (int)((g / d.W) % d.H), z = (int)(g / ((size_t)d.W * d.H));
+            for (int j = 0; j < bpg::BPG_NBR; j++) {
+                const int xx = x + off[j][0], yy = y + off[j][1], zz = z + off[j][2];
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= d.W || yy >= d.H || zz >= d.D) continue;
+                nbr[(size_t)c * bpg::BPG_NBR + j] = grid2comp[xx + (size_t)yy * d.W + (size_t)zz * d.W * d.H];
+            }
+        }
+    }
+    if (pl->Yc.ensure(yc.size()) || (!nbr.empty() && pl->d_nbr.ensure(nbr.size()))) {
         ctx->err = "device allocation of the study failed";
         return fail(BPG_ERR_NOMEM);
     }
     if (hipMemcpy(pl->Yc.p, yc.data(), yc.size() * sizeof(float), hipMemcpyHostToDevice) ||
-        hipMemcpy(pl->d_comp2grid.p, pl->comp2grid.data(), pl->Vc * sizeof(unsigned), hipMemcpyHostToDevice) ||
-        hipMemcpy(pl->d_grid2comp.p, grid2comp.data(), pl->G * sizeof(int), hipMemcpyHostToDevice)) {
+        (!nbr.empty() && hipMemcpy(pl->d_nbr.p, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice))) {
         ctx->err = "upload of the study failed";
         return fail(BPG_ERR_HIP);
     }
@@ -291,9 +337,6 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     const bpg::MapModel& m = pl->models[map];
     if (m.sign_operand ? !sign : !perm) BPG_FAIL(ctx, BPG_ERR_ARG, "missing permutation / sign matrix");
     const int N = d.N;
-    if (!m.sign_operand)
-        for (long i = 0; i < K * (long)N; i++)
-            if (perm[i] >= N) BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = pl->stream;
@@ -308,7 +351,7 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     const int NS = f32_rows(N);
     const int PP = bf16_slots(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
-    const bpg::Bf16Launch Lb = bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad) : bpg::Bf16Launch{};
+    const bpg::Bf16Launch Lb = bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad, m.sign_operand) : bpg::Bf16Launch{};
     const bpg::F32Launch Lf = bf ? bpg::F32Launch{} : bpg::plan_contract_f32((int)K, pfit, Vpad);
     const int n_sub = Lb.n_sub;
     const int n_part = bf ? Lb.n_vchunks : Lf.n_vchunks;
@@ -334,11 +377,21 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
+    // pinned staging (free at entry: every phase ends with a stream sync)
+    const size_t nQt = m.Qt.size(), nQn = m.q > 0 ? m.Qn.size() : 0;
+    BPG_HIP(ctx, pl->h_model.ensure(nQt + nQn + N));
+    BPG_HIP(ctx, pl->h_max.ensure(K));
+    if (m.sign_operand) BPG_HIP(ctx, pl->h_sign.ensure((size_t)K * N));
+    else BPG_HIP(ctx, pl->h_perm.ensure((size_t)K * N));
+    double* hm = pl->h_model.p;
+    std::memcpy(hm, m.Qt.data(), nQt * sizeof(double));
+    if (nQn) std::memcpy(hm + nQt, m.Qn.data(), nQn * sizeof(double));
+    std::memcpy(hm + nQt + nQn, m.dscale.data(), N * sizeof(double));
+
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
-    BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, m.Qt.data(), m.Qt.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    if (m.q > 0)
-        BPG_HIP(ctx, hipMemcpyAsync(pl->Qn.p, m.Qn.data(), m.Qn.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, m.dscale.data(), N * sizeof(double), hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, hm, nQt * sizeof(double), hipMemcpyHostToDevice, st));
+    if (nQn) BPG_HIP(ctx, hipMemcpyAsync(pl->Qn.p, hm + nQt, nQn * sizeof(double), hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, hm + nQt + nQn, N * sizeof(double), hipMemcpyHostToDevice, st));
     {
         bpg::TransformArgs ta{};
         ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;
@@ -348,11 +401,28 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
         BPG_HIP(ctx, bpg::launch_transform(ta, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
+    // while the transform runs: stage (and check) the permutations
+    if (m.sign_operand) {
+        std::memcpy(pl->h_sign.p, sign, (size_t)K * N * sizeof(float));
+    } else {
+        unsigned short* hp = pl->h_perm.p;
+        unsigned short worst = 0;
+        const size_t total = (size_t)K * N;
+        for (size_t i = 0; i < total; i++) {
+            hp[i] = perm[i];
+            worst = std::max(worst, perm[i]);
+        }
+        if (worst >= N) {
+            (void)hipStreamSynchronize(st);  // the staged model copies must finish
+            BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
+        }
+    }
     {
         if (m.sign_operand)
-            BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p, sign, (size_t)K * N * sizeof(float), hipMemcpyHostToDevice, st));
+            BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p, pl->h_sign.p, (size_t)K * N * sizeof(float), hipMemcpyHostToDevice, st));
         else
-            BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, perm, (size_t)K * N * sizeof(unsigned short), hipMemcpyHostToDevice, st));
+            BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, pl->h_perm.p, (size_t)K * N * sizeof(unsigned short),
+                                        hipMemcpyHostToDevice, st));
         bpg::OperandArgs oa{};
         oa.perm = m.sign_operand ? nullptr : pl->perm.p;
         oa.sign = m.sign_operand ? pl->sign.p : nullptr;
@@ -388,37 +458,39 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
         if (!cluster) BPG_HIP(ctx, bpg::launch_reduce_max(pl->partial.p, n_part, (int)K, pl->maxout.p, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[3], st));
-    std::vector<unsigned> host(K);
-    std::vector<float> hmax(K);
+    // both results are 4-byte words: the cluster maxima are uint sizes /
+    // masses, the voxel maxima floats
+    unsigned* host = reinterpret_cast<unsigned*>(pl->h_max.p);
+    const float* hmax = pl->h_max.p;
     if (cluster) {
-        // batches of maps, each with a grid-sized parent / size array
-        const size_t budget = (size_t)2 << 30;
-        long B = (long)std::max<size_t>(1, budget / (pl->G * 2 * sizeof(unsigned)));
+        // batches of maps, each with a compact parent / size array
+        const size_t budget = (size_t)8 << 30;
+        const size_t per_map = (size_t)std::max(Vc, 1) * 2 * sizeof(unsigned);
+        long B = (long)std::max<size_t>(1, budget / per_map);
         B = std::min<long>(B, K);
-        BPG_HIP(ctx, pl->parent.ensure((size_t)B * pl->G));
-        BPG_HIP(ctx, pl->sizes.ensure((size_t)B * pl->G));
+        BPG_HIP(ctx, pl->parent.ensure((size_t)B * std::max(Vc, 1)));
+        BPG_HIP(ctx, pl->sizes.ensure((size_t)B * std::max(Vc, 1)));
         BPG_HIP(ctx, pl->maxsize.ensure(K));
         BPG_HIP(ctx, hipMemsetAsync(pl->maxsize.p, 0, K * sizeof(unsigned), st));
         for (long k0 = 0; k0 < K; k0 += B) {
             bpg::CclArgs cc{};
             cc.bits = pl->bits.p + (size_t)k0 * pl->nwords;
-            cc.comp2grid = pl->d_comp2grid.p;
-            cc.grid2comp = pl->d_grid2comp.p;
+            cc.nbr = pl->d_nbr.p;
             cc.parent = pl->parent.p;
             cc.sizes = pl->sizes.p;
             cc.maxsize = pl->maxsize.p + k0;
             cc.labels = nullptr;
             cc.values = mass ? pl->stats.p + (size_t)k0 * Vc : nullptr;
-            cc.W = d.W; cc.H = d.H; cc.D = d.D; cc.Vc = Vc; cc.nwords = pl->nwords;
+            cc.Vc = Vc; cc.nwords = pl->nwords;
             cc.nmaps = (int)std::min<long>(B, K - k0);
             cc.connectivity = d.connectivity;
             BPG_HIP(ctx, bpg::launch_ccl(cc, st));
         }
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
-        BPG_HIP(ctx, hipMemcpyAsync(host.data(), pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipMemcpyAsync(host, pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     } else {
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
-        BPG_HIP(ctx, hipMemcpyAsync(hmax.data(), pl->maxout.p, K * sizeof(float), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->h_max.p, pl->maxout.p, K * sizeof(float), hipMemcpyDeviceToHost, st));
     }
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -504,12 +576,13 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
     BPG_HIP(ctx, pl->P.ensure(Vc));
     if (cluster) {
         BPG_HIP(ctx, pl->bits.ensure(pl->nwords));
-        BPG_HIP(ctx, pl->parent.ensure(pl->G));
-        BPG_HIP(ctx, pl->sizes.ensure(pl->G));
-        BPG_HIP(ctx, pl->labels.ensure(pl->G));
+        BPG_HIP(ctx, pl->parent.ensure(Vc));
+        BPG_HIP(ctx, pl->sizes.ensure(Vc));
+        BPG_HIP(ctx, pl->labels.ensure(Vc));
         BPG_HIP(ctx, pl->maxsize.ensure(1));
     }
     std::vector<float> h(Vc);
+    std::vector<unsigned> hl(cluster && labels ? Vc : 0);
     for (int c = 0; c < pl->nmaps; c++) {
         if (K[c] <= 0 || !maxdist[c]) BPG_FAIL(ctx, BPG_ERR_ARG, "empty null distribution");
         std::vector<float> srt(maxdist[c], maxdist[c] + K[c]);
@@ -520,14 +593,15 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
         if (cluster) {
             // ClusterizeOpenCL on the observed map (LIB:16822, LIB:18301-18401)
             BPG_HIP(ctx, bpg::launch_threshold_bits(obs, Vc, d.cdt, pl->bits.p, pl->nwords, st));
-            BPG_HIP(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(pl->labels.p), (int)(pl->G * 3), pl->G, st));
+            // compact roots; 0xFFFFFFFF marks voxels at or below the threshold
+            BPG_HIP(ctx, hipMemsetAsync(pl->labels.p, 0xFF, Vc * sizeof(unsigned), st));
             BPG_HIP(ctx, hipMemsetAsync(pl->maxsize.p, 0, sizeof(unsigned), st));
             bpg::CclArgs cc{};
-            cc.bits = pl->bits.p; cc.comp2grid = pl->d_comp2grid.p; cc.grid2comp = pl->d_grid2comp.p;
+            cc.bits = pl->bits.p; cc.nbr = pl->d_nbr.p;
             cc.parent = pl->parent.p; cc.sizes = pl->sizes.p; cc.maxsize = pl->maxsize.p;
             cc.labels = pl->labels.p;
             cc.values = d.inference == BPG_CLUSTER_MASS ? obs : nullptr;
-            cc.W = d.W; cc.H = d.H; cc.D = d.D; cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = 1;
+            cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = 1;
             cc.connectivity = d.connectivity;
             BPG_HIP(ctx, bpg::launch_ccl(cc, st));
         }
@@ -535,15 +609,18 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
         pa.obs = obs; pa.sorted_max = pl->sorted.p;
         pa.labels = cluster ? pl->labels.p : nullptr;
         pa.sizes = cluster ? pl->sizes.p : nullptr;
-        pa.comp2grid = pl->d_comp2grid.p;
         pa.K = (int)K[c]; pa.Vc = Vc; pa.inference = d.inference; pa.cdt = d.cdt; pa.P = pl->P.p;
         BPG_HIP(ctx, bpg::launch_pvalues(pa, st));
         BPG_HIP(ctx, hipMemcpyAsync(h.data(), pl->P.p, Vc * sizeof(float), hipMemcpyDeviceToHost, st));
         if (cluster && labels)
-            BPG_HIP(ctx, hipMemcpyAsync(labels + (size_t)c * pl->G, pl->labels.p, pl->G * sizeof(unsigned),
-                                        hipMemcpyDeviceToHost, st));
+            BPG_HIP(ctx, hipMemcpyAsync(hl.data(), pl->labels.p, Vc * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         BPG_HIP(ctx, hipStreamSynchronize(st));
         for (int v = 0; v < Vc; v++) pvalues[(size_t)c * pl->G + pl->comp2grid[v]] = h[v];
+        // grid labels: the component's minimum linear index (its root's grid
+        // index), 3*W*H*D elsewhere
+        if (cluster && labels)
+            for (int v = 0; v < Vc; v++)
+                if (hl[v] != 0xFFFFFFFFu) labels[(size_t)c * pl->G + pl->comp2grid[v]] = pl->comp2grid[hl[v]];
     }
     return BPG_OK;
 }

@@ -19,13 +19,14 @@
 // The running max keeps u; the positive constant sqrt(N-p) (t) or
 // (N-p)/r (F) is applied once after the max, which is exact: rounding is
 // monotone, so max(fl(c*u)) = fl(c*max(u)).
+//
+// Two schedules: sequential (each subtile's MFMA chain, then its epilogue)
+// and software-pipelined (the chain of subtile i issued together with the
+// epilogue of subtile i-1, which may belong to the previous voxel tile,
+// interleaved one MFMA to a few VALU instructions by sched_group_barrier).
 #pragma once
 
 #include "bpg_device.h"
-
-#ifndef BPG_CHAIN_MODE
-#define BPG_CHAIN_MODE 0
-#endif
 
 namespace bpg {
 
@@ -41,92 +42,8 @@ __device__ __forceinline__ void load_tile(const bf16x8* Y, const double* yy, int
     yv = yy[vt * 32 + (lane & 31)];
 }
 
-// One voxel tile: for each resident subtile, its MFMA chain and then its
-// statistic (the epilogue of subtile r overlaps the MFMAs of subtile r+1).
-// OUT: also write the statistic of every voxel (stats) and/or the
-// supra-threshold bitmask (bits); the max-T-only form carries no branches.
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT>
-__device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
-                                              const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
-                                              const bf16x8 (&bl)[KK], double yv, int vt, int lane,
-                                              int sub0, float (&runmax)[R][16 / PP]) {
-    constexpr int PER = 16 / PP;
-    const float yhi = (float)yv;  // NaN for padding voxels
-    const float ylo = (float)(yv - (double)yhi);
-    const int hl = lane >> 5;
-    const int v = vt * 32 + (lane & 31);
-    auto epilogue = [&](const f32x16& acc, int r) {
-        float u[PER];
-#pragma unroll
-        for (int P = 0; P < PER; P++) {
-            const float z0 = acc[P * PP];
-            float ss = z0 * z0, num = ss;
-#pragma unroll
-            for (int g = 1; g < PP; g++) {
-                const float z = acc[P * PP + g];
-                ss = fmaf(z, z, ss);
-                if (FT) num = (g < a.r) ? ss : num;
-            }
-            const float sse = (yhi - ss) + ylo;
-            u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
-            runmax[r][P] = fmaxf(runmax[r][P], u[P]);
-        }
-        if (OUT) {
-            const bool valid = v < a.Vc;
-            const float scale = FT ? a.f_scale : a.sqrt_dof;
-            const int kbase = (sub0 + r) * (2 * PER) + hl * PER;
-#pragma unroll
-            for (int P = 0; P < PER; P++) {
-                const float st = u[P] * scale;
-                const int k = kbase + P;
-                if (a.stats && valid && k < a.K) a.stats[(size_t)k * a.Vc + v] = st;
-                if (a.bits) {
-                    const unsigned long long bal = __ballot(valid && st > a.cdt);
-                    if ((lane & 31) == 0 && k < a.K)
-                        a.bits[(size_t)k * a.nwords + vt] = hl ? (unsigned)(bal >> 32) : (unsigned)bal;
-                }
-            }
-        }
-    };
-#if BPG_CHAIN_MODE == 1
-    // the R accumulation chains interleaved, then the R epilogues
-    f32x16 acc[R];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int e = 0; e < 16; e++) acc[r][e] = 0.0f;
-#pragma unroll
-    for (int kk = 0; kk < KK; kk++)
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (!SIGN) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], bh[kk], acc[r], 0, 0, 0);
-            acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bl[kk], acc[r], 0, 0, 0);
-            acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bh[kk], acc[r], 0, 0, 0);
-        }
-#pragma unroll
-    for (int r = 0; r < R; r++) epilogue(acc[r], r);
-#else
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        f32x16 acc;
-#pragma unroll
-        for (int e = 0; e < 16; e++) acc[e] = 0.0f;
-#pragma unroll
-        for (int kk = 0; kk < KK; kk++) {
-            if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], bh[kk], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bl[kk], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], bh[kk], acc, 0, 0, 0);
-        }
-        epilogue(acc, r);
-    }
-#endif
-}
-
-#if BPG_CHAIN_MODE == 2
-// Software-pipelined form: the MFMA chain of subtile i is issued together
-// with the epilogue of subtile i-1 (which may belong to the previous voxel
-// tile), and sched_group_barrier interleaves them one MFMA to a few VALU
-// instructions, so the matrix pipe and the VALU work at the same time.
+// z for one subtile and one voxel tile: lo*hi + hi*lo + hi*hi (the lo*hi
+// product vanishes for exact +-1 operands)
 template <int KK, bool SIGN>
 __device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&al)[KK], const bf16x8 (&bh)[KK],
                                         const bf16x8 (&bl)[KK]) {
@@ -142,8 +59,11 @@ __device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&a
     return acc;
 }
 
-// statistic + running max of one subtile (global subtile index st) of the
-// voxel tile vt; vt < 0 marks the empty item before the first tile
+// Statistic and running max of global subtile st over voxel tile vt; vt < 0
+// marks the empty item that opens the pipelined schedule (its NaN norm is
+// dropped by fmaxf and it writes nothing).  OUT: also write the statistic
+// of every voxel (stats) and/or the supra-threshold bitmask (bits); the
+// max-T-only form carries no branches.
 template <int PP, bool FT, bool OUT>
 __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, float yhi, float ylo, int vt,
                                     int lane, int st, float (&rm)[16 / PP]) {
@@ -183,41 +103,54 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
     }
 }
 
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT>
-__device__ __forceinline__ void contract_tile_pipe(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
-                                                   const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
-                                                   const bf16x8 (&bl)[KK], double yv, int vt, int lane, int sub0,
-                                                   float (&runmax)[R][16 / PP], f32x16& pend, float& pyhi,
-                                                   float& pylo, int& pvt) {
-    constexpr int NM = KK * (SIGN ? 2 : 3);
-    constexpr int PER = 16 / PP;
-    constexpr int NV = (PER * (PP + 5) + NM - 1) / NM;
+// pipeline state: the accumulator awaiting its epilogue and its tile's norms
+struct Pending {
+    f32x16 acc;
+    float yhi, ylo;
+    int vt;
+};
+
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+__device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
+                                              const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
+                                              const bf16x8 (&bl)[KK], double yv, int vt, int lane, int sub0,
+                                              float (&runmax)[R][16 / PP], Pending& pd) {
     const float yhi = (float)yv;  // NaN for padding voxels
     const float ylo = (float)(yv - (double)yhi);
+    if constexpr (!PIPE) {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
-        if (r == 0) epi<PP, FT, OUT>(a, pend, pyhi, pylo, pvt, lane, sub0 + R - 1, runmax[R - 1]);
-        else epi<PP, FT, OUT>(a, pend, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1]);
-#pragma unroll
-        for (int i = 0; i < NM; i++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
+        for (int r = 0; r < R; r++) {
+            const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
+            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r]);
         }
-        pend = acc;
+    } else {
+        constexpr int NM = KK * (SIGN ? 2 : 3);
+        constexpr int PER = 16 / PP;
+        constexpr int NV = (PER * (PP + 5) + NM - 1) / NM;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
+            if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1]);
+            else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1]);
+#pragma unroll
+            for (int i = 0; i < NM; i++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);  // NV VALU
+            }
+            pd.acc = acc;
+        }
+        pd.yhi = yhi;
+        pd.ylo = ylo;
+        pd.vt = vt;
     }
-    pyhi = yhi;
-    pylo = ylo;
-    pvt = vt;
 }
-#endif
 
 // Block = 4 waves, each with R resident operand subtiles; the block walks a
 // chunk of voxel tiles, loading tile t+1 (ping-pong registers, straight from
 // L2; the 4 waves share the lines through the CU's L1) while tile t is
 // multiplied.  At the end, a 32-lane butterfly per permutation and one
-// atomic per permutation per wave.
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT>
+// partial maximum per (voxel chunk, permutation).
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
 __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
     constexpr int PER = 16 / PP;
     const int lane = threadIdx.x & 63;
@@ -253,48 +186,27 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
 #pragma unroll
         for (int P = 0; P < PER; P++) runmax[r][P] = -INFINITY;
 
+    Pending pd;
+#pragma unroll
+    for (int e = 0; e < 16; e++) pd.acc[e] = 0.0f;
+    pd.yhi = __builtin_nanf("");
+    pd.ylo = 0.0f;
+    pd.vt = -1;
+
     const int vt0 = vc * L.vt_per_chunk;
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
-#if BPG_CHAIN_MODE == 2
-    // pipeline state: the accumulator awaiting its epilogue, with its tile's
-    // norms; the first item is empty (NaN norm: dropped by fmaxf, no output)
-    f32x16 pend;
-#pragma unroll
-    for (int e = 0; e < 16; e++) pend[e] = 0.0f;
-    float pyhi = __builtin_nanf(""), pylo = 0.0f;
-    int pvt = -1;
     bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
     double y0 = 0.0, y1 = 0.0;
     if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
     for (int vt = vt0; vt < vt1; vt += 2) {
         if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
-        contract_tile_pipe<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax, pend, pyhi,
-                                                     pylo, pvt);
+        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax, pd);
         if (vt + 1 >= vt1) break;
         if (vt + 2 < vt1) load_tile<KK>(Y, a.yy, vt + 2, lane, b0h, b0l, y0);
-        contract_tile_pipe<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax, pend,
-                                                     pyhi, pylo, pvt);
+        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax, pd);
     }
-    epi<PP, FT, OUT>(a, pend, pyhi, pylo, pvt, lane, sub0 + R - 1, runmax[R - 1]);
-#elif BPG_PINGPONG
-    bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
-    double y0 = 0.0, y1 = 0.0;
-    if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
-    for (int vt = vt0; vt < vt1; vt += 2) {
-        if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
-        contract_tile<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax);
-        if (vt + 1 >= vt1) break;
-        if (vt + 2 < vt1) load_tile<KK>(Y, a.yy, vt + 2, lane, b0h, b0l, y0);
-        contract_tile<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax);
-    }
-#else
-    for (int vt = vt0; vt < vt1; vt++) {
-        bf16x8 bh[KK], bl[KK];
-        double yv;
-        load_tile<KK>(Y, a.yy, vt, lane, bh, bl, yv);
-        contract_tile<KK, R, PP, SIGN, FT, OUT>(a, ahi, alo, bh, bl, yv, vt, lane, sub0, runmax);
-    }
-#endif
+    if constexpr (PIPE) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1]);
+
     const int hl = lane >> 5;
     const float scale = FT ? a.f_scale : a.sqrt_dof;
 #pragma unroll
@@ -313,25 +225,26 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
         }
 }
 
-
-template <int KK, int R, int PP, bool SIGN, bool FT>
+template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
     const int blocks = L.per_xcd * 8;
     if (a.stats || a.bits)
-        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, true>), dim3(blocks), dim3(256), 0, s, a, L);
+        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s, a, L);
     else
-        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, false>), dim3(blocks), dim3(256), 0, s, a, L);
+        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, false, PIPE>), dim3(blocks), dim3(256), 0, s, a, L);
     return hipGetLastError();
 }
 
 template <int KK, int PP>
 hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool sign, hipStream_t s) {
-    constexpr int R = (KK <= 4) ? BPG_R_SMALL : (KK <= BPG_R2_KK) ? 2 : 1;
+    constexpr int RS = bf16_rows_per_wave(KK, true);
+    constexpr int R = bf16_rows_per_wave(KK, false);
+    constexpr bool PIPE = bf16_pipelined(KK, false);
     if (sign) {
         if (a.ftest || PP != 1) return hipErrorInvalidValue;
-        if constexpr (PP == 1) return run_bf16<KK, R, 1, true, false>(a, L, s);
+        if constexpr (PP == 1) return run_bf16<KK, RS, 1, true, false, bf16_pipelined(KK, true)>(a, L, s);
     }
-    return a.ftest ? run_bf16<KK, R, PP, false, true>(a, L, s) : run_bf16<KK, R, PP, false, false>(a, L, s);
+    return a.ftest ? run_bf16<KK, R, PP, false, true, PIPE>(a, L, s) : run_bf16<KK, R, PP, false, false, PIPE>(a, L, s);
 }
 
 template <int KK>

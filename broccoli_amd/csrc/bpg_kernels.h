@@ -3,23 +3,36 @@
 
 #include <hip/hip_runtime.h>
 
-// Tuning knobs of the split-BF16 contraction: operand subtiles kept per
-// wave for KK <= 4 (N <= 64) and the largest KK that keeps two, the number
-// of blocks the voxel range is split over, and ping-pong tile loads.
+// Tuning knobs of the split-BF16 contraction:
+//  - operand subtiles kept per wave: BPG_R_SIGN for exact +-1 operands with
+//    KK <= 4 (N <= 64), BPG_R_SMALL for general operands with KK <= 4, 2
+//    up to KK = BPG_R2_KK, else 1;
+//  - BPG_PIPE_SMALL / _MID / _LARGE: software-pipelined epilogue for
+//    general operands at KK <= 4, KK <= BPG_R2_KK and above;
+//  - the number of blocks the voxel range is split over.
+#ifndef BPG_R_SIGN
+#define BPG_R_SIGN 4
+#endif
 #ifndef BPG_R_SMALL
-#define BPG_R_SMALL 4
+#define BPG_R_SMALL 3
 #endif
 #ifndef BPG_R2_KK
 #define BPG_R2_KK 8
+#endif
+#ifndef BPG_PIPE_SMALL
+#define BPG_PIPE_SMALL 1
+#endif
+#ifndef BPG_PIPE_MID
+#define BPG_PIPE_MID 0
+#endif
+#ifndef BPG_PIPE_LARGE
+#define BPG_PIPE_LARGE 1
 #endif
 #ifndef BPG_TARGET_BLOCKS
 #define BPG_TARGET_BLOCKS 4096
 #endif
 #ifndef BPG_MAX_VCHUNKS
 #define BPG_MAX_VCHUNKS 1024
-#endif
-#ifndef BPG_PINGPONG
-#define BPG_PINGPONG 1
 #endif
 
 namespace bpg {
@@ -75,8 +88,14 @@ struct ContractArgs {
 struct Bf16Launch {
     int n_mgroups, n_vchunks, vt_per_chunk, per_xcd, n_sub, n_vt;
 };
-int bf16_rows_per_wave(int KK);
-Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad);
+// operand subtiles per wave (the planner and the kernels share this rule)
+constexpr int bf16_rows_per_wave(int KK, bool sign) {
+    return KK <= 4 ? (sign ? BPG_R_SIGN : BPG_R_SMALL) : (KK <= BPG_R2_KK ? 2 : 1);
+}
+constexpr bool bf16_pipelined(int KK, bool sign) {
+    return !sign && (KK <= 4 ? BPG_PIPE_SMALL : (KK <= BPG_R2_KK ? BPG_PIPE_MID : BPG_PIPE_LARGE)) != 0;
+}
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign);
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
                                 bool sign_operand, hipStream_t s);
 
@@ -107,9 +126,8 @@ hipError_t launch_observed(const ObsArgs& a, hipStream_t s);
 struct PvalArgs {
     const float* obs;           // [Vc]
     const float* sorted_max;    // [K] ascending
-    const unsigned* labels;     // grid labels (cluster modes) or null
-    const unsigned* sizes;      // grid sizes (cluster modes) or null
-    const unsigned* comp2grid;  // [Vc]
+    const unsigned* labels;     // [Vc] compact roots (cluster modes) or null
+    const unsigned* sizes;      // [Vc] component sizes / masses at the roots, or null
     int K, Vc, inference;
     float cdt;
     float* P;                   // [Vc]
@@ -117,20 +135,24 @@ struct PvalArgs {
 hipError_t launch_pvalues(const PvalArgs& a, hipStream_t s);
 
 // Connected components of the supra-threshold voxels of `nmaps` maps, given
-// as compact-order bitmasks [nmaps][nwords].  parent/sizes are grid-sized
-// per map slot ([nmaps][G]); only active voxels are touched.  sizes[root]
-// counts voxels, or with `values` sums (uint)(T * 10000.0f) over the
-// component (cluster mass); maxsize[map] is the largest.
+// as compact-order bitmasks [nmaps][nwords].  Union-find runs on compact
+// indices (in-mask voxels in linear grid order, so the minimum compact index
+// of a component is its minimum grid index): parent/sizes are [nmaps][Vc],
+// and only active voxels are touched.  nbr holds, per compact voxel, the
+// compact indices of its 13 lower-index 26-neighbours (-1 outside the mask),
+// the 3 face neighbours first, so 6-connectivity reads the first 3.
+// sizes[root] counts voxels, or with `values` sums (uint)(T * 10000.0f) over
+// the component (cluster mass); maxsize[map] is the largest.
+constexpr int BPG_NBR = 13;
 struct CclArgs {
     const unsigned* bits;       // [nmaps][nwords]
-    const unsigned* comp2grid;  // [Vc]
-    const int* grid2comp;       // [G] (-1 outside the mask)
-    unsigned* parent;           // [nmaps][G]
-    unsigned* sizes;            // [nmaps][G]
+    const int* nbr;             // [Vc][BPG_NBR]
+    unsigned* parent;           // [nmaps][Vc]
+    unsigned* sizes;            // [nmaps][Vc]
     unsigned* maxsize;          // [nmaps]
-    unsigned* labels;           // grid labels of map 0 (or null)
+    unsigned* labels;           // compact root of every active voxel of map 0 (or null)
     const float* values;        // [nmaps][Vc] statistic values (cluster mass) or null
-    int W, H, D, Vc, nwords, nmaps, connectivity;
+    int Vc, nwords, nmaps, connectivity;
 };
 hipError_t launch_ccl(const CclArgs& a, hipStream_t s);
 

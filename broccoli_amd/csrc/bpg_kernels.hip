@@ -187,11 +187,9 @@ template <int KK>
 hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s);
 
 
-int bf16_rows_per_wave(int KK) { return KK <= 4 ? BPG_R_SMALL : (KK <= BPG_R2_KK ? 2 : 1); }
-
-Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad) {
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign) {
     Bf16Launch L{};
-    const int R = bf16_rows_per_wave(KK);
+    const int R = bf16_rows_per_wave(KK, sign);
     L.n_sub = (K + 2 * (16 / PP) - 1) / (2 * (16 / PP));
     L.n_vt = Vpad / 32;
     L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
@@ -461,8 +459,7 @@ __global__ __launch_bounds__(256) void k_pvalues(PvalArgs a) {
     float T = a.obs[v];
     if (a.inference != BPG_VOXEL) {
         if (!(T > a.cdt)) { a.P[v] = 0.0f; return; }
-        const unsigned g = a.comp2grid[v];
-        T = (float)a.sizes[a.labels[g]];
+        T = (float)a.sizes[a.labels[v]];
         if (a.inference == BPG_CLUSTER_MASS) T = T / 10000.0f;
     }
     int lo = 0, hi = a.K;
@@ -481,20 +478,23 @@ hipError_t launch_pvalues(const PvalArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// connected components (union-find on grid indices, min-index roots)
+// connected components (union-find on compact indices, min-index roots)
 // ---------------------------------------------------------------------------
 // The reference propagates the minimum label among neighbours until nothing
 // changes (KC:42-401); its fixpoint labels every component with its minimum
 // linear index.  Here each supra-threshold voxel is linked to its
 // lower-index neighbours with atomicMin unions, so every tree is rooted at
-// its minimum index and the labels are the same.
-__device__ __forceinline__ unsigned uf_find(const unsigned* p, unsigned x) {
-    unsigned px = p[x];
-    while (px != x) {
-        x = px;
-        px = p[x];
+// its minimum index and the labels are the same.  Finds halve paths: each
+// write replaces a parent by an ancestor, which concurrent finds and unions
+// tolerate.
+__device__ __forceinline__ unsigned uf_find(unsigned* p, unsigned x) {
+    for (;;) {
+        const unsigned px = p[x];
+        if (px == x) return x;
+        const unsigned ppx = p[px];
+        if (ppx != px) p[x] = ppx;
+        x = ppx;
     }
-    return x;
 }
 
 __device__ __forceinline__ void uf_union(unsigned* p, unsigned a, unsigned b) {
@@ -513,54 +513,52 @@ __device__ __forceinline__ void uf_union(unsigned* p, unsigned a, unsigned b) {
     }
 }
 
+// One thread per 32-voxel word of one map.  Stages: 0 init, 1 union with the
+// lower-index neighbours, 2 component sums (runs of equal roots within the
+// word are added with one atomic), 3 largest sum per map.
 __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)a.nmaps * a.nwords) return;
     const int map = (int)(t / a.nwords);
     const int w = (int)(t % a.nwords);
-    const size_t G = (size_t)a.W * a.H * a.D;
-    unsigned* parent = a.parent + (size_t)map * G;
-    unsigned* sizes = a.sizes + (size_t)map * G;
+    unsigned* parent = a.parent + (size_t)map * a.Vc;
+    unsigned* sizes = a.sizes + (size_t)map * a.Vc;
     const unsigned* bits = a.bits + (size_t)map * a.nwords;
+    const int nb = a.connectivity == 6 ? 3 : BPG_NBR;
     unsigned word = bits[w];
+    unsigned run_root = 0xFFFFFFFFu, run_sum = 0u, best = 0u;
     while (word) {
         const int b = __ffs(word) - 1;
         word &= word - 1;
         const int c = w * 32 + b;
         if (c >= a.Vc) break;
-        const unsigned g = a.comp2grid[c];
         if (stage == 0) {
-            parent[g] = g;
-            sizes[g] = 0u;
+            parent[c] = (unsigned)c;
+            sizes[c] = 0u;
         } else if (stage == 1) {
-            const int x = (int)(g % a.W);
-            const int y = (int)((g / a.W) % a.H);
-            const int z = (int)(g / ((size_t)a.W * a.H));
-            for (int dz = -1; dz <= 0; dz++)
-                for (int dy = -1; dy <= 1; dy++)
-                    for (int dx = -1; dx <= 1; dx++) {
-                        // lower-index neighbours only: each pair is linked once
-                        if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
-                        const int nz = (dx != 0) + (dy != 0) + (dz != 0);
-                        if (a.connectivity == 6 && nz != 1) continue;
-                        const int xx = x + dx, yy = y + dy, zz = z + dz;
-                        if (xx < 0 || yy < 0 || zz < 0 || xx >= a.W || yy >= a.H) continue;
-                        const unsigned ng = (unsigned)(xx + (size_t)yy * a.W + (size_t)zz * a.W * a.H);
-                        const int nc = a.grid2comp[ng];
-                        if (nc < 0) continue;
-                        if (!((bits[nc >> 5] >> (nc & 31)) & 1u)) continue;
-                        uf_union(parent, g, ng);
-                    }
+            const int* nc = a.nbr + (size_t)c * BPG_NBR;
+            for (int j = 0; j < nb; j++) {
+                const int n = nc[j];
+                if (n < 0 || !((bits[n >> 5] >> (n & 31)) & 1u)) continue;
+                uf_union(parent, (unsigned)c, (unsigned)n);
+            }
         } else if (stage == 2) {
-            const unsigned root = uf_find(parent, g);
+            const unsigned root = uf_find(parent, (unsigned)c);
             // extent (KC:403-430) or mass (KC:432-458)
             const unsigned add = a.values ? (unsigned)(a.values[(size_t)map * a.Vc + c] * 10000.0f) : 1u;
-            atomicAdd(&sizes[root], add);
-            if (a.labels && map == 0) a.labels[g] = root;
+            if (root != run_root) {
+                if (run_sum) atomicAdd(&sizes[run_root], run_sum);
+                run_root = root;
+                run_sum = 0u;
+            }
+            run_sum += add;
+            if (a.labels && map == 0) a.labels[c] = root;
         } else {
-            if (parent[g] == g) atomicMax(&a.maxsize[map], sizes[g]);
+            if (parent[c] == (unsigned)c) best = max(best, sizes[c]);
         }
     }
+    if (stage == 2 && run_sum) atomicAdd(&sizes[run_root], run_sum);
+    if (stage == 3 && best) atomicMax(&a.maxsize[map], best);
 }
 
 hipError_t launch_ccl(const CclArgs& a, hipStream_t s) {

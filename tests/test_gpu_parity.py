@@ -218,3 +218,18 @@ def test_constant_axis_in_and_out_of_the_nuisance_space(ctx, variant, fit_cols):
         err = np.abs(stats - ref) / (1.0 + np.abs(ref))
         assert err.max() <= TOL, (prec, err.max())
         check_maxT(md, ref, stats)
+
+
+def test_out_of_range_permutation_is_refused_and_plan_stays_usable(ctx):
+    st = synth.study(2, scale=0.1)
+    st["inference"] = capi.VOXEL
+    K = 16
+    perms = synth.permutations(st, K, seed=2)
+    bad = perms["perm"].copy()
+    bad[5, 3] = st["N"]
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        with pytest.raises(capi.BpgError, match="out of range"):
+            plan.permutation_phase(0, perm=bad)
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    assert (np.abs(stats - ref) / (1.0 + np.abs(ref))).max() <= TOL
