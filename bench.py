@@ -23,6 +23,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -39,14 +40,80 @@ PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 PEAK_FP32_TFLOPS = 157.3
 
 
+def run_batch(args):
+    """Workload 4: 64 studies, one batched call per step."""
+    studies = synth.studies_batch(64)
+    ctx = capi.Context(0)
+    plans, perms, Ks = [], [], []
+    for i, st in enumerate(studies):
+        mx = capi.max_permutations(2 if st["test"] == capi.GROUP_MEAN else 0,
+                                   capi.CORRELATION, st["N"], 0)
+        K = int(min(st["K"], mx))
+        Ks.append(K)
+        perms.append(synth.permutations(st, K, seed=20260400 + i))
+        plans.append(capi.Plan(ctx, st, "split_bf16"))
+    pv = sum(K * st["V"] for K, st in zip(Ks, studies))
+    flops = sum(2.0 * st["N"] * st["p"] * K * st["V"] for K, st in zip(Ks, studies))
+    print(f"# 64 studies ready: {pv:.3e} perm*voxels per step", file=sys.stderr, flush=True)
+    maps = [0] * len(plans)
+    for _ in range(args.warmup):
+        capi.permutation_phases(plans, maps, perms)
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        capi.permutation_phases(plans, maps, perms)
+    hip_sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    kern = sum(p.timing().contract_ms for p in plans)
+    t0 = time.perf_counter()
+    for p, d in zip(plans, perms):
+        p.permutation_phase(0, **d)
+    seq = time.perf_counter() - t0
+    print(f"# batch: {dt*1e3:.3f} ms/step (one study after another: {seq*1e3:.3f} ms)", file=sys.stderr, flush=True)
+    achieved = flops / dt / 1e12
+    res = {
+        "metric": METRIC, "value": pv / dt, "unit": "perm·voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "accumulate": "fp32", "data": "synthetic",
+        "config": {"workload": WORKLOADS[4], "studies": len(plans), "perm_voxels": pv,
+                   "V_mean": float(np.mean([st["V"] for st in studies])), "precision": "split_bf16"},
+        "one_after_another_ms": seq * 1e3,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_contract_bf16",
+                     "algorithmic_flop_per_step": flops, "sum_of_contraction_ms": kern,
+                     "note": "achieved over the wall time of the batch (the studies overlap)"},
+    }
+    for p in plans:
+        p.close()
+    ctx.close()
+    print(json.dumps(res))
+
+
 WORKLOADS = {
     1: "one-sample t by sign flipping, 91x109x91 grid, ellipsoid mask, N=20, p=1, 1000 sign flips, voxel max-T",
+    4: "64 studies on the 91x109x91 grid, own ellipsoid masks (50k-230k voxels), N~U{12..120}, p~U{1..6}, "
+       "t on the last covariate (sign flips when p=1), 5000 permutations each, voxel max-T, one batched call",
     2: "two-sample GLM t-test with covariates, 91x109x91 grid, ellipsoid mask, N=60 (30/30), p=4, "
        "contrast [0 1 0 0], 10000 row permutations, voxel max-T",
     3: "F test over 3 group dummies, 91x109x91 grid, N=100, p=6, smooth noise (FWHM 3), 10000 permutations, "
        "cluster-extent max (F>4, 26-connectivity)",
     5: "t test on a covariate, 182x218x182 grid (~1.83M voxels), N=200, p=8, 10000 row permutations, voxel max-T",
 }
+
+
+# rocprofv3 summaries of the dominant kernel per workload: HBM traffic per
+# launch from the FETCH_SIZE / WRITE_SIZE passes, scaled by a copy of known
+# size (the counters cannot be read from inside this process)
+PROFILES = {2: ROOT / "profiles" / "k_contract_bf16_two_sample_glm_10k.md"}
+
+
+def profiled_traffic(workload):
+    f = PROFILES.get(workload)
+    if f is None or not f.exists():
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch: (\d+)", f.read_text())
+    return (int(m.group(1)), str(f.relative_to(ROOT))) if m else (None, None)
 
 
 def hip_sync():
@@ -151,9 +218,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 5],
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5],
                     help="2 (default, the metric's workload); 1: sign-flip group mean, K=1000; "
-                         "3: F test with cluster-extent inference; 5: 182x218x182 grid, N=200, p=8")
+                         "3: F test with cluster-extent inference; 4: 64 studies in one batched call; "
+                         "5: 182x218x182 grid, N=200, p=8")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = full size)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, launch a plain elementwise copy of this many MiB three times "
@@ -164,6 +232,9 @@ def main():
               "MI355X (sharding over permutations is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == 4:
+        run_batch(args)
+        return
     st = synth.study(args.workload, scale=args.scale)
     K = args.K if args.workload == 2 or args.K != 10000 else st["K"]
     perms = synth.permutations(st, K, seed=20260000 + args.workload)
@@ -182,6 +253,10 @@ def main():
     value = K * V / dt
     flops = 2.0 * N * p * K * V
     achieved = flops / (kern_ms * 1e-3) / 1e12
+    traffic, traffic_src = profiled_traffic(args.workload) if args.scale == 1.0 else (None, None)
+    # the data pass read once (4 bytes per subject and voxel), the FP64 norms,
+    # the max-T array
+    algo_bytes = 4.0 * N * V + 8.0 * V + 4.0 * K
     res = {
         "metric": METRIC, "value": value, "unit": "perm·voxels/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
@@ -191,9 +266,11 @@ def main():
                    f"{WORKLOADS[args.workload]} (grid scale {args.scale})",
                    "V_mask": V, "N": N, "p": p, "K": K, "precision": "split_bf16"},
         "phase_breakdown_ms": {"transform": tm.transform_ms, "operand": tm.operand_ms,
-                               "contract": tm.contract_ms, "cluster": tm.cluster_ms},
+                               "contract": tm.contract_ms, "cluster": tm.cluster_ms,
+                               "library_call": tm.phase_ms},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": algo_bytes,
                      "kernel": "k_contract_bf16", "algorithmic_flop_per_launch": flops,
                      "avg_launch_ms": kern_ms,
                      "mfma_pipe_busy_est": mfma_busy(st, K, kern_ms, tm.slots_per_perm)},

@@ -61,6 +61,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
     build_driver()
+    build_cli()
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -79,6 +80,23 @@ def build_driver() -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     return exe
+
+
+CLI = PKG / "RandomiseGroupLevel"
+
+
+def build_cli() -> Path:
+    """The RandomiseGroupLevel command line front end (cli/), linked against
+    libbpg.so and zlib."""
+    srcs = [PKG / "cli" / "randomise_group_level.cpp", PKG / "cli" / "nifti_io.cpp"]
+    tmp = CLI.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'cli'}", *map(str, srcs),
+           "-o", str(tmp), f"-L{PKG}", "-lbpg", "-lz", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -31,7 +31,7 @@ ERRORS = {-1: "BPG_ERR_ARG", -2: "BPG_ERR_HIP", -3: "BPG_ERR_NOMEM", -4: "BPG_ER
 EXPORTS = [
     "bpg_create", "bpg_destroy", "bpg_last_error", "bpg_device_name",
     "bpg_plan_create", "bpg_plan_destroy", "bpg_plan_num_voxels",
-    "bpg_plan_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
+    "bpg_plan_permutation_phase", "bpg_plans_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
     "bpg_plan_pvalues", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
     "bpg_gen_perm_correlation", "bpg_detect_designs", "bpg_max_permutations",
     "bpg_model_precompute",
@@ -80,6 +80,7 @@ def load() -> ctypes.CDLL:
         "bpg_plan_destroy": (None, [vp]),
         "bpg_plan_num_voxels": (c_long, [vp]),
         "bpg_plan_permutation_phase": (c_int, [vp, c_int, vp, vp, c_long, vp, vp]),
+        "bpg_plans_permutation_phase": (c_int, [vp, c_int, vp, vp, vp, vp, vp]),
         "bpg_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
         "bpg_plan_observed_maps": (c_int, [vp, vp]),
         "bpg_plan_pvalues": (c_int, [vp, vp, vp, vp, vp]),
@@ -272,3 +273,31 @@ class Plan:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def permutation_phases(plans, maps, perms):
+    """The permutation phases of several studies in one call
+    (bpg_plans_permutation_phase): plans[i] with map maps[i] and perms[i], a
+    dict with "perm" or "sign".  Returns the list of maxdist arrays."""
+    n = len(plans)
+    lib = plans[0]._lib
+    P = (c_void_p * n)(*[p.handle for p in plans])
+    M = (c_int * n)(*maps)
+    keep = []
+    pp = (c_void_p * n)()
+    ss = (c_void_p * n)()
+    Ks = (c_long * n)()
+    outs = []
+    for i, d in enumerate(perms):
+        if "sign" in d:
+            a = np.ascontiguousarray(d["sign"], np.float32)
+            ss[i] = a.ctypes.data
+        else:
+            a = np.ascontiguousarray(d["perm"], np.uint16)
+            pp[i] = a.ctypes.data
+        keep.append(a)
+        Ks[i] = a.shape[0]
+        outs.append(np.zeros(a.shape[0], np.float32))
+    O = (c_void_p * n)(*[o.ctypes.data for o in outs])
+    _check(lib.bpg_plans_permutation_phase(P, n, M, pp, ss, Ks, O), plans[0].ctx.handle)
+    return outs

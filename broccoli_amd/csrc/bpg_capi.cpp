@@ -104,6 +104,14 @@ struct bpg_plan {
     hipEvent_t ev[6] = {};
     bpg_timing timing{};
     bool have_obs = false;
+    // the phase enqueued on `stream` and not yet finished
+    struct Pending {
+        bool active = false;
+        long K = 0;
+        bool cluster = false, mass = false, bf = false;
+        int pfit = 0, PP = 0, launches = 0;
+        std::chrono::steady_clock::time_point t0;
+    } pending;
 
     DBuf<float> Yc;
     DBuf<int> d_nbr;
@@ -326,14 +334,16 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
-extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned short* perm,
-                                          const float* sign, long K, float* maxdist, float* stats) {
-    if (!pl) return BPG_ERR_ARG;
+namespace {
+
+// Everything of one permutation phase up to the copies of its results into
+// pinned host memory, enqueued on the plan's stream; phase_finish waits and
+// quantises.  Between the two the host is free (to enqueue other plans).
+int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float* sign, long K, float* stats) {
     bpg_ctx* ctx = pl->ctx;
     const bpg_study_desc& d = pl->d;
     if (map < 0 || map >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
     if (K <= 0 || K > (1L << 26)) BPG_FAIL(ctx, BPG_ERR_ARG, "number of permutations out of range");
-    if (!maxdist) BPG_FAIL(ctx, BPG_ERR_ARG, "null maxdist");
     const bpg::MapModel& m = pl->models[map];
     if (m.sign_operand ? !sign : !perm) BPG_FAIL(ctx, BPG_ERR_ARG, "missing permutation / sign matrix");
     const int N = d.N;
@@ -495,7 +505,30 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[5], st));
-    BPG_HIP(ctx, hipStreamSynchronize(st));
+    auto& ps = pl->pending;
+    ps.active = true;
+    ps.K = K;
+    ps.cluster = cluster;
+    ps.mass = mass;
+    ps.bf = bf;
+    ps.pfit = pfit;
+    ps.PP = PP;
+    ps.launches = launches;
+    ps.t0 = t0;
+    return BPG_OK;
+}
+
+int phase_finish(bpg_plan* pl, float* maxdist) {
+    bpg_ctx* ctx = pl->ctx;
+    auto& ps = pl->pending;
+    if (!ps.active) BPG_FAIL(ctx, BPG_ERR_ARG, "no permutation phase in flight");
+    ps.active = false;
+    BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
+    const long K = ps.K;
+    const bool cluster = ps.cluster, mass = ps.mass;
+    const int Vc = pl->Vc;
+    const unsigned* host = reinterpret_cast<const unsigned*>(pl->h_max.p);
+    const float* hmax = pl->h_max.p;
     for (long k = 0; k < K; k++) {
         if (cluster) {
             // CalculateLargestCluster, LIB:18453-18460: extent, or mass / 1e4
@@ -510,16 +543,55 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     }
     const auto t1 = std::chrono::steady_clock::now();
     bpg_timing& tm = pl->timing;
-    tm.phase_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    tm.phase_ms = std::chrono::duration<double, std::milli>(t1 - ps.t0).count();
     tm.transform_ms = elapsed(pl->ev[0], pl->ev[1]);
     tm.operand_ms = elapsed(pl->ev[1], pl->ev[2]);
     tm.contract_ms = elapsed(pl->ev[2], pl->ev[3]);
     tm.cluster_ms = cluster ? elapsed(pl->ev[3], pl->ev[4]) : 0.0;
-    tm.contract_launches = launches;
+    tm.contract_launches = ps.launches;
     tm.perm_voxels = K * (long)Vc;
-    tm.fit_columns = pfit;
-    tm.slots_per_perm = bf ? PP : pfit;
+    tm.fit_columns = ps.pfit;
+    tm.slots_per_perm = ps.bf ? ps.PP : ps.pfit;
     return BPG_OK;
+}
+
+}  // namespace
+
+extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned short* perm,
+                                          const float* sign, long K, float* maxdist, float* stats) {
+    if (!pl) return BPG_ERR_ARG;
+    if (!maxdist) BPG_FAIL(pl->ctx, BPG_ERR_ARG, "null maxdist");
+    if (pl->pending.active) BPG_FAIL(pl->ctx, BPG_ERR_ARG, "a permutation phase of this plan is in flight");
+    const int rc = phase_enqueue(pl, map, perm, sign, K, stats);
+    if (rc != BPG_OK) return rc;
+    return phase_finish(pl, maxdist);
+}
+
+extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const int* maps,
+                                           const unsigned short* const* perms, const float* const* signs,
+                                           const long* K, float* const* maxdist) {
+    if (!plans || n <= 0 || !maps || !K || !maxdist) return BPG_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!plans[i] || !maxdist[i]) return BPG_ERR_ARG;
+        for (int j = 0; j < i; j++)
+            if (plans[j] == plans[i]) BPG_FAIL(plans[i]->ctx, BPG_ERR_ARG, "a plan appears twice in one batch");
+        if (plans[i]->pending.active)
+            BPG_FAIL(plans[i]->ctx, BPG_ERR_ARG, "a permutation phase of this plan is in flight");
+    }
+    // enqueue every study (each plan has its own stream), then collect; the
+    // first error is returned after the enqueued work has been waited for
+    int rc = BPG_OK;
+    int enq = 0;
+    for (; enq < n; enq++) {
+        rc = phase_enqueue(plans[enq], maps[enq], perms ? perms[enq] : nullptr, signs ? signs[enq] : nullptr,
+                           K[enq], nullptr);
+        if (rc != BPG_OK) break;
+    }
+    for (int i = 0; i < enq; i++) {
+        const int r = phase_finish(plans[i], maxdist[i]);
+        if (rc == BPG_OK) rc = r;
+    }
+    return rc;
 }
 
 extern "C" int bpg_plan_last_timing(const bpg_plan* pl, bpg_timing* out) {

@@ -15,6 +15,7 @@
 //   - plain FP32 on the vector ALUs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -516,6 +517,12 @@ __device__ __forceinline__ void uf_union(unsigned* p, unsigned a, unsigned b) {
 // One thread per 32-voxel word of one map.  Stages: 0 init, 1 union with the
 // lower-index neighbours, 2 component sums (runs of equal roots within the
 // word are added with one atomic), 3 largest sum per map.
+//
+// Stage 1 skips links that other links already imply (26-connectivity): the
+// voxel below (z-1) touches the other 8 voxels of the plane below, and the
+// voxel behind (y-1) touches its two x-neighbours, so when one of them is
+// active the union with it stands for the others; their own links to it
+// are made by the in-plane rules of that plane.
 __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)a.nmaps * a.nwords) return;
@@ -524,24 +531,32 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     unsigned* parent = a.parent + (size_t)map * a.Vc;
     unsigned* sizes = a.sizes + (size_t)map * a.Vc;
     const unsigned* bits = a.bits + (size_t)map * a.nwords;
-    const int nb = a.connectivity == 6 ? 3 : BPG_NBR;
     unsigned word = bits[w];
     unsigned run_root = 0xFFFFFFFFu, run_sum = 0u, best = 0u;
+    auto active = [&](int n) { return n >= 0 && ((bits[n >> 5] >> (n & 31)) & 1u); };
     while (word) {
-        const int b = __ffs(word) - 1;
+        const int c = w * 32 + __ffs(word) - 1;
         word &= word - 1;
-        const int c = w * 32 + b;
         if (c >= a.Vc) break;
         if (stage == 0) {
             parent[c] = (unsigned)c;
             sizes[c] = 0u;
         } else if (stage == 1) {
+            // neighbour order: x-1, y-1, z-1, then (x-1,y-1), (x+1,y-1), then
+            // the 8 others of the plane below
             const int* nc = a.nbr + (size_t)c * BPG_NBR;
-            for (int j = 0; j < nb; j++) {
-                const int n = nc[j];
-                if (n < 0 || !((bits[n >> 5] >> (n & 31)) & 1u)) continue;
-                uf_union(parent, (unsigned)c, (unsigned)n);
-            }
+            const int nx = nc[0], ny = nc[1], nz = nc[2];
+            if (active(nx)) uf_union(parent, (unsigned)c, (unsigned)nx);
+            const bool ay = active(ny), az = active(nz);
+            if (ay) uf_union(parent, (unsigned)c, (unsigned)ny);
+            if (az) uf_union(parent, (unsigned)c, (unsigned)nz);
+            if (a.connectivity == 6) continue;
+            if (!ay)
+                for (int j = 3; j < 5; j++)
+                    if (active(nc[j])) uf_union(parent, (unsigned)c, (unsigned)nc[j]);
+            if (!az)
+                for (int j = 5; j < BPG_NBR; j++)
+                    if (active(nc[j])) uf_union(parent, (unsigned)c, (unsigned)nc[j]);
         } else if (stage == 2) {
             const unsigned root = uf_find(parent, (unsigned)c);
             // extent (KC:403-430) or mass (KC:432-458)

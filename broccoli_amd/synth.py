@@ -125,3 +125,34 @@ def permutations(st: dict, K: int, compat: bool = False, seed: int | None = None
     if designs[0] == capi.TWOSAMPLE:
         return dict(perm=capi.gen_perm_two_sample(N, int(n1[0]), K, compat=compat, seed=seed))
     return dict(perm=capi.gen_perm_correlation(N, K, compat=compat, seed=seed))
+
+
+def studies_batch(n: int = 64, scale: float = 1.0, seed: int | None = None, K: int = 5000) -> list:
+    """Workload 4: n independent studies on the 91x109x91 grid, each with its
+    own ellipsoid mask (50k-230k voxels at full scale), N ~ U{12..120},
+    p ~ U{1..6} (intercept + Gaussian covariates), one t contrast on the last
+    covariate (p = 1: the intercept, tested by sign flipping).  The data are
+    N(0, 1); to keep host memory small the studies take the first N volumes
+    of one shared pool of 120 noise volumes."""
+    seed = 20260004 if seed is None else seed
+    rng = np.random.default_rng(seed)
+    W, H, D = _scaled(GRID_2MM, scale)
+    f = W / GRID_2MM[0]
+    pool = rng.standard_normal((120, D, H, W), dtype=np.float32)
+    out = []
+    for i in range(n):
+        s = rng.uniform(0.61, 1.0)
+        center = (45 * f + rng.uniform(-4, 4) * f, 54 * f + rng.uniform(-4, 4) * f, 45 * f + rng.uniform(-4, 4) * f)
+        semi = (36 * f * s, 44 * f * s, 34.4 * f * s)
+        mask = ellipsoid_mask(W, H, D, center, semi)
+        N = int(rng.integers(12, 121))
+        p = int(rng.integers(1, 7))
+        X = np.c_[np.ones(N), rng.standard_normal((N, p - 1))].astype(np.float32)
+        C = np.zeros((1, p), np.float32)
+        C[0, p - 1] = 1.0
+        st = dict(W=W, H=H, D=D, mask=mask, workload=4, seed=seed, N=N, p=p, nc=1, X=X, C=C, Y=pool[:N], K=K,
+                  test=capi.GROUP_MEAN if p == 1 else capi.TTEST, inference=capi.VOXEL,
+                  perm_rng=np.random.default_rng(rng.integers(1, 2**62)))
+        st["V"] = int(mask.sum())
+        out.append(st)
+    return out

@@ -122,6 +122,17 @@ long bpg_plan_num_voxels(const bpg_plan* plan); /* V_mask */
  *      statistic of every in-mask voxel (compact order), for parity checks. */
 int bpg_plan_permutation_phase(bpg_plan* plan, int map, const unsigned short* perm,
                                const float* sign, long K, float* maxdist, float* stats);
+/* ---- the permutation phases of n studies at once (no counterpart in the
+ *      reference, which analyses one study per RandomiseGroupLevel run,
+ *      RGL:44-1715): every plan's phase is enqueued on its own stream before
+ *      any is waited for, so small studies overlap on the GPU.  Per study i:
+ *      plans[i] (distinct plans), maps[i], perms[i] or signs[i] (either
+ *      array may be NULL when no study needs it), K[i], maxdist[i] as for
+ *      bpg_plan_permutation_phase.  Returns the first error; the studies
+ *      enqueued before it are still completed. */
+int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const int* maps,
+                                const unsigned short* const* perms, const float* const* signs,
+                                const long* K, float* const* maxdist);
 int bpg_plan_last_timing(const bpg_plan* plan, bpg_timing* out);
 
 /* ---- observed statistical maps on the original data and design

@@ -233,3 +233,30 @@ def test_out_of_range_permutation_is_refused_and_plan_stays_usable(ctx):
         md, stats = plan.permutation_phase(0, want_stats=True, **perms)
     ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
     assert (np.abs(stats - ref) / (1.0 + np.abs(ref))).max() <= TOL
+
+
+def test_batched_phases_equal_single_study_calls(ctx):
+    """bpg_plans_permutation_phase over several studies (mixed sign-flip and
+    GLM designs, different N, p and masks) gives, study by study, exactly
+    the null of a single-study call, which matches the FP64 oracle."""
+    studies = synth.studies_batch(6, scale=0.25, K=64)
+    plans = [capi.Plan(ctx, st, "split_bf16") for st in studies]
+    perms = []
+    for i, st in enumerate(studies):
+        K = int(min(64, capi.max_permutations(2 if st["test"] == capi.GROUP_MEAN else 0, capi.CORRELATION,
+                                              st["N"], 0)))
+        perms.append(synth.permutations(st, K, seed=100 + i))
+    try:
+        batch = capi.permutation_phases(plans, [0] * len(plans), perms)
+        for i, (pl, d) in enumerate(zip(plans, perms)):
+            single, stats = pl.permutation_phase(0, want_stats=True, **d)
+            assert np.array_equal(batch[i], single), i
+            K = len(single)
+            ref = oracle.perm_stats(studies[i], d, K)[:, vox(studies[i])]
+            assert (np.abs(stats - ref) / (1.0 + np.abs(ref))).max() <= TOL
+            check_maxT(single, ref, stats)
+        with pytest.raises(capi.BpgError, match="twice"):
+            capi.permutation_phases([plans[0], plans[0]], [0, 0], [perms[0], perms[0]])
+    finally:
+        for pl in plans:
+            pl.close()
