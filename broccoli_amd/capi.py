@@ -18,7 +18,7 @@ import os
 LIB_PATH = Path(os.environ.get("BPG_LIB", Path(__file__).resolve().with_name("libbpg.so")))
 
 TTEST, FTEST, GROUP_MEAN = 0, 1, 2
-VOXEL, CLUSTER_EXTENT, CLUSTER_MASS = 0, 1, 2
+VOXEL, CLUSTER_EXTENT, CLUSTER_MASS, TFCE = 0, 1, 2, 3
 TWOSAMPLE, CORRELATION, MEAN_DESIGN = 0, 1, 2
 PREC_FP32, PREC_SPLIT_BF16 = 0, 1
 PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16}

@@ -108,7 +108,7 @@ struct bpg_plan {
     struct Pending {
         bool active = false;
         long K = 0;
-        bool cluster = false, mass = false, bf = false;
+        bool cluster = false, mass = false, bf = false, tfce = false;
         int pfit = 0, PP = 0, launches = 0;
         std::chrono::steady_clock::time_point t0;
     } pending;
@@ -124,19 +124,22 @@ struct bpg_plan {
     DBuf<float> stats;
     DBuf<unsigned> bits;
     DBuf<unsigned> parent, sizes, maxsize, labels;
+    DBuf<float> tfce;
+    DBuf<int> nlev;
     DBuf<double> oX, oXp, oC, osc;
     DBuf<float> d_obs, sorted, P;
     HBuf<double> h_model;
     HBuf<unsigned short> h_perm;
     HBuf<float> h_sign, h_max;
+    HBuf<int> h_nlev;
 
     void release_all() {
-        h_model.release(); h_perm.release(); h_sign.release(); h_max.release();
+        h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release();
         Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
-        parent.release(); sizes.release(); maxsize.release(); labels.release();
+        parent.release(); sizes.release(); maxsize.release(); labels.release(); tfce.release(); nlev.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
         d_obs.release(); sorted.release(); P.release();
     }
@@ -199,7 +202,8 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
     const bpg_study_desc& d = *desc;
     if (d.test != BPG_TTEST && d.test != BPG_FTEST && d.test != BPG_GROUP_MEAN)
         BPG_FAIL(ctx, BPG_ERR_ARG, "unknown statistical test");
-    if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS)
+    if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS &&
+        d.inference != BPG_TFCE)
         BPG_FAIL(ctx, BPG_ERR_ARG, "unknown inference mode");
     if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16)
         BPG_FAIL(ctx, BPG_ERR_ARG, "unknown precision");
@@ -214,8 +218,6 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
         BPG_FAIL(ctx, BPG_ERR_ARG, "F contrast must have 1..min(10, p) rows");
     if (d.N <= p || d.N > BPG_MAX_SUBJECTS) BPG_FAIL(ctx, BPG_ERR_ARG, "need p < N <= 256 subjects");
     if (d.test == BPG_FTEST && d.N <= p + 1) BPG_FAIL(ctx, BPG_ERR_ARG, "F test needs N > p + 1");
-    if (d.precision == BPG_PREC_SPLIT_BF16 && p > 16)
-        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "split-BF16 form supports p <= 16; use the FP32 form");
 
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     bpg_plan* pl = new (std::nothrow) bpg_plan();
@@ -239,6 +241,11 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
         const int rc = bpg::build_obs_model(dm, pl->obs, ctx->err);
         if (rc) { delete pl; return rc; }
     }
+    // the split-BF16 layout holds at most 16 fit columns per permutation
+    // (one lane's accumulator slots); wider designs run the FP32 form
+    int max_pfit = 0;
+    for (const auto& m : pl->models) max_pfit = std::max(max_pfit, m.pfit);
+    if (d.precision == BPG_PREC_SPLIT_BF16 && max_pfit > 16) pl->d.precision = BPG_PREC_FP32;
     // compaction: in-mask voxels in linear order (mask == 1.0f, K2:1586)
     pl->G = (size_t)d.W * d.H * d.D;
     std::vector<int> grid2comp(pl->G, -1);
@@ -351,11 +358,12 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = pl->stream;
     const bool bf = d.precision == BPG_PREC_SPLIT_BF16;
-    const bool cluster = d.inference != BPG_VOXEL;
-    // cluster mass needs the statistic of every supra-threshold voxel: the
+    const bool cluster = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
+    // cluster mass and TFCE need the statistic of every voxel: the
     // contraction then also writes the statistic values
     const bool mass = d.inference == BPG_CLUSTER_MASS;
-    const bool want_stats = stats != nullptr || mass;
+    const bool tfce = d.inference == BPG_TFCE;
+    const bool want_stats = stats != nullptr || mass || tfce;
     const int pfit = m.pfit;
     const int KK = (N + 15) / 16;
     const int NS = f32_rows(N);
@@ -385,7 +393,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     BPG_HIP(ctx, pl->partial.ensure((size_t)n_part * K));
     BPG_HIP(ctx, pl->maxout.ensure(K));
     if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
-    if (cluster) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
+    if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
     // pinned staging (free at entry: every phase ends with a stream sync)
     const size_t nQt = m.Qt.size(), nQn = m.q > 0 ? m.Qn.size() : 0;
@@ -498,6 +506,56 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         }
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
         BPG_HIP(ctx, hipMemcpyAsync(host, pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    } else if (tfce && Vc > 0) {
+        // threshold-free cluster enhancement (LIB:16756-16762,
+        // LIB:18464-18508): the levels 0, 0.2846, ... (float accumulation)
+        // up to each map's quantised max; level 0 adds sqrt(size) * 0 * 0
+        // and is skipped
+        BPG_HIP(ctx, hipMemcpyAsync(pl->h_max.p, pl->maxout.p, K * sizeof(float), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipStreamSynchronize(st));
+        BPG_HIP(ctx, pl->h_nlev.ensure(K));
+        int Lmax = 0;
+        for (long k = 0; k < K; k++) {
+            const float top = pl->h_max.p[k] > -INFINITY ? (float)quantize(pl->h_max.p[k]) / 10000.0f : empty_max();
+            int L = 0;
+            for (float h = 0.0f; h <= top; h += 0.2846f) L++;
+            pl->h_nlev.p[k] = L;
+            Lmax = std::max(Lmax, L);
+        }
+        std::vector<float> levels(std::max(Lmax, 1));
+        {
+            float h = 0.0f;
+            for (int j = 0; j < Lmax; j++, h += 0.2846f) levels[j] = h;
+        }
+        BPG_HIP(ctx, pl->nlev.ensure(K));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->nlev.p, pl->h_nlev.p, K * sizeof(int), hipMemcpyHostToDevice, st));
+        const size_t budget = (size_t)8 << 30;
+        const size_t per_map = (size_t)Vc * 3 * sizeof(unsigned);
+        const long B = std::min<long>(K, (long)std::max<size_t>(1, budget / per_map));
+        BPG_HIP(ctx, pl->parent.ensure((size_t)B * Vc));
+        BPG_HIP(ctx, pl->sizes.ensure((size_t)B * Vc));
+        BPG_HIP(ctx, pl->tfce.ensure((size_t)B * Vc));
+        for (long k0 = 0; k0 < K; k0 += B) {
+            const int nb = (int)std::min<long>(B, K - k0);
+            BPG_HIP(ctx, hipMemsetAsync(pl->tfce.p, 0, (size_t)nb * Vc * sizeof(float), st));
+            bpg::CclArgs cc{};
+            cc.bits = pl->bits.p;
+            cc.nbr = pl->d_nbr.p;
+            cc.parent = pl->parent.p;
+            cc.sizes = pl->sizes.p;
+            cc.tfce = pl->tfce.p;
+            cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = nb;
+            cc.connectivity = d.connectivity;
+            for (int j = 1; j < Lmax; j++) {
+                BPG_HIP(ctx, bpg::launch_level_bits(pl->stats.p + (size_t)k0 * Vc, pl->nlev.p + k0, j, levels[j], nb,
+                                                    Vc, Vpad, pl->bits.p, pl->nwords, st));
+                cc.level = levels[j];
+                BPG_HIP(ctx, bpg::launch_ccl_tfce_level(cc, st));
+            }
+            BPG_HIP(ctx, bpg::launch_rowmax(pl->tfce.p, nb, Vc, pl->maxout.p + k0, st));
+        }
+        BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->h_max.p, pl->maxout.p, K * sizeof(float), hipMemcpyDeviceToHost, st));
     } else {
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
         BPG_HIP(ctx, hipMemcpyAsync(pl->h_max.p, pl->maxout.p, K * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -510,6 +568,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     ps.K = K;
     ps.cluster = cluster;
     ps.mass = mass;
+    ps.tfce = tfce;
     ps.bf = bf;
     ps.pfit = pfit;
     ps.PP = PP;
@@ -547,7 +606,7 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     tm.transform_ms = elapsed(pl->ev[0], pl->ev[1]);
     tm.operand_ms = elapsed(pl->ev[1], pl->ev[2]);
     tm.contract_ms = elapsed(pl->ev[2], pl->ev[3]);
-    tm.cluster_ms = cluster ? elapsed(pl->ev[3], pl->ev[4]) : 0.0;
+    tm.cluster_ms = (cluster || ps.tfce) ? elapsed(pl->ev[3], pl->ev[4]) : 0.0;
     tm.contract_launches = ps.launches;
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = ps.pfit;
@@ -641,7 +700,7 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
     hipStream_t st = pl->stream;
     const bpg_study_desc& d = pl->d;
     const int Vc = pl->Vc;
-    const bool cluster = d.inference != BPG_VOXEL;
+    const bool cluster = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
     std::fill(pvalues, pvalues + (size_t)pl->nmaps * pl->G, 0.0f);
     if (labels) std::fill(labels, labels + (size_t)pl->nmaps * pl->G, (unsigned)(pl->G * 3));
     if (Vc == 0) return BPG_OK;

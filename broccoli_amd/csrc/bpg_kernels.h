@@ -152,8 +152,19 @@ struct CclArgs {
     unsigned* maxsize;          // [nmaps]
     unsigned* labels;           // compact root of every active voxel of map 0 (or null)
     const float* values;        // [nmaps][Vc] statistic values (cluster mass) or null
+    float* tfce;                // [nmaps][Vc] TFCE sums (stage 4) or null
+    float level;                // TFCE threshold of stage 4
     int Vc, nwords, nmaps, connectivity;
 };
+// stage 4 (TFCE): tfce[map][c] += sqrt(size of c's component) * level^2
+hipError_t launch_ccl_tfce_level(const CclArgs& a, hipStream_t s);
+
+// TFCE level masks: bits[m][.] = stats[m][v] > level for maps with
+// nlev[m] > j (empty otherwise)
+hipError_t launch_level_bits(const float* stats, const int* nlev, int j, float level, int nmaps, int Vc,
+                             int Vpad, unsigned* bits, int nwords, hipStream_t s);
+// out[m] = max over v < Vc of x[m][v]
+hipError_t launch_rowmax(const float* x, int nmaps, int Vc, float* out, hipStream_t s);
 hipError_t launch_ccl(const CclArgs& a, hipStream_t s);
 
 // obs > cdt -> compact bitmask [nwords]

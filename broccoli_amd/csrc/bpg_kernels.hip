@@ -458,7 +458,9 @@ __global__ __launch_bounds__(256) void k_pvalues(PvalArgs a) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.Vc) return;
     float T = a.obs[v];
-    if (a.inference != BPG_VOXEL) {
+    // TFCE compares the statistic itself with the TFCE null, as the
+    // reference does (LIB:16824-16837)
+    if (a.inference == BPG_CLUSTER_EXTENT || a.inference == BPG_CLUSTER_MASS) {
         if (!(T > a.cdt)) { a.P[v] = 0.0f; return; }
         T = (float)a.sizes[a.labels[v]];
         if (a.inference == BPG_CLUSTER_MASS) T = T / 10000.0f;
@@ -568,8 +570,16 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
             }
             run_sum += add;
             if (a.labels && map == 0) a.labels[c] = root;
-        } else {
+        } else if (stage == 3) {
             if (parent[c] == (unsigned)c) best = max(best, sizes[c]);
+        } else {
+            // CalculateTFCEValues (KC:482-510): value = sqrt(size) * h * h,
+            // added to the voxel's sum; each operation rounded on its own
+            // (no fused multiply-add), sqrt correctly rounded
+            const float size = (float)sizes[uf_find(parent, (unsigned)c)];
+            const float value = __fmul_rn(__fmul_rn(__fsqrt_rn(size), a.level), a.level);
+            float* acc = a.tfce + (size_t)map * a.Vc + c;
+            *acc = __fadd_rn(*acc, value);
         }
     }
     if (stage == 2 && run_sum) atomicAdd(&sizes[run_root], run_sum);
@@ -586,6 +596,67 @@ hipError_t launch_ccl(const CclArgs& a, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// init, union and sums at one TFCE level, then the level's contributions
+hipError_t launch_ccl_tfce_level(const CclArgs& a, hipStream_t s) {
+    const size_t total = (size_t)a.nmaps * a.nwords;
+    if (total == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    for (int stage : {0, 1, 2, 4}) {
+        hipLaunchKernelGGL(k_ccl, dim3(blocks), dim3(256), 0, s, a, stage);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// one thread per (map, padded voxel); Vpad is a multiple of 256, so every
+// wave lies in one map and covers two whole words
+__global__ __launch_bounds__(256) void k_level_bits(const float* stats, const int* nlev, int j, float level,
+                                                    int nmaps, int Vc, int Vpad, unsigned* bits, int nwords) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nmaps * Vpad) return;
+    const int m = (int)(t / Vpad);
+    const int v = (int)(t % Vpad);
+    const bool act = v < Vc && j < nlev[m] && stats[(size_t)m * Vc + v] > level;
+    const unsigned long long bal = __ballot(act);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        const int w0 = v >> 5;
+        if (w0 < nwords) bits[(size_t)m * nwords + w0] = (unsigned)bal;
+        if (w0 + 1 < nwords) bits[(size_t)m * nwords + w0 + 1] = (unsigned)(bal >> 32);
+    }
+}
+
+hipError_t launch_level_bits(const float* stats, const int* nlev, int j, float level, int nmaps, int Vc,
+                             int Vpad, unsigned* bits, int nwords, hipStream_t s) {
+    const size_t total = (size_t)nmaps * Vpad;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_level_bits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, stats, nlev, j,
+                       level, nmaps, Vc, Vpad, bits, nwords);
+    return hipGetLastError();
+}
+
+// one block per map
+__global__ __launch_bounds__(256) void k_rowmax(const float* x, int Vc, float* out) {
+    const float* row = x + (size_t)blockIdx.x * Vc;
+    float m = -INFINITY;
+    for (int v = threadIdx.x; v < Vc; v += 256) m = fmaxf(m, row[v]);
+    __shared__ float red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+hipError_t launch_rowmax(const float* x, int nmaps, int Vc, float* out, hipStream_t s) {
+    if (nmaps <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rowmax, dim3(nmaps), dim3(256), 0, s, x, Vc, out);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void k_threshold_bits(const float* obs, int Vc, float cdt,

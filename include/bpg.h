@@ -44,6 +44,7 @@ extern "C" {
 #define BPG_VOXEL 0
 #define BPG_CLUSTER_EXTENT 1
 #define BPG_CLUSTER_MASS 2
+#define BPG_TFCE 3  /* threshold-free cluster enhancement (LIB:18464-18508) */
 /* group designs, CONST:82-83 (+ the CLI's mean design, RGL:972-976) */
 #define BPG_TWOSAMPLE 0
 #define BPG_CORRELATION 1
@@ -62,8 +63,10 @@ typedef struct bpg_plan bpg_plan;
 
 typedef struct {
     int test;         /* BPG_TTEST / BPG_FTEST / BPG_GROUP_MEAN               */
-    int inference;    /* BPG_VOXEL / BPG_CLUSTER_EXTENT / BPG_CLUSTER_MASS    */
-    int precision;    /* BPG_PREC_*                                           */
+    int inference;    /* BPG_VOXEL / BPG_CLUSTER_EXTENT / BPG_CLUSTER_MASS /
+                         BPG_TFCE                                             */
+    int precision;    /* BPG_PREC_* (split-BF16 plans of designs with more
+                         than 16 fit columns run the FP32 form)              */
     int connectivity; /* 26 (the reference's) or 6                            */
     int W, H, D;      /* grid                                                 */
     int N;            /* subjects                                             */

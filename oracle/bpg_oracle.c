@@ -826,7 +826,8 @@ void orc_pvalues(int inference, int W, int H, int D, const float* map,
     for (long i = 0; i < (long)V; i++) {
         if (mask[i] != 1.0f) { P[i] = 0.0f; continue; }
         float tv;
-        if (inference == ORC_VOXEL) tv = map[i];
+        /* TFCE compares the statistic itself with the TFCE null (LIB:16824) */
+        if (inference == ORC_VOXEL || inference == ORC_TFCE) tv = map[i];
         else {
             if (!(map[i] > thr)) { P[i] = 0.0f; continue; }
             tv = (float)sizes[labels[i]];
@@ -862,12 +863,38 @@ static int build_R(const orc_study* st, int c, double* R) {
     return rc;
 }
 
+float orc_tfce_max(int W, int H, int D, const float* map, const float* mask, int connectivity,
+                   float* tfce) {
+    size_t V = (size_t)W * H * D;
+    unsigned* lab = (unsigned*)malloc(sizeof(unsigned) * V);
+    unsigned* sz = (unsigned*)malloc(sizeof(unsigned) * V);
+    float* acc = tfce ? tfce : (float*)malloc(sizeof(float) * V);
+    for (size_t i = 0; i < V; i++) acc[i] = 0.0f;
+    const float top = orc_max_quantized(W, H, D, map, mask);
+    const float delta = 0.2846f;
+    for (float h = 0.0f; h <= top; h += delta) {
+        orc_ccl(W, H, D, map, mask, h, connectivity, lab, sz);
+        for (size_t i = 0; i < V; i++) {
+            if (mask[i] != 1.0f || lab[i] >= V) continue;
+            const float value = sqrtf((float)sz[lab[i]]) * h * h;
+            acc[i] += value;
+        }
+    }
+    const float best = orc_max_quantized(W, H, D, acc, mask);
+    free(lab);
+    free(sz);
+    if (!tfce) free(acc);
+    return best;
+}
+
 /* Statistic of one permutation map -> max statistic (A.5): quantised max
  * (voxel), largest cluster size, or largest cluster mass / 1e4. */
 static float null_statistic(const orc_study* st, const float* map, unsigned* labels,
                             unsigned* aux) {
     if (st->inference == ORC_VOXEL)
         return orc_max_quantized(st->W, st->H, st->D, map, st->mask);
+    if (st->inference == ORC_TFCE)
+        return orc_tfce_max(st->W, st->H, st->D, map, st->mask, st->connectivity, NULL);
     unsigned best = orc_ccl(st->W, st->H, st->D, map, st->mask, st->cdt,
                             st->connectivity, labels, aux);
     if (st->inference == ORC_CLUSTER_EXTENT) return (float)best;
@@ -954,6 +981,9 @@ int orc_randomise(const orc_study* st, int precision, float* maps,
                             if ((int)q > m) m = (int)q;
                         }
                     maxdist[c][k] = (float)((float)m / 10000.0f);
+                } else if (st->inference == ORC_TFCE) {
+                    for (size_t i = 0; i < V; i++) map[i] = (float)map64[i];
+                    maxdist[c][k] = null_statistic(st, map, labels, aux);
                 } else {
                     /* threshold in FP64, label on the FP64 map */
                     for (size_t i = 0; i < V; i++)
@@ -989,7 +1019,7 @@ int orc_randomise(const orc_study* st, int precision, float* maps,
     }
     for (int c = 0; c < nmaps; c++) {
         float* mc = maps + (size_t)c * V;
-        if (st->inference != ORC_VOXEL) {
+        if (st->inference == ORC_CLUSTER_EXTENT || st->inference == ORC_CLUSTER_MASS) {
             orc_ccl(W, H, D, mc, st->mask, st->cdt, st->connectivity, labels, aux);
             if (st->inference == ORC_CLUSTER_MASS)
                 orc_cluster_mass(W, H, D, mc, st->mask, st->cdt, st->connectivity, labels, aux);

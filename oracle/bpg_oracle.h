@@ -35,6 +35,7 @@ extern "C" {
 #define ORC_VOXEL 0
 #define ORC_CLUSTER_EXTENT 1
 #define ORC_CLUSTER_MASS 2
+#define ORC_TFCE 3
 #define ORC_TWOSAMPLE 0
 #define ORC_CORRELATION 1
 #define ORC_MEAN 2
@@ -95,6 +96,15 @@ unsigned orc_cluster_mass(int W, int H, int D, const float* map, const float* ma
                           float thr, int connectivity, const unsigned* labels,
                           unsigned* masses);
 
+/* Threshold-free cluster enhancement of a permuted map (LIB:16756-16762,
+ * LIB:18464-18508, KC:482-510): thresholds h = 0, 0.2846, ... (float
+ * accumulation) up to the map's quantised max; at each h the components of
+ * {T > h} (connectivity as orc_ccl) add sqrt(size) * h * h to their voxels;
+ * returns the quantised max of the sums over the mask.  tfce (optional,
+ * may be NULL) receives the per-voxel sums. */
+float orc_tfce_max(int W, int H, int D, const float* map, const float* mask, int connectivity,
+                   float* tfce);
+
 /* ---------------- p-values (KC:516-652) ---------------- */
 void orc_pvalues(int inference, int W, int H, int D, const float* map,
                  const float* mask, const unsigned* labels, const unsigned* sizes,
@@ -103,7 +113,7 @@ void orc_pvalues(int inference, int W, int H, int D, const float* map,
 /* ---------------- the whole path (LIB:13144-13376, 16589-16873) ------- */
 typedef struct {
     int test;             /* ORC_TTEST / ORC_FTEST / ORC_GROUP_MEAN */
-    int inference;        /* ORC_VOXEL / ORC_CLUSTER_EXTENT / ORC_CLUSTER_MASS */
+    int inference;        /* ORC_VOXEL / ORC_CLUSTER_EXTENT / ORC_CLUSTER_MASS / ORC_TFCE */
     int W, H, D;
     int N, p, nc;
     const float* Y;

@@ -54,6 +54,8 @@ def lib() -> ctypes.CDLL:
         _lib.orc_quantize.argtypes = [c_float]
         _lib.orc_ccl.restype = c_uint
         _lib.orc_ccl.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_float, c_int, c_void_p, c_void_p]
+        _lib.orc_tfce_max.restype = c_float
+        _lib.orc_tfce_max.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]
         _lib.orc_cluster_mass.restype = c_uint
         _lib.orc_cluster_mass.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_float, c_int, c_void_p,
                                           c_void_p]
@@ -161,6 +163,17 @@ def cluster_mass(stat, mask, thr, labels):
     mk = np.ascontiguousarray(mask, np.float32)
     best = lib().orc_cluster_mass(W, H, D, P(st), P(mk), c_float(thr), 26, P(labels), P(ms))
     return ms, int(best)
+
+
+def tfce_max(stat, mask, connectivity=26, want_values=False):
+    """Quantised max TFCE of one map (LIB:18464-18508, KC:482-510), and the
+    per-voxel sums when asked."""
+    D, H, W = mask.shape
+    st = np.ascontiguousarray(stat, np.float32)
+    mk = np.ascontiguousarray(mask, np.float32)
+    vals = np.zeros(mask.size, np.float32) if want_values else None
+    m = lib().orc_tfce_max(W, H, D, P(st), P(mk), connectivity, P(vals))
+    return (m, vals) if want_values else m
 
 
 def max_quantized(stat, mask):

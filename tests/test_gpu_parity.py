@@ -260,3 +260,62 @@ def test_batched_phases_equal_single_study_calls(ctx):
     finally:
         for pl in plans:
             pl.close()
+
+
+@pytest.mark.parametrize("N,p,nc,ftest", [(256, 25, 1, False), (40, 12, 10, True), (17, 16, 1, False)])
+def test_design_size_limits(ctx, N, p, nc, ftest):
+    """The reference's limits: up to 25 regressors (RGL:645), 10 F rows
+    (K4 cbeta[10]), and 256 subjects here.  Over 16 fit columns the
+    split-BF16 request runs the FP32 form."""
+    rng = np.random.default_rng(N + p)
+    st = synth.study(2, scale=0.1)
+    st["N"], st["p"], st["nc"] = N, p, nc
+    st["X"] = np.c_[np.ones(N), rng.standard_normal((N, p - 1))].astype(np.float32)
+    if ftest:
+        st["test"] = capi.FTEST
+        C = np.zeros((nc, p), np.float32)
+        for i in range(nc):
+            C[i, i + 1] = 1.0
+    else:
+        C = np.zeros((1, p), np.float32)
+        C[0, p - 1] = 1.0
+    st["C"] = C
+    st["Y"] = rng.standard_normal((N,) + st["mask"].shape).astype(np.float32)
+    st["inference"] = capi.VOXEL
+    K = 24
+    perms = dict(perm=capi.gen_perm_correlation(N, K, compat=False, seed=7))
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        tm = plan.timing()
+    assert tm.fit_columns == p - 1
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+    print(f"N={N} p={p} nc={nc} slots={tm.slots_per_perm} err={err.max():.2e}")
+    assert err.max() <= TOL
+    check_maxT(md, ref, stats)
+
+
+@pytest.mark.parametrize("conn", [26, 6])
+def test_tfce_null_and_voxel_pvalues(ctx, conn):
+    """TFCE (LIB:16756-16762, LIB:18464-18508, KC:482-510): the null of each
+    permutation is the quantised max TFCE of its map, bit-exact against the
+    oracle on the GPU's own statistic values; p-values compare the observed
+    statistic itself with that null, as the reference does (LIB:16824)."""
+    st = synth.study(3, scale=0.2)
+    st["inference"] = capi.TFCE
+    st["connectivity"] = conn
+    K = 20
+    perms = synth.permutations(st, K, seed=19)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        obs = plan.observed_maps()
+        P = plan.pvalues([md])
+    for k in range(K):
+        ref = oracle.tfce_max(scatter(st, stats[k]), st["mask"], conn)
+        assert md[k] == np.float32(ref), (k, md[k], ref)
+    assert md.max() > 0
+    D, H, W = st["mask"].shape
+    Pref = np.zeros(W * H * D, np.float32)
+    oracle.lib().orc_pvalues(3, W, H, D, oracle.P(obs[0]), oracle.P(st["mask"]), None, None, st["cdt"], K,
+                             oracle.P(md), oracle.P(Pref))
+    assert np.array_equal(P[0], Pref)

@@ -364,6 +364,46 @@ def test_cluster_mass_matches_scipy_label_sums():
         assert ms[lab[first]] == sums[r - 1]
 
 
+def _tfce_scipy(vol, mask, conn):
+    # thresholds 0, 0.2846, ... accumulated in float32 up to the quantised max
+    # (LIB:16756-16762, LIB:18470); scipy labels each level, sizes by bincount
+    top = np.float32(np.trunc(np.float32(vol[mask == 1].max()) * np.float32(1e4)) / np.float32(1e4))
+    acc = np.zeros(vol.shape, np.float32)
+    struct = ndimage.generate_binary_structure(3, 1 if conn == 6 else 3)
+    h = np.float32(0.0)
+    while h <= top:
+        lab, n = ndimage.label((vol > h) & (mask == 1), structure=struct)
+        sizes = np.bincount(lab.ravel()).astype(np.float32)
+        val = (np.sqrt(sizes[lab]) * h) * h
+        acc = np.where(lab > 0, (acc + val).astype(np.float32), acc)
+        h = np.float32(h + np.float32(0.2846))
+    q = np.trunc(acc[mask == 1].max() * np.float32(1e4))
+    return np.float32(np.float32(q) / np.float32(1e4)), acc
+
+
+@pytest.mark.parametrize("conn", [26, 6])
+def test_tfce_matches_scipy_levels(conn):
+    rng = np.random.default_rng(40 + conn)
+    vol = (2.5 * ndimage.gaussian_filter(rng.standard_normal((13, 15, 17)), 1.3)).astype(np.float32)
+    mask = (rng.random(vol.shape) < 0.97).astype(np.float32)
+    m, vals = oracle.tfce_max(vol, mask, conn, want_values=True)
+    ref, acc = _tfce_scipy(vol, mask, conn)
+    assert np.array_equal(vals.reshape(vol.shape)[mask == 1], acc[mask == 1])
+    assert m == ref and m > 0
+
+
+def test_tfce_uniform_cube_closed_form():
+    # one cube of n voxels at value v: sum over h_j < v of sqrt(n) h_j^2
+    vol = np.zeros((8, 8, 8), np.float32)
+    vol[2:5, 2:6, 1:4] = 1.7
+    n = 3 * 4 * 3
+    mask = np.ones_like(vol)
+    m = oracle.tfce_max(vol, mask, 26)
+    hs = 0.2846 * np.arange(6)  # 0 .. 1.423 < 1.7 <= 1.7076
+    expect = np.sqrt(n) * np.sum(hs ** 2)
+    assert abs(m - expect) <= 2e-4
+
+
 def test_ccl_diagonal_contacts_separate_6_and_26():
     vol = np.zeros((5, 5, 5), np.float32)
     vol[0, 0, 0] = vol[1, 1, 1] = vol[2, 2, 2] = 1.0  # corner-to-corner chain
