@@ -65,6 +65,7 @@ def run_batch(args):
     hip_sync()
     dt = (time.perf_counter() - t0) / args.steps
     kern = sum(p.timing().contract_ms for p in plans)
+    n_fp32 = sum(p.timing().precision_used == capi.PREC_FP32 for p in plans)
     t0 = time.perf_counter()
     for p, d in zip(plans, perms):
         p.permutation_phase(0, **d)
@@ -79,6 +80,7 @@ def run_batch(args):
         "config": {"workload": WORKLOADS[4], "studies": len(plans), "perm_voxels": pv,
                    "V_mean": float(np.mean([st["V"] for st in studies])), "precision": "split_bf16"},
         "one_after_another_ms": seq * 1e3,
+        "studies_rerun_in_fp32": int(n_fp32),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_contract_bf16",
                      "algorithmic_flop_per_step": flops, "sum_of_contraction_ms": kern,
@@ -275,6 +277,8 @@ def main():
                      "avg_launch_ms": kern_ms,
                      "mfma_pipe_busy_est": mfma_busy(st, K, kern_ms, tm.slots_per_perm)},
         "fit_columns": tm.fit_columns,
+        "precision_used": "split_bf16" if tm.precision_used == capi.PREC_SPLIT_BF16 else "fp32",
+        "ill_conditioned_waves": tm.unsafe_waves,
     }
     if not args.no_fp32:
         plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, args.steps // 2), 1)

@@ -32,6 +32,7 @@ EXPORTS = [
     "bpg_create", "bpg_destroy", "bpg_last_error", "bpg_device_name",
     "bpg_plan_create", "bpg_plan_destroy", "bpg_plan_num_voxels",
     "bpg_plan_permutation_phase", "bpg_plans_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
+    "bpg_plan_glm",
     "bpg_plan_pvalues", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
     "bpg_gen_perm_correlation", "bpg_detect_designs", "bpg_max_permutations",
     "bpg_model_precompute",
@@ -49,7 +50,8 @@ class Timing(ctypes.Structure):
     _fields_ = [("phase_ms", c_double), ("transform_ms", c_double), ("operand_ms", c_double),
                 ("contract_ms", c_double), ("cluster_ms", c_double),
                 ("contract_launches", c_int), ("perm_voxels", c_long),
-                ("fit_columns", c_int), ("slots_per_perm", c_int)]
+                ("fit_columns", c_int), ("slots_per_perm", c_int), ("precision_used", c_int),
+                ("unsafe_waves", c_int)]
 
 
 class BpgError(RuntimeError):
@@ -81,6 +83,7 @@ def load() -> ctypes.CDLL:
         "bpg_plan_num_voxels": (c_long, [vp]),
         "bpg_plan_permutation_phase": (c_int, [vp, c_int, vp, vp, c_long, vp, vp]),
         "bpg_plans_permutation_phase": (c_int, [vp, c_int, vp, vp, vp, vp, vp]),
+        "bpg_plan_glm": (c_int, [vp, vp, vp, vp, vp]),
         "bpg_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
         "bpg_plan_observed_maps": (c_int, [vp, vp]),
         "bpg_plan_pvalues": (c_int, [vp, vp, vp, vp, vp]),
@@ -251,6 +254,17 @@ class Plan:
         out = np.zeros(self.nmaps * self.G, np.float32)
         _check(self._lib.bpg_plan_observed_maps(self.handle, _ptr(out)), self.ctx.handle)
         return out.reshape(self.nmaps, self.G)
+
+    def glm(self):
+        """Betas (p, G), maps (nmaps, G), residuals (N, G), residual variances (G)."""
+        p = 1 if self.study["test"] == GROUP_MEAN else self.study["p"]
+        N = self.study["N"]
+        b = np.zeros(p * self.G, np.float32)
+        m = np.zeros(self.nmaps * self.G, np.float32)
+        r = np.zeros(N * self.G, np.float32)
+        v = np.zeros(self.G, np.float32)
+        _check(self._lib.bpg_plan_glm(self.handle, _ptr(b), _ptr(m), _ptr(r), _ptr(v)), self.ctx.handle)
+        return b.reshape(p, self.G), m.reshape(self.nmaps, self.G), r.reshape(N, self.G), v
 
     def pvalues(self, maxdists, want_labels: bool = False):
         Ks = np.array([len(m) for m in maxdists], np.int64)

@@ -109,6 +109,10 @@ struct bpg_plan {
         bool active = false;
         long K = 0;
         bool cluster = false, mass = false, bf = false, tfce = false;
+        int map = 0;
+        const unsigned short* perm = nullptr;
+        const float* sign = nullptr;
+        float* stats = nullptr;
         int pfit = 0, PP = 0, launches = 0;
         std::chrono::steady_clock::time_point t0;
     } pending;
@@ -126,22 +130,24 @@ struct bpg_plan {
     DBuf<unsigned> parent, sizes, maxsize, labels;
     DBuf<float> tfce;
     DBuf<int> nlev;
+    DBuf<unsigned> unsafe;
     DBuf<double> oX, oXp, oC, osc;
-    DBuf<float> d_obs, sorted, P;
+    DBuf<float> d_obs, sorted, P, glm_out;
     HBuf<double> h_model;
     HBuf<unsigned short> h_perm;
     HBuf<float> h_sign, h_max;
     HBuf<int> h_nlev;
+    HBuf<unsigned> h_unsafe;
 
     void release_all() {
-        h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release();
+        h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release(); h_unsafe.release();
         Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
-        parent.release(); sizes.release(); maxsize.release(); labels.release(); tfce.release(); nlev.release();
+        parent.release(); sizes.release(); maxsize.release(); labels.release(); tfce.release(); nlev.release(); unsafe.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
-        d_obs.release(); sorted.release(); P.release();
+        d_obs.release(); sorted.release(); P.release(); glm_out.release();
     }
 };
 
@@ -346,7 +352,8 @@ namespace {
 // Everything of one permutation phase up to the copies of its results into
 // pinned host memory, enqueued on the plan's stream; phase_finish waits and
 // quantises.  Between the two the host is free (to enqueue other plans).
-int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float* sign, long K, float* stats) {
+int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float* sign, long K, float* stats,
+                  bool force_fp32 = false) {
     bpg_ctx* ctx = pl->ctx;
     const bpg_study_desc& d = pl->d;
     if (map < 0 || map >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
@@ -357,7 +364,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = pl->stream;
-    const bool bf = d.precision == BPG_PREC_SPLIT_BF16;
+    const bool bf = d.precision == BPG_PREC_SPLIT_BF16 && !force_fp32;
     const bool cluster = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
     // cluster mass and TFCE need the statistic of every voxel: the
     // contraction then also writes the statistic values
@@ -392,6 +399,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     else BPG_HIP(ctx, pl->perm.ensure((size_t)K * N));
     BPG_HIP(ctx, pl->partial.ensure((size_t)n_part * K));
     BPG_HIP(ctx, pl->maxout.ensure(K));
+    BPG_HIP(ctx, pl->unsafe.ensure(1));
+    BPG_HIP(ctx, pl->h_unsafe.ensure(1));
     if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
@@ -470,6 +479,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.sqrt_dof = (float)std::sqrt((double)m.dof);
         ca.f_scale = (float)((double)m.dof / (double)m.r);
         ca.cdt = d.cdt;
+        ca.tau = bf ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+        ca.unsafe = pl->unsafe.p;
+        BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
         else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
         launches = 1;
@@ -562,9 +574,14 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     }
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->h_unsafe.p, pl->unsafe.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[5], st));
     auto& ps = pl->pending;
     ps.active = true;
+    ps.map = map;
+    ps.perm = perm;
+    ps.sign = sign;
+    ps.stats = stats;
     ps.K = K;
     ps.cluster = cluster;
     ps.mass = mass;
@@ -583,6 +600,16 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     if (!ps.active) BPG_FAIL(ctx, BPG_ERR_ARG, "no permutation phase in flight");
     ps.active = false;
     BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
+    if (ps.bf && *pl->h_unsafe.p > 0) {
+        // ill-conditioned samples: the split-BF16 z is not accurate enough
+        // for them, so the phase is repeated in the FP32 form
+        const auto t0 = ps.t0;
+        const int rc = phase_enqueue(pl, ps.map, ps.perm, ps.sign, ps.K, ps.stats, true);
+        if (rc != BPG_OK) return rc;
+        ps.t0 = t0;
+        ps.active = false;
+        BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
+    }
     const long K = ps.K;
     const bool cluster = ps.cluster, mass = ps.mass;
     const int Vc = pl->Vc;
@@ -611,6 +638,8 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = ps.pfit;
     tm.slots_per_perm = ps.bf ? ps.PP : ps.pfit;
+    tm.precision_used = ps.bf ? BPG_PREC_SPLIT_BF16 : BPG_PREC_FP32;
+    tm.unsafe_waves = (int)*pl->h_unsafe.p;
     return BPG_OK;
 }
 
@@ -661,6 +690,11 @@ extern "C" int bpg_plan_last_timing(const bpg_plan* pl, bpg_timing* out) {
 
 extern "C" int bpg_plan_observed_maps(bpg_plan* pl, float* maps) {
     if (!pl || !maps) return BPG_ERR_ARG;
+    return bpg_plan_glm(pl, nullptr, maps, nullptr, nullptr);
+}
+
+extern "C" int bpg_plan_glm(bpg_plan* pl, float* betas, float* maps, float* residuals, float* resvar) {
+    if (!pl) return BPG_ERR_ARG;
     bpg_ctx* ctx = pl->ctx;
     const bpg::ObsModel& o = pl->obs;
     BPG_HIP(ctx, hipSetDevice(ctx->device));
@@ -679,14 +713,35 @@ extern "C" int bpg_plan_observed_maps(bpg_plan* pl, float* maps) {
     a.Yc = pl->Yc.p; a.X = pl->oX.p; a.Xp = pl->oXp.p; a.C = pl->oC.p; a.sc = pl->osc.p;
     a.N = N; a.p = o.p; a.nc = o.nc; a.Vc = Vc; a.ftest = o.ftest; a.nmaps = pl->nmaps;
     a.out = pl->d_obs.p;
+    // optional outputs in one scratch block [p + N + 1][Vc]
+    const size_t extra = (size_t)(betas ? o.p : 0) + (residuals ? N : 0) + (resvar ? 1 : 0);
+    if (extra) BPG_HIP(ctx, pl->glm_out.ensure(extra * std::max(Vc, 1)));
+    {
+        float* q = pl->glm_out.p;
+        if (betas) { a.betas = q; q += (size_t)o.p * Vc; }
+        if (residuals) { a.residuals = q; q += (size_t)N * Vc; }
+        if (resvar) a.resvar = q;
+    }
     BPG_HIP(ctx, bpg::launch_observed(a, st));
-    std::vector<float> h((size_t)pl->nmaps * Vc);
-    if (Vc > 0)
+    std::vector<float> h((size_t)pl->nmaps * Vc), hx(extra * Vc);
+    if (Vc > 0) {
         BPG_HIP(ctx, hipMemcpyAsync(h.data(), pl->d_obs.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (extra)
+            BPG_HIP(ctx, hipMemcpyAsync(hx.data(), pl->glm_out.p, hx.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                        st));
+    }
     BPG_HIP(ctx, hipStreamSynchronize(st));
-    std::fill(maps, maps + (size_t)pl->nmaps * pl->G, 0.0f);
-    for (int c = 0; c < pl->nmaps; c++)
-        for (int v = 0; v < Vc; v++) maps[(size_t)c * pl->G + pl->comp2grid[v]] = h[(size_t)c * Vc + v];
+    // out-of-mask voxels are 0 in every output (K1:51-128, K1:1456-1673)
+    auto scatter = [&](float* dst, const float* src, int vols) {
+        std::fill(dst, dst + (size_t)vols * pl->G, 0.0f);
+        for (int c = 0; c < vols; c++)
+            for (int v = 0; v < Vc; v++) dst[(size_t)c * pl->G + pl->comp2grid[v]] = src[(size_t)c * Vc + v];
+    };
+    if (maps) scatter(maps, h.data(), pl->nmaps);
+    const float* q = hx.data();
+    if (betas) { scatter(betas, q, o.p); q += (size_t)o.p * Vc; }
+    if (residuals) { scatter(residuals, q, N); q += (size_t)N * Vc; }
+    if (resvar) scatter(resvar, q, 1);
     pl->have_obs = true;
     return BPG_OK;
 }

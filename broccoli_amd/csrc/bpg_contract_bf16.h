@@ -66,8 +66,9 @@ __device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&a
 // max-T-only form carries no branches.
 template <int PP, bool FT, bool OUT>
 __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, float yhi, float ylo, int vt,
-                                    int lane, int st, float (&rm)[16 / PP]) {
+                                    int lane, int st, float (&rm)[16 / PP], bool& bad) {
     constexpr int PER = 16 / PP;
+    const float tyy = a.tau * yhi;  // NaN on padding: never flagged
     float u[PER];
 #pragma unroll
     for (int P = 0; P < PER; P++) {
@@ -80,6 +81,7 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
             if (FT) num = (g < a.r) ? ss : num;
         }
         const float sse = (yhi - ss) + ylo;
+        bad |= sse < tyy;
         u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
         rm[P] = fmaxf(rm[P], u[P]);
     }
@@ -114,14 +116,14 @@ template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
 __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
                                               const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
                                               const bf16x8 (&bl)[KK], double yv, int vt, int lane, int sub0,
-                                              float (&runmax)[R][16 / PP], Pending& pd) {
+                                              float (&runmax)[R][16 / PP], Pending& pd, bool& bad) {
     const float yhi = (float)yv;  // NaN for padding voxels
     const float ylo = (float)(yv - (double)yhi);
     if constexpr (!PIPE) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
-            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r]);
+            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r], bad);
         }
     } else {
         constexpr int NM = KK * (SIGN ? 2 : 3);
@@ -130,8 +132,8 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
-            if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1]);
-            else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1]);
+            if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad);
+            else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad);
 #pragma unroll
             for (int i = 0; i < NM; i++) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA
@@ -197,15 +199,21 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
     bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
     double y0 = 0.0, y1 = 0.0;
+    bool bad = false;
     if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
     for (int vt = vt0; vt < vt1; vt += 2) {
         if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
-        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax, pd);
+        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax, pd, bad);
         if (vt + 1 >= vt1) break;
         if (vt + 2 < vt1) load_tile<KK>(Y, a.yy, vt + 2, lane, b0h, b0l, y0);
-        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax, pd);
+        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax, pd,
+                                                      bad);
     }
-    if constexpr (PIPE) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1]);
+    if constexpr (PIPE)
+        epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad);
+    // rows past K (zero operand rows) have z = 0 and SSE = |y*|^2: never
+    // flagged.  One vector atomic per wave that saw an ill-conditioned sample.
+    if (__any(bad) && lane == 0) atomicAdd(a.unsafe, 1u);
 
     const int hl = lane >> 5;
     const float scale = FT ? a.f_scale : a.sqrt_dof;

@@ -81,7 +81,17 @@ struct ContractArgs {
     float sqrt_dof;        // t: sqrt(N - p)
     float f_scale;         // F: (N - p) / r
     float cdt;
+    // conditioning check: SSE = |y*|^2 - |z|^2 loses relative accuracy as
+    // SSE / |y*|^2 shrinks; a wave that meets SSE < tau * |y*|^2 adds one
+    // to *unsafe
+    float tau;
+    unsigned* unsafe;
 };
+// SSE / |y*|^2 below which a sample is counted as ill-conditioned: the
+// split-BF16 z carries ~1e-5 relative error per unit of |z|^2 / SSE, the
+// FP32 z ~4e-7
+constexpr float BPG_TAU_BF16 = 0.125f;
+constexpr float BPG_TAU_FP32 = 1.0f / 256.0f;
 // Grid of the split-BF16 contraction (see bpg_contract_bf16.h): blocks of
 // 4 waves x R operand subtiles (32 permutation rows each) over chunks of
 // 32-voxel tiles.
@@ -120,6 +130,9 @@ struct ObsArgs {
     const double* sc;      // t: [nc]; F: [nc*nc]
     int N, p, nc, Vc, ftest, nmaps;
     float* out;            // [nmaps][Vc]
+    float* betas;          // [p][Vc] or null
+    float* residuals;      // [N][Vc] or null
+    float* resvar;         // [Vc] or null
 };
 hipError_t launch_observed(const ObsArgs& a, hipStream_t s);
 

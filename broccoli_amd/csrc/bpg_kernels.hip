@@ -173,12 +173,10 @@ hipError_t launch_operand_f32(const OperandArgs& a, hipStream_t s) {
 // v_rsq_f32 / v_rcp_f32 directly: SSE is a sum of squares of the order of N,
 // never denormal, so the denormal-scaling wrappers of rsqrtf / division are
 // not needed.
-__device__ __forceinline__ float stat_t(float z0, float ss, double yyv, float sqrt_dof) {
-    const float sse = (float)(yyv - (double)ss);
+__device__ __forceinline__ float stat_t(float z0, float sse, float sqrt_dof) {
     return z0 * sqrt_dof * __builtin_amdgcn_rsqf(sse);
 }
-__device__ __forceinline__ float stat_f(float num, float ss, double yyv, float f_scale) {
-    const float sse = (float)(yyv - (double)ss);
+__device__ __forceinline__ float stat_f(float num, float sse, float f_scale) {
     return num * f_scale * __builtin_amdgcn_rcpf(sse);
 }
 
@@ -288,6 +286,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
     const int v0 = vc * L.vox_per_chunk;
     const int v1 = min(v0 + L.vox_per_chunk, a.Vpad);
     const int lane = tid & 63;
+    bool bad = false;
     for (int vb = v0; vb < v1; vb += 256) {
         const int v = vb + tid;
         const bool inb = v < a.Vpad;
@@ -309,6 +308,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
             }
         }
         const double yyv = inb ? a.yy[v] : 1.0;
+        const float tyy = a.tau * (float)yyv;
 #pragma unroll
         for (int k = 0; k < PTT; k++) {
             float ss = 0.0f, num = 0.0f;
@@ -320,7 +320,9 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
                     if (i < a.r) num = fmaf(zi, zi, num);
                 }
             }
-            float st = FT ? stat_f(num, ss, yyv, a.f_scale) : stat_t(z[k * P], ss, yyv, a.sqrt_dof);
+            const float sse = (float)(yyv - (double)ss);
+            bad |= valid && k0 + k < a.K && sse < tyy;
+            float st = FT ? stat_f(num, sse, a.f_scale) : stat_t(z[k * P], sse, a.sqrt_dof);
             if (!valid) st = -INFINITY;
             const int kk = k0 + k;
             if (a.stats && valid && kk < a.K) a.stats[(size_t)kk * a.Vc + v] = st;
@@ -342,6 +344,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
             if (lane == 0 && m > -INFINITY) atomicMax(&sMax[k], enc_max(m));
         }
     }
+    if (__any(bad) && lane == 0) atomicAdd(a.unsafe, 1u);
     __syncthreads();
     if (tid < PTT && k0 + tid < a.K)
         a.partial[(size_t)vc * a.K + k0 + tid] = sMax[tid] ? dec_max(sMax[tid]) : -INFINITY;
@@ -412,17 +415,21 @@ __global__ __launch_bounds__(256) void k_observed(ObsArgs a) {
         for (int r = 0; r < BPG_MAX_REGRESSORS; r++)
             if (r < p) beta[r] += a.Xp[r + (size_t)s * p] * ys;
     }
+    if (a.betas)
+        for (int r = 0; r < p; r++) a.betas[(size_t)r * a.Vc + v] = (float)beta[r];
     double se = 0.0, se2 = 0.0;
     for (int s = 0; s < N; s++) {
         double e = (double)a.Yc[(size_t)s * a.Vc + v];
 #pragma unroll
         for (int r = 0; r < BPG_MAX_REGRESSORS; r++)
             if (r < p) e -= a.X[s + (size_t)r * N] * beta[r];
+        if (a.residuals) a.residuals[(size_t)s * a.Vc + v] = (float)e;
         se += e;
         se2 += e * e;
     }
     if (!a.ftest) {
         const double var = se2 / (double)(N - p);
+        if (a.resvar) a.resvar[v] = (float)var;
         for (int c = 0; c < a.nmaps; c++) {
             double cv = 0.0;
             for (int r = 0; r < p; r++) cv += a.C[r + (size_t)c * p] * beta[r];
@@ -431,6 +438,7 @@ __global__ __launch_bounds__(256) void k_observed(ObsArgs a) {
     } else {
         const double mean = se / (double)N;
         const double var = (se2 - (double)N * mean * mean) / (double)(N - p - 1);
+        if (a.resvar) a.resvar[v] = (float)var;
         double cb[BPG_MAX_FROWS];
         for (int c = 0; c < a.nc; c++) {
             double acc = 0.0;

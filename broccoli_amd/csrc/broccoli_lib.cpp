@@ -118,6 +118,17 @@ void BROCCOLI_LIB::RecordError(int slot, int status, bool buffer) {
     if (ctx_) last_error_ = bpg_last_error(ctx_);
 }
 
+void BROCCOLI_LIB::SetOutputBetaVolumesMNI(float* output) { out_betas_ = output; }
+void BROCCOLI_LIB::SetOutputResidualsMNI(float* output) { out_residuals_ = output; }
+void BROCCOLI_LIB::SetOutputResidualVariances(float* output) { out_resvar_ = output; }
+void BROCCOLI_LIB::SetSaveResidualsMNI(bool value) { save_residuals_ = value; }
+void BROCCOLI_LIB::SetSaveResidualVariances(bool value) { save_resvar_ = value; }
+void BROCCOLI_LIB::SetBetasOnly(bool value) { betas_only_ = value; }
+void BROCCOLI_LIB::SetContrastsOnly(bool value) { contrasts_only_ = value; }
+void BROCCOLI_LIB::SetBetasAndContrastsOnly(bool value) { betas_and_contrasts_only_ = value; }
+
+void BROCCOLI_LIB::PerformGLMTTestSecondLevelWrapper() { PerformGLM(BPG_TTEST); }
+void BROCCOLI_LIB::PerformGLMFTestSecondLevelWrapper() { PerformGLM(BPG_FTEST); }
 void BROCCOLI_LIB::PerformMeanSecondLevelPermutationWrapper() { Perform(BPG_GROUP_MEAN); }
 void BROCCOLI_LIB::PerformGLMTTestSecondLevelPermutationWrapper() { Perform(BPG_TTEST); }
 void BROCCOLI_LIB::PerformGLMFTestSecondLevelPermutationWrapper() { Perform(BPG_FTEST); }
@@ -242,6 +253,49 @@ void BROCCOLI_LIB::Perform(int test) {
         }
     }
     t_pval_ = ms_since(t2);
+    bpg_plan_destroy(plan);
+}
+
+// Perform{GLMTTest,GLMFTest}SecondLevelWrapper (LIB:12872-13086): betas
+// always, statistical maps unless one of the *Only flags is set, residual
+// variances and residuals when asked for (LIB:12950-12962).
+void BROCCOLI_LIB::PerformGLM(int test) {
+    const int slot = test == BPG_FTEST ? kSlotStatMapsGLMFTest : kSlotStatMapsGLMTTest;
+    if (!initiated_ || !ctx_) {
+        run_kernel_errors_[slot] = BPG_ERR_NODEVICE;
+        return;
+    }
+    test_ = test;
+    float one = 1.0f;
+    bpg_study_desc d{};
+    d.test = test;
+    d.inference = 0;
+    d.precision = precision_;
+    d.connectivity = connectivity_;
+    d.W = (int)W_;
+    d.H = (int)H_;
+    d.D = (int)D_;
+    d.N = (int)N_;
+    d.p = (int)p_;
+    d.nc = (int)nc_;
+    d.Y = h_Y_;
+    d.mask = h_mask_;
+    d.X = X_;
+    d.C = C_ ? C_ : &one;
+    d.cdt = cdt_;
+    bpg_plan* plan = nullptr;
+    int rc = bpg_plan_create(ctx_, &d, &plan);
+    if (rc != BPG_OK) {
+        RecordError(kSlotBetaWeightsGLM, rc, rc == BPG_ERR_NOMEM);
+        return;
+    }
+    const bool maps = !betas_only_ && !contrasts_only_ && !betas_and_contrasts_only_;
+    rc = bpg_plan_glm(plan, out_betas_, maps ? out_maps_ : nullptr, save_residuals_ ? out_residuals_ : nullptr,
+                      save_resvar_ ? out_resvar_ : nullptr);
+    if (rc != BPG_OK) {
+        RecordError(kSlotBetaWeightsGLM, rc, rc == BPG_ERR_NOMEM);
+        RecordError(slot, rc, false);
+    }
     bpg_plan_destroy(plan);
 }
 

@@ -93,6 +93,11 @@ typedef struct {
                              constant vector lies in the nuisance space (it is
                              permutation-invariant and carries no signal)     */
     int slots_per_perm;   /* split-BF16 accumulator rows per permutation      */
+    int precision_used;   /* BPG_PREC_* of the results                        */
+    int unsafe_waves;     /* waves that met a sample with SSE below 1/8 (split-
+                             BF16) or 1/256 (FP32) of |y*|^2, in the run whose
+                             results are returned; a split-BF16 run with any
+                             is repeated in FP32                               */
 } bpg_timing;
 
 /* ---- context: replaces BROCCOLI_LIB(platform, device, ...) +
@@ -142,6 +147,17 @@ int bpg_plan_last_timing(const bpg_plan* plan, bpg_timing* out);
  *      (CalculateStatisticalMapsGLM{T,F}TestSecondLevel, LIB:15858-15950;
  *      K1:51, K1:1456, K1:1560).  maps: host grid, nmaps*W*H*D. */
 int bpg_plan_observed_maps(bpg_plan* plan, float* maps);
+
+/* ---- the second-level GLM without permutations
+ *      (PerformGLMTTestSecondLevelWrapper / PerformGLMFTestSecondLevelWrapper,
+ *      LIB:12872-13086; CalculateBetaWeightsGLM K1:51-128,
+ *      CalculateStatisticalMapsGLM{T,F}Test K1:1456-1673), in FP64 on the
+ *      original data and design.  Each output is optional (NULL):
+ *      betas p*W*H*D (group mean: 1), maps nmaps*W*H*D as
+ *      bpg_plan_observed_maps, residuals N*W*H*D, resvar W*H*D (t: sum e^2 /
+ *      (N - p); F: sum (e - mean e)^2 / (N - p - 1)).  Out-of-mask voxels
+ *      are 0.  Also records the observed maps for bpg_plan_pvalues. */
+int bpg_plan_glm(bpg_plan* plan, float* betas, float* maps, float* residuals, float* resvar);
 
 /* ---- permutation p-values (CalculatePermutationPValues, LIB:16794-16873,
  *      KC:516-652), on the observed maps of the last bpg_plan_observed_maps

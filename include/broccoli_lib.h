@@ -82,8 +82,20 @@ public:
     void SetOutputStatisticalMapsMNI(float* output);
     void SetOutputPValuesMNI(float* output);
     void SetOutputPermutationDistributions(float** output);
+    // outputs of the GLM without permutations (HDR:263, 280, 282) and what
+    // it copies out (HDR:137-139; SetSaveResiduals*, LIB:4122-4130)
+    void SetOutputBetaVolumesMNI(float* output);
+    void SetOutputResidualsMNI(float* output);
+    void SetOutputResidualVariances(float* output);
+    void SetSaveResidualsMNI(bool value);
+    void SetSaveResidualVariances(bool value);
+    void SetBetasOnly(bool value);
+    void SetContrastsOnly(bool value);
+    void SetBetasAndContrastsOnly(bool value);
 
-    // run (HDR:452-454)
+    // run (HDR:447-454)
+    void PerformGLMTTestSecondLevelWrapper();
+    void PerformGLMFTestSecondLevelWrapper();
     void PerformMeanSecondLevelPermutationWrapper();
     void PerformGLMTTestSecondLevelPermutationWrapper();
     void PerformGLMFTestSecondLevelPermutationWrapper();
@@ -116,6 +128,7 @@ public:
 private:
     void Init(cl_uint platform, cl_uint device);
     void Perform(int test);
+    void PerformGLM(int test);
     void RecordError(int slot, int status, bool buffer);
 
     bpg_ctx* ctx_ = nullptr;
@@ -149,6 +162,11 @@ private:
     float* out_maps_ = nullptr;
     float* out_p_ = nullptr;
     float** out_dist_ = nullptr;
+    float* out_betas_ = nullptr;
+    float* out_residuals_ = nullptr;
+    float* out_resvar_ = nullptr;
+    bool save_residuals_ = false, save_resvar_ = false;
+    bool betas_only_ = false, contrasts_only_ = false, betas_and_contrasts_only_ = false;
     int precision_ = 1;
     int connectivity_ = 26;
     double t_perm_ = 0.0, t_obs_ = 0.0, t_pval_ = 0.0;

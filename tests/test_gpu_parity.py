@@ -54,6 +54,10 @@ def test_permuted_statistics_match_fp64(ctx, wl, scale, K, prec):
     print(f"workload {wl} scale {scale} K {K} {prec} V={st['V']}", flush=True)
     with capi.Plan(ctx, st, prec) as plan:
         md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        tm = plan.timing()
+    # the benchmark designs are well conditioned: no fallback
+    assert tm.unsafe_waves == 0
+    assert tm.precision_used == (capi.PREC_SPLIT_BF16 if prec == "split_bf16" else capi.PREC_FP32)
     ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
     err = np.abs(stats - ref) / (1.0 + np.abs(ref))
     print(f"  max |t-t64|/(1+|t|) = {err.max():.3e}")
@@ -262,11 +266,13 @@ def test_batched_phases_equal_single_study_calls(ctx):
             pl.close()
 
 
-@pytest.mark.parametrize("N,p,nc,ftest", [(256, 25, 1, False), (40, 12, 10, True), (17, 16, 1, False)])
+@pytest.mark.parametrize("N,p,nc,ftest", [(256, 25, 1, False), (40, 12, 10, True), (24, 16, 1, False)])
 def test_design_size_limits(ctx, N, p, nc, ftest):
     """The reference's limits: up to 25 regressors (RGL:645), 10 F rows
     (K4 cbeta[10]), and 256 subjects here.  Over 16 fit columns the
-    split-BF16 request runs the FP32 form."""
+    split-BF16 request runs the FP32 form; with few residual degrees of
+    freedom (24 subjects, 16 regressors) ill-conditioned samples send the
+    phase to the FP32 form too."""
     rng = np.random.default_rng(N + p)
     st = synth.study(2, scale=0.1)
     st["N"], st["p"], st["nc"] = N, p, nc
@@ -288,9 +294,14 @@ def test_design_size_limits(ctx, N, p, nc, ftest):
         md, stats = plan.permutation_phase(0, want_stats=True, **perms)
         tm = plan.timing()
     assert tm.fit_columns == p - 1
+    if p > 17:
+        assert tm.precision_used == capi.PREC_FP32
+    if tm.precision_used == capi.PREC_SPLIT_BF16:
+        assert tm.unsafe_waves == 0
     ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
     err = np.abs(stats - ref) / (1.0 + np.abs(ref))
-    print(f"N={N} p={p} nc={nc} slots={tm.slots_per_perm} err={err.max():.2e}")
+    print(f"N={N} p={p} nc={nc} slots={tm.slots_per_perm} precision={tm.precision_used} "
+          f"unsafe={tm.unsafe_waves} err={err.max():.2e}")
     assert err.max() <= TOL
     check_maxT(md, ref, stats)
 
@@ -319,3 +330,33 @@ def test_tfce_null_and_voxel_pvalues(ctx, conn):
     oracle.lib().orc_pvalues(3, W, H, D, oracle.P(obs[0]), oracle.P(st["mask"]), None, None, st["cdt"], K,
                              oracle.P(md), oracle.P(Pref))
     assert np.array_equal(P[0], Pref)
+
+
+@pytest.mark.parametrize("ftest", [False, True])
+def test_glm_without_permutations(ctx, ftest):
+    """bpg_plan_glm (PerformGLM{T,F}TestSecondLevelWrapper, LIB:12872-13086):
+    betas, maps, residuals and residual variances against numpy least
+    squares in FP64 (K1:51-128, K1:1456-1673: t uses sum e^2 / (N - p), F
+    the centred sum / (N - p - 1)); zeros outside the mask."""
+    st = synth.study(3 if ftest else 2, scale=0.15)
+    st["inference"] = capi.VOXEL
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        b, m, r, v = plan.glm()
+        obs = plan.observed_maps()
+    N, p = st["N"], st["p"]
+    mask = st["mask"].ravel() == 1.0
+    Y = st["Y"].reshape(N, -1)[:, mask].astype(np.float64)
+    X = st["X"].astype(np.float64)
+    beta, *_ = np.linalg.lstsq(X, Y, rcond=None)
+    e = Y - X @ beta
+    if ftest:
+        var = ((e - e.mean(0)) ** 2).sum(0) / (N - p - 1)
+    else:
+        var = (e ** 2).sum(0) / (N - p)
+    assert np.allclose(b[:, mask], beta, rtol=1e-5, atol=1e-6)
+    assert np.allclose(r[:, mask], e, rtol=1e-5, atol=1e-5)
+    assert np.allclose(v[mask], var, rtol=1e-5)
+    assert np.array_equal(m, obs)
+    for a in (b, r):
+        assert np.all(a[:, ~mask] == 0)
+    assert np.all(v[~mask] == 0) and np.all(m[:, ~mask] == 0)

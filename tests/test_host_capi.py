@@ -38,7 +38,12 @@ def test_dropin_class_symbols_exported():
               "BROCCOLI_LIB::BROCCOLI_LIB(unsigned int, unsigned int, int, bool)",
               "BROCCOLI_LIB::SetDesignMatrix(float*, float*)",
               "BROCCOLI_LIB::SetPermutationMatrices(unsigned short**)",
-              "BROCCOLI_LIB::GetOpenCLRunKernelErrors()"]:
+              "BROCCOLI_LIB::GetOpenCLRunKernelErrors()",
+              "BROCCOLI_LIB::PerformGLMTTestSecondLevelWrapper()",
+              "BROCCOLI_LIB::PerformGLMFTestSecondLevelWrapper()",
+              "BROCCOLI_LIB::SetOutputBetaVolumesMNI(float*)",
+              "BROCCOLI_LIB::SetOutputResidualsMNI(float*)",
+              "BROCCOLI_LIB::SetOutputResidualVariances(float*)"]:
         assert m in out, m
 
 
