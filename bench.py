@@ -66,6 +66,7 @@ def run_batch(args):
     dt = (time.perf_counter() - t0) / args.steps
     kern = sum(p.timing().contract_ms for p in plans)
     n_fp32 = sum(p.timing().precision_used == capi.PREC_FP32 for p in plans)
+    p_fp32 = sum(p.timing().fp32_permutations for p in plans)
     t0 = time.perf_counter()
     for p, d in zip(plans, perms):
         p.permutation_phase(0, **d)
@@ -81,6 +82,7 @@ def run_batch(args):
                    "V_mean": float(np.mean([st["V"] for st in studies])), "precision": "split_bf16"},
         "one_after_another_ms": seq * 1e3,
         "studies_rerun_in_fp32": int(n_fp32),
+        "permutations_repeated_in_fp32": int(p_fp32), "permutations_total": int(sum(Ks)),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_contract_bf16",
                      "algorithmic_flop_per_step": flops, "sum_of_contraction_ms": kern,
@@ -279,6 +281,7 @@ def main():
         "fit_columns": tm.fit_columns,
         "precision_used": "split_bf16" if tm.precision_used == capi.PREC_SPLIT_BF16 else "fp32",
         "ill_conditioned_waves": tm.unsafe_waves,
+        "permutations_repeated_in_fp32": tm.fp32_permutations,
     }
     if not args.no_fp32:
         plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, args.steps // 2), 1)

@@ -51,7 +51,7 @@ class Timing(ctypes.Structure):
                 ("contract_ms", c_double), ("cluster_ms", c_double),
                 ("contract_launches", c_int), ("perm_voxels", c_long),
                 ("fit_columns", c_int), ("slots_per_perm", c_int), ("precision_used", c_int),
-                ("unsafe_waves", c_int)]
+                ("unsafe_waves", c_int), ("fp32_permutations", c_long)]
 
 
 class BpgError(RuntimeError):

@@ -110,6 +110,9 @@ struct bpg_plan {
         long K = 0;
         bool cluster = false, mass = false, bf = false, tfce = false;
         int map = 0;
+        long fp32_perms = 0;  // permutations repeated in FP32
+        bool have_times = false;  // kernel times kept from before a partial repeat
+        double t_transform = 0, t_operand = 0, t_contract = 0, t_cluster = 0;
         const unsigned short* perm = nullptr;
         const float* sign = nullptr;
         float* stats = nullptr;
@@ -128,9 +131,10 @@ struct bpg_plan {
     DBuf<float> stats;
     DBuf<unsigned> bits;
     DBuf<unsigned> parent, sizes, maxsize, labels;
-    DBuf<float> tfce;
+    DBuf<float> tfce, sqrt_size;
     DBuf<int> nlev;
     DBuf<unsigned> unsafe;
+    DBuf<unsigned char> flags;
     DBuf<double> oX, oXp, oC, osc;
     DBuf<float> d_obs, sorted, P, glm_out;
     HBuf<double> h_model;
@@ -138,14 +142,15 @@ struct bpg_plan {
     HBuf<float> h_sign, h_max;
     HBuf<int> h_nlev;
     HBuf<unsigned> h_unsafe;
+    HBuf<unsigned char> h_flags;
 
     void release_all() {
-        h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release(); h_unsafe.release();
+        h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release(); h_unsafe.release(); h_flags.release();
         Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
-        parent.release(); sizes.release(); maxsize.release(); labels.release(); tfce.release(); nlev.release(); unsafe.release();
+        parent.release(); sizes.release(); maxsize.release(); labels.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
         d_obs.release(); sorted.release(); P.release(); glm_out.release();
     }
@@ -401,6 +406,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     BPG_HIP(ctx, pl->maxout.ensure(K));
     BPG_HIP(ctx, pl->unsafe.ensure(1));
     BPG_HIP(ctx, pl->h_unsafe.ensure(1));
+    BPG_HIP(ctx, pl->flags.ensure(K));
+    BPG_HIP(ctx, pl->h_flags.ensure(K));
     if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
@@ -481,7 +488,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.cdt = d.cdt;
         ca.tau = bf ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
         ca.unsafe = pl->unsafe.p;
+        ca.flags = pl->flags.p;
         BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));
+        BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
         else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
         launches = 1;
@@ -547,6 +556,13 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         BPG_HIP(ctx, pl->parent.ensure((size_t)B * Vc));
         BPG_HIP(ctx, pl->sizes.ensure((size_t)B * Vc));
         BPG_HIP(ctx, pl->tfce.ensure((size_t)B * Vc));
+        if (pl->sqrt_size.n < (size_t)Vc + 1) {
+            // sqrt(n) for every possible component size, correctly rounded
+            std::vector<float> tab((size_t)Vc + 1);
+            for (size_t n = 0; n <= (size_t)Vc; n++) tab[n] = std::sqrt((float)n);
+            BPG_HIP(ctx, pl->sqrt_size.ensure(tab.size()));
+            BPG_HIP(ctx, hipMemcpy(pl->sqrt_size.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
         for (long k0 = 0; k0 < K; k0 += B) {
             const int nb = (int)std::min<long>(B, K - k0);
             BPG_HIP(ctx, hipMemsetAsync(pl->tfce.p, 0, (size_t)nb * Vc * sizeof(float), st));
@@ -556,6 +572,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             cc.parent = pl->parent.p;
             cc.sizes = pl->sizes.p;
             cc.tfce = pl->tfce.p;
+            cc.sqrt_size = pl->sqrt_size.p;
             cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = nb;
             cc.connectivity = d.connectivity;
             for (int j = 1; j < Lmax; j++) {
@@ -579,6 +596,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     auto& ps = pl->pending;
     ps.active = true;
     ps.map = map;
+    ps.fp32_perms = bf ? 0 : K;
+    ps.have_times = false;
     ps.perm = perm;
     ps.sign = sign;
     ps.stats = stats;
@@ -594,6 +613,66 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     return BPG_OK;
 }
 
+// Repeats the flagged permutations of the finished split-BF16 phase in the
+// FP32 form (synchronously) and merges their results into the pinned result
+// array and the caller's statistics.  With more than half flagged the whole
+// phase is repeated.
+int repeat_in_fp32(bpg_plan* pl) {
+    bpg_ctx* ctx = pl->ctx;
+    auto& ps = pl->pending;
+    auto saved = ps;
+    const long K = ps.K;
+    const int N = pl->d.N, Vc = pl->Vc;
+    std::vector<long> S;
+    for (long k = 0; k < K; k++)
+        if (pl->h_flags.p[k]) S.push_back(k);
+    if (S.empty()) return BPG_OK;
+    if ((long)S.size() * 2 > K) {
+        const int rc = phase_enqueue(pl, ps.map, ps.perm, ps.sign, K, ps.stats, true);
+        if (rc != BPG_OK) return rc;
+        BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
+        ps.active = false;
+        ps.t0 = saved.t0;
+        return BPG_OK;
+    }
+    const long Ks = (long)S.size();
+    // the kernel times reported are those of the split-BF16 run
+    auto& sv = saved;
+    sv.have_times = true;
+    sv.t_transform = elapsed(pl->ev[0], pl->ev[1]);
+    sv.t_operand = elapsed(pl->ev[1], pl->ev[2]);
+    sv.t_contract = elapsed(pl->ev[2], pl->ev[3]);
+    sv.t_cluster = elapsed(pl->ev[3], pl->ev[4]);
+    std::vector<unsigned> keep(reinterpret_cast<const unsigned*>(pl->h_max.p),
+                               reinterpret_cast<const unsigned*>(pl->h_max.p) + K);
+    std::vector<unsigned short> sp;
+    std::vector<float> ss, sst;
+    if (ps.sign) {
+        ss.resize((size_t)Ks * N);
+        for (long i = 0; i < Ks; i++) std::copy(ps.sign + S[i] * N, ps.sign + (S[i] + 1) * N, ss.begin() + i * N);
+    } else {
+        sp.resize((size_t)Ks * N);
+        for (long i = 0; i < Ks; i++) std::copy(ps.perm + S[i] * N, ps.perm + (S[i] + 1) * N, sp.begin() + i * N);
+    }
+    if (ps.stats) sst.resize((size_t)Ks * Vc);
+    const int rc = phase_enqueue(pl, ps.map, ps.sign ? nullptr : sp.data(), ps.sign ? ss.data() : nullptr, Ks,
+                                 ps.stats ? sst.data() : nullptr, true);
+    if (rc != BPG_OK) return rc;
+    BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
+    const unsigned fp32_unsafe = *pl->h_unsafe.p;
+    const unsigned* sub = reinterpret_cast<const unsigned*>(pl->h_max.p);
+    for (long i = 0; i < Ks; i++) keep[S[i]] = sub[i];
+    std::copy(keep.begin(), keep.end(), reinterpret_cast<unsigned*>(pl->h_max.p));
+    if (ps.stats)
+        for (long i = 0; i < Ks; i++)
+            std::copy(sst.begin() + i * Vc, sst.begin() + (i + 1) * Vc, saved.stats + S[i] * Vc);
+    ps = saved;
+    ps.active = false;
+    ps.fp32_perms = Ks;
+    *pl->h_unsafe.p = fp32_unsafe;
+    return BPG_OK;
+}
+
 int phase_finish(bpg_plan* pl, float* maxdist) {
     bpg_ctx* ctx = pl->ctx;
     auto& ps = pl->pending;
@@ -602,13 +681,12 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
     if (ps.bf && *pl->h_unsafe.p > 0) {
         // ill-conditioned samples: the split-BF16 z is not accurate enough
-        // for them, so the phase is repeated in the FP32 form
-        const auto t0 = ps.t0;
-        const int rc = phase_enqueue(pl, ps.map, ps.perm, ps.sign, ps.K, ps.stats, true);
+        // for them.  The flagged permutations are repeated in the FP32 form
+        // (all of them when most are flagged) and their results replace the
+        // split-BF16 ones.
+        BPG_HIP(ctx, hipMemcpy(pl->h_flags.p, pl->flags.p, ps.K, hipMemcpyDeviceToHost));
+        const int rc = repeat_in_fp32(pl);
         if (rc != BPG_OK) return rc;
-        ps.t0 = t0;
-        ps.active = false;
-        BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
     }
     const long K = ps.K;
     const bool cluster = ps.cluster, mass = ps.mass;
@@ -630,16 +708,17 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     const auto t1 = std::chrono::steady_clock::now();
     bpg_timing& tm = pl->timing;
     tm.phase_ms = std::chrono::duration<double, std::milli>(t1 - ps.t0).count();
-    tm.transform_ms = elapsed(pl->ev[0], pl->ev[1]);
-    tm.operand_ms = elapsed(pl->ev[1], pl->ev[2]);
-    tm.contract_ms = elapsed(pl->ev[2], pl->ev[3]);
-    tm.cluster_ms = (cluster || ps.tfce) ? elapsed(pl->ev[3], pl->ev[4]) : 0.0;
+    tm.transform_ms = ps.have_times ? ps.t_transform : elapsed(pl->ev[0], pl->ev[1]);
+    tm.operand_ms = ps.have_times ? ps.t_operand : elapsed(pl->ev[1], pl->ev[2]);
+    tm.contract_ms = ps.have_times ? ps.t_contract : elapsed(pl->ev[2], pl->ev[3]);
+    tm.cluster_ms = !(cluster || ps.tfce) ? 0.0 : ps.have_times ? ps.t_cluster : elapsed(pl->ev[3], pl->ev[4]);
     tm.contract_launches = ps.launches;
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = ps.pfit;
     tm.slots_per_perm = ps.bf ? ps.PP : ps.pfit;
     tm.precision_used = ps.bf ? BPG_PREC_SPLIT_BF16 : BPG_PREC_FP32;
     tm.unsafe_waves = (int)*pl->h_unsafe.p;
+    tm.fp32_permutations = ps.fp32_perms;
     return BPG_OK;
 }
 
@@ -666,14 +745,31 @@ extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const 
         if (plans[i]->pending.active)
             BPG_FAIL(plans[i]->ctx, BPG_ERR_ARG, "a permutation phase of this plan is in flight");
     }
-    // enqueue every study (each plan has its own stream), then collect; the
-    // first error is returned after the enqueued work has been waited for
+    // enqueue every study (each plan has its own stream); then enqueue the
+    // FP32 repeats of the ill-conditioned ones together, then collect.  The
+    // first error is returned after the enqueued work has been waited for.
     int rc = BPG_OK;
     int enq = 0;
     for (; enq < n; enq++) {
         rc = phase_enqueue(plans[enq], maps[enq], perms ? perms[enq] : nullptr, signs ? signs[enq] : nullptr,
                            K[enq], nullptr);
         if (rc != BPG_OK) break;
+    }
+    for (int i = 0; i < enq && rc == BPG_OK; i++) {
+        // studies with mostly ill-conditioned permutations: their FP32
+        // repeats are enqueued together; a few flagged permutations are
+        // repeated when the study is collected
+        bpg_plan* pl = plans[i];
+        auto& ps = pl->pending;
+        if (hipStreamSynchronize(pl->stream) != hipSuccess || !ps.bf || *pl->h_unsafe.p == 0) continue;
+        if (hipMemcpy(pl->h_flags.p, pl->flags.p, ps.K, hipMemcpyDeviceToHost) != hipSuccess) continue;
+        long nf = 0;
+        for (long k = 0; k < ps.K; k++) nf += pl->h_flags.p[k] != 0;
+        if (nf * 2 <= ps.K) continue;
+        const auto t0 = ps.t0;
+        const int r = phase_enqueue(pl, ps.map, ps.perm, ps.sign, ps.K, ps.stats, true);
+        ps.t0 = t0;
+        if (r != BPG_OK) rc = r;
     }
     for (int i = 0; i < enq; i++) {
         const int r = phase_finish(plans[i], maxdist[i]);

@@ -116,14 +116,15 @@ template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
 __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
                                               const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
                                               const bf16x8 (&bl)[KK], double yv, int vt, int lane, int sub0,
-                                              float (&runmax)[R][16 / PP], Pending& pd, bool& bad) {
+                                              float (&runmax)[R][16 / PP], Pending& pd,
+                                              bool (&bad)[PIPE ? 1 : R]) {
     const float yhi = (float)yv;  // NaN for padding voxels
     const float ylo = (float)(yv - (double)yhi);
     if constexpr (!PIPE) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
-            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r], bad);
+            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[PIPE ? 0 : r]);
         }
     } else {
         constexpr int NM = KK * (SIGN ? 2 : 3);
@@ -132,8 +133,8 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
-            if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad);
-            else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad);
+            if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0]);
+            else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad[0]);
 #pragma unroll
             for (int i = 0; i < NM; i++) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA
@@ -199,7 +200,13 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
     bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
     double y0 = 0.0, y1 = 0.0;
-    bool bad = false;
+    // this lane met an ill-conditioned sample, per resident subtile (one
+    // flag for all of them in the pipelined schedule, which has no registers
+    // to spare); the steady state pays one compare per sample
+    constexpr int NB = PIPE ? 1 : R;
+    bool bad[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) bad[r] = false;
     if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
     for (int vt = vt0; vt < vt1; vt += 2) {
         if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
@@ -210,10 +217,27 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
                                                       bad);
     }
     if constexpr (PIPE)
-        epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad);
+        epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0]);
     // rows past K (zero operand rows) have z = 0 and SSE = |y*|^2: never
-    // flagged.  One vector atomic per wave that saw an ill-conditioned sample.
-    if (__any(bad) && lane == 0) atomicAdd(a.unsafe, 1u);
+    // flagged.  Rare: a wave that met an ill-conditioned sample adds one to
+    // the counter and flags the permutations of each (subtile, half-wave)
+    // where it happened (of every resident subtile in the pipelined form).
+    bool any = false;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const unsigned long long m = __ballot(bad[b]);
+        any |= m != 0ull;
+        if (m && (lane & 31) == 0) {
+            const int hl = lane >> 5;
+            if ((hl ? (m >> 32) : (m & 0xFFFFFFFFull)) != 0ull)
+                for (int r = (PIPE ? 0 : b); r < (PIPE ? R : b + 1); r++)
+                    for (int P = 0; P < PER; P++) {
+                        const int k = (sub0 + r) * (2 * PER) + hl * PER + P;
+                        if (k < a.K) a.flags[k] = 1;
+                    }
+        }
+    }
+    if (any && lane == 0) atomicAdd(a.unsafe, 1u);
 
     const int hl = lane >> 5;
     const float scale = FT ? a.f_scale : a.sqrt_dof;

@@ -6,9 +6,10 @@
 // Tuning knobs of the split-BF16 contraction:
 //  - operand subtiles kept per wave: BPG_R_SIGN for exact +-1 operands with
 //    KK <= 4 (N <= 64), BPG_R_SMALL for general operands with KK <= 4, 2
-//    up to KK = BPG_R2_KK, else 1;
+//    up to KK = BPG_R2_KK, else 1 (R = 2 at KK = 13 needs 256 VGPRs and
+//    200 AGPRs, one wave per SIMD, and halves the data-tile loads);
 //  - BPG_PIPE_SMALL / _MID / _LARGE: software-pipelined epilogue for
-//    general operands at KK <= 4, KK <= BPG_R2_KK and above;
+//    general operands at KK <= 4, KK <= 8 and above;
 //  - the number of blocks the voxel range is split over.
 #ifndef BPG_R_SIGN
 #define BPG_R_SIGN 4
@@ -17,7 +18,7 @@
 #define BPG_R_SMALL 3
 #endif
 #ifndef BPG_R2_KK
-#define BPG_R2_KK 8
+#define BPG_R2_KK 16
 #endif
 #ifndef BPG_PIPE_SMALL
 #define BPG_PIPE_SMALL 1
@@ -86,6 +87,7 @@ struct ContractArgs {
     // to *unsafe
     float tau;
     unsigned* unsafe;
+    unsigned char* flags;  // [K]: 1 for a permutation with an ill-conditioned sample
 };
 // SSE / |y*|^2 below which a sample is counted as ill-conditioned: the
 // split-BF16 z carries ~1e-5 relative error per unit of |z|^2 / SSE, the
@@ -103,7 +105,7 @@ constexpr int bf16_rows_per_wave(int KK, bool sign) {
     return KK <= 4 ? (sign ? BPG_R_SIGN : BPG_R_SMALL) : (KK <= BPG_R2_KK ? 2 : 1);
 }
 constexpr bool bf16_pipelined(int KK, bool sign) {
-    return !sign && (KK <= 4 ? BPG_PIPE_SMALL : (KK <= BPG_R2_KK ? BPG_PIPE_MID : BPG_PIPE_LARGE)) != 0;
+    return !sign && (KK <= 4 ? BPG_PIPE_SMALL : (KK <= 8 ? BPG_PIPE_MID : BPG_PIPE_LARGE)) != 0;
 }
 Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign);
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
@@ -166,6 +168,7 @@ struct CclArgs {
     unsigned* labels;           // compact root of every active voxel of map 0 (or null)
     const float* values;        // [nmaps][Vc] statistic values (cluster mass) or null
     float* tfce;                // [nmaps][Vc] TFCE sums (stage 4) or null
+    const float* sqrt_size;     // [Vc + 1] correctly rounded sqrt(n) (stage 4)
     float level;                // TFCE threshold of stage 4
     int Vc, nwords, nmaps, connectivity;
 };

@@ -321,7 +321,10 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
                 }
             }
             const float sse = (float)(yyv - (double)ss);
-            bad |= valid && k0 + k < a.K && sse < tyy;
+            if (valid && k0 + k < a.K && sse < tyy) {
+                bad = true;
+                a.flags[k0 + k] = 1;
+            }
             float st = FT ? stat_f(num, sse, a.f_scale) : stat_t(z[k * P], sse, a.sqrt_dof);
             if (!valid) st = -INFINITY;
             const int kk = k0 + k;
@@ -524,6 +527,16 @@ __device__ __forceinline__ void uf_union(unsigned* p, unsigned a, unsigned b) {
     }
 }
 
+// CalculateTFCEValues (KC:482-510): value = sqrt(size) * h * h, added to
+// the voxel's sum, every operation rounded on its own (no fused
+// multiply-add); sqrt(size) comes from a table of correctly rounded values
+// (v_sqrt_f32 is not correctly rounded).
+__device__ __forceinline__ void tfce_add(float* acc, float sqrt_size, float h) {
+#pragma clang fp contract(off)
+    const float value = sqrt_size * h * h;
+    *acc = *acc + value;
+}
+
 // One thread per 32-voxel word of one map.  Stages: 0 init, 1 union with the
 // lower-index neighbours, 2 component sums (runs of equal roots within the
 // word are added with one atomic), 3 largest sum per map.
@@ -581,13 +594,7 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
         } else if (stage == 3) {
             if (parent[c] == (unsigned)c) best = max(best, sizes[c]);
         } else {
-            // CalculateTFCEValues (KC:482-510): value = sqrt(size) * h * h,
-            // added to the voxel's sum; each operation rounded on its own
-            // (no fused multiply-add), sqrt correctly rounded
-            const float size = (float)sizes[uf_find(parent, (unsigned)c)];
-            const float value = __fmul_rn(__fmul_rn(__fsqrt_rn(size), a.level), a.level);
-            float* acc = a.tfce + (size_t)map * a.Vc + c;
-            *acc = __fadd_rn(*acc, value);
+            tfce_add(a.tfce + (size_t)map * a.Vc + c, a.sqrt_size[sizes[uf_find(parent, (unsigned)c)]], a.level);
         }
     }
     if (stage == 2 && run_sum) atomicAdd(&sizes[run_root], run_sum);

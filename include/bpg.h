@@ -95,9 +95,10 @@ typedef struct {
     int slots_per_perm;   /* split-BF16 accumulator rows per permutation      */
     int precision_used;   /* BPG_PREC_* of the results                        */
     int unsafe_waves;     /* waves that met a sample with SSE below 1/8 (split-
-                             BF16) or 1/256 (FP32) of |y*|^2, in the run whose
-                             results are returned; a split-BF16 run with any
-                             is repeated in FP32                               */
+                             BF16) or 1/256 (FP32) of |y*|^2; the split-BF16
+                             permutations with such samples are repeated in
+                             FP32, and this counts the FP32 run's waves       */
+    long fp32_permutations; /* permutations whose results come from FP32      */
 } bpg_timing;
 
 /* ---- context: replaces BROCCOLI_LIB(platform, device, ...) +

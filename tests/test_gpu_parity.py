@@ -301,7 +301,7 @@ def test_design_size_limits(ctx, N, p, nc, ftest):
     ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
     err = np.abs(stats - ref) / (1.0 + np.abs(ref))
     print(f"N={N} p={p} nc={nc} slots={tm.slots_per_perm} precision={tm.precision_used} "
-          f"unsafe={tm.unsafe_waves} err={err.max():.2e}")
+          f"fp32 perms={tm.fp32_permutations} unsafe={tm.unsafe_waves} err={err.max():.2e}")
     assert err.max() <= TOL
     check_maxT(md, ref, stats)
 
@@ -360,3 +360,31 @@ def test_glm_without_permutations(ctx, ftest):
     for a in (b, r):
         assert np.all(a[:, ~mask] == 0)
     assert np.all(v[~mask] == 0) and np.all(m[:, ~mask] == 0)
+
+
+def test_ill_conditioned_permutations_repeat_in_fp32(ctx):
+    """A voxel whose data is (almost) the group regressor: under the
+    identity permutation the model explains nearly all of it, SSE is a tiny
+    fraction of |y*|^2, and that permutation is repeated in FP32; the
+    others stay split-BF16.  Every statistic still meets the FP64 bound."""
+    st = synth.study(2, scale=0.12)
+    st["inference"] = capi.VOXEL
+    vidx = vox(st)
+    Y = st["Y"].reshape(st["N"], -1).copy()
+    rng = np.random.default_rng(5)
+    # SSE / |y*|^2 ~ 0.04 under the identity: below 1/8 (split-BF16 check),
+    # above 1/256 (FP32 check)
+    Y[:, vidx[3]] = 3.0 * st["X"][:, 1] + 0.3 * rng.standard_normal(st["N"])
+    st["Y"] = Y.reshape(st["Y"].shape).astype(np.float32)
+    K = 200  # a flagged half-wave marks its resident permutations (15 here)
+    perms = synth.permutations(st, K, seed=8)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        tm = plan.timing()
+    print(f"fp32 permutations {tm.fp32_permutations}, unsafe waves {tm.unsafe_waves}, precision {tm.precision_used}")
+    assert tm.precision_used == capi.PREC_SPLIT_BF16
+    assert 1 <= tm.fp32_permutations < K // 2 and tm.unsafe_waves == 0
+    ref = oracle.perm_stats(st, perms, K)[:, vidx]
+    err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+    assert err.max() <= TOL, err.max()
+    check_maxT(md, ref, stats)
