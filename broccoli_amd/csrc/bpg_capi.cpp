@@ -21,6 +21,7 @@ struct bpg_ctx {
     std::string err;
     std::string name;
     std::string arch;
+    size_t lds_per_block = 0;  // shared memory one block may use
 };
 
 namespace {
@@ -130,7 +131,7 @@ struct bpg_plan {
     DBuf<float> partial, maxout;
     DBuf<float> stats;
     DBuf<unsigned> bits;
-    DBuf<unsigned> parent, sizes, maxsize, labels;
+    DBuf<unsigned> parent, sizes, maxsize, labels, overflow;
     DBuf<float> tfce, sqrt_size;
     DBuf<int> nlev;
     DBuf<unsigned> unsafe;
@@ -150,7 +151,7 @@ struct bpg_plan {
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
-        parent.release(); sizes.release(); maxsize.release(); labels.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
+        parent.release(); sizes.release(); maxsize.release(); labels.release(); overflow.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
         d_obs.release(); sorted.release(); P.release(); glm_out.release();
     }
@@ -192,6 +193,7 @@ extern "C" int bpg_create(int device, bpg_ctx** out) {
     }
     c->device = device;
     c->arch = prop.gcnArchName;
+    c->lds_per_block = prop.sharedMemPerBlockOptin ? prop.sharedMemPerBlockOptin : prop.sharedMemPerBlock;
     c->name = prop.name[0] ? std::string(prop.name) : "AMD GPU " + c->arch;
     if (c->arch.compare(0, 6, "gfx950") != 0) {
         delete c;
@@ -523,7 +525,25 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             cc.Vc = Vc; cc.nwords = pl->nwords;
             cc.nmaps = (int)std::min<long>(B, K - k0);
             cc.connectivity = d.connectivity;
-            BPG_HIP(ctx, bpg::launch_ccl(cc, st));
+            if (bpg::ccl_lds_fits(pl->nwords, ctx->lds_per_block)) {
+                // whole maps in LDS; the few with too many active voxels
+                // (listed by the kernel) take the global path
+                BPG_HIP(ctx, pl->overflow.ensure((size_t)cc.nmaps + 1));
+                BPG_HIP(ctx, hipMemsetAsync(pl->overflow.p + cc.nmaps, 0, sizeof(unsigned), st));
+                BPG_HIP(ctx, bpg::launch_ccl_lds(cc, ctx->lds_per_block, pl->overflow.p, pl->overflow.p + cc.nmaps,
+                                                 st));
+                unsigned n_over = 0;
+                BPG_HIP(ctx, hipMemcpyAsync(&n_over, pl->overflow.p + cc.nmaps, sizeof(unsigned),
+                                            hipMemcpyDeviceToHost, st));
+                BPG_HIP(ctx, hipStreamSynchronize(st));
+                if (n_over > 0) {
+                    cc.map_ids = pl->overflow.p;
+                    cc.nmaps = (int)n_over;
+                    BPG_HIP(ctx, bpg::launch_ccl(cc, st));
+                }
+            } else {
+                BPG_HIP(ctx, bpg::launch_ccl(cc, st));
+            }
         }
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
         BPG_HIP(ctx, hipMemcpyAsync(host, pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
@@ -592,6 +612,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipMemcpyAsync(pl->h_unsafe.p, pl->unsafe.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    if (bf) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[5], st));
     auto& ps = pl->pending;
     ps.active = true;
@@ -684,7 +705,6 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
         // for them.  The flagged permutations are repeated in the FP32 form
         // (all of them when most are flagged) and their results replace the
         // split-BF16 ones.
-        BPG_HIP(ctx, hipMemcpy(pl->h_flags.p, pl->flags.p, ps.K, hipMemcpyDeviceToHost));
         const int rc = repeat_in_fp32(pl);
         if (rc != BPG_OK) return rc;
     }
@@ -762,7 +782,6 @@ extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const 
         bpg_plan* pl = plans[i];
         auto& ps = pl->pending;
         if (hipStreamSynchronize(pl->stream) != hipSuccess || !ps.bf || *pl->h_unsafe.p == 0) continue;
-        if (hipMemcpy(pl->h_flags.p, pl->flags.p, ps.K, hipMemcpyDeviceToHost) != hipSuccess) continue;
         long nf = 0;
         for (long k = 0; k < ps.K; k++) nf += pl->h_flags.p[k] != 0;
         if (nf * 2 <= ps.K) continue;

@@ -549,7 +549,8 @@ __device__ __forceinline__ void tfce_add(float* acc, float sqrt_size, float h) {
 __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)a.nmaps * a.nwords) return;
-    const int map = (int)(t / a.nwords);
+    const int slot = (int)(t / a.nwords);
+    const int map = a.map_ids ? (int)a.map_ids[slot] : slot;
     const int w = (int)(t % a.nwords);
     unsigned* parent = a.parent + (size_t)map * a.Vc;
     unsigned* sizes = a.sizes + (size_t)map * a.Vc;
@@ -599,6 +600,158 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     }
     if (stage == 2 && run_sum) atomicAdd(&sizes[run_root], run_sum);
     if (stage == 3 && best) atomicMax(&a.maxsize[map], best);
+}
+
+// ---- whole-map CCL in LDS ----
+constexpr int CCL_LDS_THREADS = 1024;
+constexpr size_t CCL_LDS_BYTES = 160 * 1024;  // LDS of one CU (gfx950)
+
+__device__ __forceinline__ unsigned lds_find(unsigned* p, unsigned x) {
+    volatile unsigned* vp = p;
+    for (;;) {
+        const unsigned px = vp[x];
+        if (px == x) return x;
+        const unsigned ppx = vp[px];
+        if (ppx != px) vp[x] = ppx;
+        x = ppx;
+    }
+}
+
+__device__ __forceinline__ void lds_union(unsigned* p, unsigned a, unsigned b) {
+    for (;;) {
+        a = lds_find(p, a);
+        b = lds_find(p, b);
+        if (a == b) return;
+        if (a < b) {
+            const unsigned t = a;
+            a = b;
+            b = t;
+        }
+        const unsigned old = atomicMin(&p[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+// LDS capacity for active voxels when the block may use lds_bytes, the
+// scan takes 2 * 1024 words and the bitmask and its ranks 2 * nwords
+int ccl_lds_cap(int nwords, size_t lds_bytes) {
+    const long words = (long)(std::min(lds_bytes, CCL_LDS_BYTES) / 4) - 2L * CCL_LDS_THREADS - 2L * nwords;
+    return words > 0 ? (int)(words / 2) : 0;
+}
+
+__global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap, unsigned* overflow,
+                                                              unsigned* n_overflow) {
+    extern __shared__ unsigned smem[];
+    __shared__ unsigned scan[2][CCL_LDS_THREADS];
+    const int map = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nw = a.nwords;
+    unsigned* sbits = smem;           // [nw]
+    unsigned* srank = sbits + nw;     // [nw] active voxels before each word
+    unsigned* spar = srank + nw;      // [cap]
+    unsigned* ssz = spar + cap;       // [cap]
+    const unsigned* bits = a.bits + (size_t)map * nw;
+    for (int w = tid; w < nw; w += CCL_LDS_THREADS) sbits[w] = bits[w];
+    __syncthreads();
+    // ranks: contiguous chunks per thread, block scan of the chunk counts
+    const int per = (nw + CCL_LDS_THREADS - 1) / CCL_LDS_THREADS;
+    const int w0 = min(tid * per, nw), w1 = min(w0 + per, nw);
+    unsigned cnt = 0;
+    for (int w = w0; w < w1; w++) cnt += __popc(sbits[w]);
+    int cur = 0;
+    scan[0][tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < CCL_LDS_THREADS; off <<= 1) {
+        const unsigned v = scan[cur][tid] + (tid >= off ? scan[cur][tid - off] : 0u);
+        scan[cur ^ 1][tid] = v;
+        cur ^= 1;
+        __syncthreads();
+    }
+    const unsigned total = scan[cur][CCL_LDS_THREADS - 1];
+    unsigned base = scan[cur][tid] - cnt;
+    for (int w = w0; w < w1; w++) {
+        srank[w] = base;
+        base += __popc(sbits[w]);
+    }
+    if (total > (unsigned)cap) {  // the whole block takes this branch
+        if (tid == 0) overflow[atomicAdd(n_overflow, 1u)] = (unsigned)map;
+        return;
+    }
+    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS) {
+        spar[i] = i;
+        ssz[i] = 0u;
+    }
+    __syncthreads();
+    auto active = [&](int n) { return n >= 0 && ((sbits[n >> 5] >> (n & 31)) & 1u); };
+    auto rank = [&](int n) { return srank[n >> 5] + __popc(sbits[n >> 5] & ((1u << (n & 31)) - 1u)); };
+    const int nb6 = a.connectivity == 6;
+    for (int w = tid; w < nw; w += CCL_LDS_THREADS) {
+        unsigned word = sbits[w];
+        while (word) {
+            const int b = __ffs(word) - 1;
+            word &= word - 1;
+            const int c = w * 32 + b;
+            if (c >= a.Vc) break;
+            const unsigned lc = srank[w] + __popc(sbits[w] & ((1u << b) - 1u));
+            // the same implied-link rules as the global path
+            const int* nc = a.nbr + (size_t)c * BPG_NBR;
+            const int nx = nc[0], ny = nc[1], nz = nc[2];
+            if (active(nx)) lds_union(spar, lc, rank(nx));
+            const bool ay = active(ny), az = active(nz);
+            if (ay) lds_union(spar, lc, rank(ny));
+            if (az) lds_union(spar, lc, rank(nz));
+            if (nb6) continue;
+            if (!ay)
+                for (int j = 3; j < 5; j++)
+                    if (active(nc[j])) lds_union(spar, lc, rank(nc[j]));
+            if (!az)
+                for (int j = 5; j < BPG_NBR; j++)
+                    if (active(nc[j])) lds_union(spar, lc, rank(nc[j]));
+        }
+    }
+    __syncthreads();
+    for (int w = tid; w < nw; w += CCL_LDS_THREADS) {
+        unsigned word = sbits[w];
+        unsigned li = srank[w];
+        while (word) {
+            const int b = __ffs(word) - 1;
+            word &= word - 1;
+            const int c = w * 32 + b;
+            if (c >= a.Vc) break;
+            // extent (KC:403-430) or mass (KC:432-458)
+            const unsigned add = a.values ? (unsigned)(a.values[(size_t)map * a.Vc + c] * 10000.0f) : 1u;
+            atomicAdd(&ssz[lds_find(spar, li)], add);
+            li++;
+        }
+    }
+    __syncthreads();
+    unsigned best = 0;
+    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS)
+        if (spar[i] == i) best = max(best, ssz[i]);
+    for (int o = 32; o > 0; o >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, o, 64));
+    __syncthreads();
+    if ((tid & 63) == 0) scan[0][tid >> 6] = best;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned m = 0;
+        for (int i = 0; i < CCL_LDS_THREADS / 64; i++) m = max(m, scan[0][i]);
+        a.maxsize[map] = m;
+    }
+}
+
+bool ccl_lds_fits(int nwords, size_t lds_bytes) { return nwords > 0 && ccl_lds_cap(nwords, lds_bytes) >= 4096; }
+
+hipError_t launch_ccl_lds(const CclArgs& a, size_t lds_bytes, unsigned* overflow, unsigned* n_overflow,
+                          hipStream_t s) {
+    if (a.nmaps <= 0) return hipSuccess;
+    const int cap = ccl_lds_cap(a.nwords, lds_bytes);
+    const size_t dyn = 4 * ((size_t)2 * a.nwords + (size_t)2 * cap);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ccl_lds),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ccl_lds, dim3(a.nmaps), dim3(CCL_LDS_THREADS), dyn, s, a, cap, overflow, n_overflow);
+    return hipGetLastError();
 }
 
 hipError_t launch_ccl(const CclArgs& a, hipStream_t s) {

@@ -388,3 +388,22 @@ def test_ill_conditioned_permutations_repeat_in_fp32(ctx):
     err = np.abs(stats - ref) / (1.0 + np.abs(ref))
     assert err.max() <= TOL, err.max()
     check_maxT(md, ref, stats)
+
+
+def test_cluster_extent_large_components_take_the_global_path(ctx):
+    """Maps with more active voxels than the LDS path holds (here every
+    voxel is above a threshold of -1) are labelled by the global path; both
+    paths agree with the oracle."""
+    st = synth.study(3, scale=0.5)
+    st["inference"] = capi.CLUSTER_EXTENT
+    K = 6
+    perms = synth.permutations(st, K, seed=29)
+    for cdt in (-1.0, 3.0):
+        st["cdt"] = cdt
+        with capi.Plan(ctx, st, "split_bf16") as plan:
+            md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        for k in range(K):
+            _, _, best = oracle.ccl(scatter(st, stats[k]), st["mask"], cdt, 26)
+            assert md[k] == float(best), (cdt, k)
+        if cdt < 0:
+            assert np.all(md == st["V"])
