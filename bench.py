@@ -103,7 +103,11 @@ WORKLOADS = {
     3: "F test over 3 group dummies, 91x109x91 grid, N=100, p=6, smooth noise (FWHM 3), 10000 permutations, "
        "cluster-extent max (F>4, 26-connectivity)",
     5: "t test on a covariate, 182x218x182 grid (~1.83M voxels), N=200, p=8, 10000 row permutations, voxel max-T",
+    6: "first-level permutation test: 64x64x33 EPI grid (3.75x3.75x4 mm), ellipsoid mask, T=240, p=12 "
+       "(intercept, trend, block task, sinusoid, 8 nuisance), AR(4) noise, t on the task, AR(4) whitening "
+       "(8 mm smoothing), 10000 time-point permutations, voxel max-T",
 }
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector
 
 
 # rocprofv3 summaries of the dominant kernel per workload: HBM traffic per
@@ -118,6 +122,76 @@ def profiled_traffic(workload):
         return None, None
     m = re.search(r"traffic_bytes_per_launch: (\d+)", f.read_text())
     return (int(m.group(1)), str(f.relative_to(ROOT))) if m else (None, None)
+
+
+def run_first_level(args):
+    """Workload 6: the first-level permutation test of one subject."""
+    st = synth.first_level_workload()
+    T, p = st["T"], st["p"]
+    K = args.K
+    perm = capi.gen_perm_first_level(T, K, compat=False, seed=20260006)
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    plan = capi.FlPlan(ctx, st)
+    setup_wall = time.perf_counter() - t0
+    V = plan.num_voxels
+    print(f"# first-level study ready: V={V} T={T} p={p} K={K} set-up {plan.timing().transform_ms:.1f} ms "
+          f"(wall {setup_wall*1e3:.0f} ms)", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        plan.permutation_phase(0, perm)
+    hip_sync()
+    t0 = time.perf_counter()
+    kern = []
+    for _ in range(args.steps):
+        md, _ = plan.permutation_phase(0, perm)
+        kern.append(plan.timing().contract_ms)
+    hip_sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    tm = plan.timing()
+    kern_ms = float(np.mean(kern))
+    # per permutation, voxel and time point: the AR(4) recursion (4 FMA),
+    # the projections on the p design columns (p FMA), |x|^2 (1 FMA), sum x
+    flops = float(T) * (2 * 4 + 2 * p + 2 + 1) * K * V
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    res = {
+        "metric": "perm·voxels/s (first-level max-T)", "value": K * V / dt, "unit": "perm·voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WORKLOADS[6], "V_mask": V, "T": T, "p": p, "K": K},
+        "setup_ms": plan.timing().transform_ms,
+        "phase_breakdown_ms": {"permutation_kernel": kern_ms, "library_call": tm.phase_ms},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "k_fl_perm",
+                     "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms},
+    }
+    plan.close()
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_first_level(st, perm, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(res))
+
+
+def cpu_baseline_first_level(st, perm, target_s):
+    """The oracle's FP64 restatement of the first-level loop (inverse
+    whitening of the whole grid, then the statistic map, OpenMP over the
+    host cores) on a bounded sample of the permutations."""
+    from oracle import oracle
+
+    ncores = min(len(os.sched_getaffinity(0)), 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
+    wh, ar = oracle.fl_prepare(st["Y"], st["mask"], st["X"], st.get("fwhm", 8.0), st["voxel"])
+    V = int((st["mask"] == 1).sum())
+    t0 = time.perf_counter()
+    oracle.fl_null(0, 0, wh, ar, st["mask"], st["X"], st["C"], perm[:1])
+    per = time.perf_counter() - t0
+    kc = int(max(1, min(2000, target_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.fl_null(0, 0, wh, ar, st["mask"], st["X"], st["C"], perm[:kc])
+    dt = time.perf_counter() - t0
+    return dict(value=kc * V / dt, unit="perm·voxels/s", cores=ncores, kind="port",
+                sample=f"{kc} of the {perm.shape[0]} permutations, all {V} voxels, {dt:.1f} s, FP64 "
+                       "inverse whitening + statistic map + quantised max per permutation")
 
 
 def hip_sync():
@@ -222,10 +296,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5],
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5, 6],
                     help="2 (default, the metric's workload); 1: sign-flip group mean, K=1000; "
                          "3: F test with cluster-extent inference; 4: 64 studies in one batched call; "
-                         "5: 182x218x182 grid, N=200, p=8")
+                         "5: 182x218x182 grid, N=200, p=8; 6: first-level permutation test")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = full size)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, launch a plain elementwise copy of this many MiB three times "
@@ -238,6 +312,9 @@ def main():
 
     if args.workload == 4:
         run_batch(args)
+        return
+    if args.workload == 6:
+        run_first_level(args)
         return
     st = synth.study(args.workload, scale=args.scale)
     K = args.K if args.workload == 2 or args.K != 10000 else st["K"]

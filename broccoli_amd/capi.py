@@ -20,7 +20,7 @@ LIB_PATH = Path(os.environ.get("BPG_LIB", Path(__file__).resolve().with_name("li
 TTEST, FTEST, GROUP_MEAN = 0, 1, 2
 VOXEL, CLUSTER_EXTENT, CLUSTER_MASS, TFCE = 0, 1, 2, 3
 TWOSAMPLE, CORRELATION, MEAN_DESIGN = 0, 1, 2
-PREC_FP32, PREC_SPLIT_BF16 = 0, 1
+PREC_FP32, PREC_SPLIT_BF16, PREC_FP64 = 0, 1, 2
 PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16}
 
 OK = 0
@@ -36,6 +36,9 @@ EXPORTS = [
     "bpg_plan_pvalues", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
     "bpg_gen_perm_correlation", "bpg_detect_designs", "bpg_max_permutations",
     "bpg_model_precompute",
+    "bpg_gen_perm_first_level", "bpg_fl_plan_create", "bpg_fl_plan_destroy", "bpg_fl_plan_num_voxels",
+    "bpg_fl_plan_ar_estimates", "bpg_fl_plan_whitened", "bpg_fl_plan_permutation_phase",
+    "bpg_fl_plan_last_timing",
 ]
 
 
@@ -44,6 +47,13 @@ class StudyDesc(ctypes.Structure):
                 ("connectivity", c_int), ("W", c_int), ("H", c_int), ("D", c_int),
                 ("N", c_int), ("p", c_int), ("nc", c_int), ("Y", c_void_p),
                 ("mask", c_void_p), ("X", c_void_p), ("C", c_void_p), ("cdt", c_float)]
+
+
+class FlDesc(ctypes.Structure):
+    _fields_ = [("test", c_int), ("inference", c_int), ("connectivity", c_int), ("W", c_int), ("H", c_int),
+                ("D", c_int), ("T", c_int), ("p", c_int), ("nc", c_int), ("volumes", c_void_p),
+                ("mask", c_void_p), ("X", c_void_p), ("C", c_void_p), ("voxel_size", c_float * 3),
+                ("ar_smoothing_fwhm", c_float), ("cdt", c_float)]
 
 
 class Timing(ctypes.Structure):
@@ -93,6 +103,14 @@ def load() -> ctypes.CDLL:
         "bpg_detect_designs": (c_int, [c_int, c_int, c_int, vp, vp, c_int, vp, vp, vp]),
         "bpg_max_permutations": (c_double, [c_int, c_int, c_int, c_int]),
         "bpg_model_precompute": (c_int, [c_int, c_int, c_int, c_int, vp, vp, vp, vp]),
+        "bpg_gen_perm_first_level": (c_int, [c_int, c_long, c_int, c_ulonglong, vp]),
+        "bpg_fl_plan_create": (c_int, [vp, POINTER(FlDesc), POINTER(c_void_p)]),
+        "bpg_fl_plan_destroy": (None, [vp]),
+        "bpg_fl_plan_num_voxels": (c_long, [vp]),
+        "bpg_fl_plan_ar_estimates": (c_int, [vp, vp]),
+        "bpg_fl_plan_whitened": (c_int, [vp, vp]),
+        "bpg_fl_plan_permutation_phase": (c_int, [vp, c_int, vp, c_long, vp, vp]),
+        "bpg_fl_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -315,3 +333,91 @@ def permutation_phases(plans, maps, perms):
     O = (c_void_p * n)(*[o.ctypes.data for o in outs])
     _check(lib.bpg_plans_permutation_phase(P, n, M, pp, ss, Ks, O), plans[0].ctx.handle)
     return outs
+
+
+# ------------------------------------------------------------- first level
+
+def gen_perm_first_level(T: int, K: int, compat: bool = True, seed: int = 0) -> np.ndarray:
+    """Time-point permutations of the first-level test, (K, T) uint16."""
+    out = np.zeros((K, T), np.uint16)
+    _check(load().bpg_gen_perm_first_level(T, K, int(compat), seed, _ptr(out)))
+    return out
+
+
+class FlPlan:
+    """A first-level study on the device (bpg_fl_plan): regression and AR(4)
+    whitening run at creation.
+
+    study: dict with Y (T, D, H, W) float32, mask (D, H, W), X (T, p), C
+    (nc, p), voxel (3 sizes in mm); optional test (TTEST), inference
+    (VOXEL), cdt, connectivity (26), fwhm (AR smoothing, 8 mm).
+    """
+
+    def __init__(self, ctx: Context, study: dict):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        s = study
+        T, D, H, W = s["Y"].shape
+        self._Y = np.ascontiguousarray(s["Y"], np.float32)
+        self._mask = np.ascontiguousarray(s["mask"], np.float32)
+        self._X = np.asfortranarray(s["X"], np.float32).ravel(order="F").copy()
+        C = np.atleast_2d(np.asarray(s["C"], np.float32))
+        self._C = np.ascontiguousarray(C).ravel()
+        d = FlDesc()
+        d.test = s.get("test", TTEST)
+        d.inference = s.get("inference", VOXEL)
+        d.connectivity = s.get("connectivity", 26)
+        d.W, d.H, d.D, d.T = W, H, D, T
+        d.p, d.nc = s["X"].shape[1], C.shape[0]
+        d.volumes, d.mask, d.X, d.C = _ptr(self._Y), _ptr(self._mask), _ptr(self._X), _ptr(self._C)
+        for i in range(3):
+            d.voxel_size[i] = float(s["voxel"][i])
+        d.ar_smoothing_fwhm = float(s.get("fwhm", 8.0))
+        d.cdt = float(s.get("cdt", 0.0))
+        self.desc = d
+        h = c_void_p()
+        _check(self._lib.bpg_fl_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        self.shape = (T, D, H, W)
+        self.G = W * H * D
+        self.nmaps = C.shape[0] if d.test == TTEST else 1
+
+    @property
+    def num_voxels(self) -> int:
+        return self._lib.bpg_fl_plan_num_voxels(self.handle)
+
+    def ar_estimates(self) -> np.ndarray:
+        out = np.zeros(4 * self.G, np.float32)
+        _check(self._lib.bpg_fl_plan_ar_estimates(self.handle, _ptr(out)), self.ctx.handle)
+        return out.reshape((4,) + self.shape[1:])
+
+    def whitened(self) -> np.ndarray:
+        out = np.zeros(self.shape[0] * self.G, np.float32)
+        _check(self._lib.bpg_fl_plan_whitened(self.handle, _ptr(out)), self.ctx.handle)
+        return out.reshape(self.shape)
+
+    def permutation_phase(self, map_index: int, perm, want_stats: bool = False):
+        """(maxdist float32[K], stats float32[K, V_mask] or None)."""
+        perm = np.ascontiguousarray(perm, np.uint16)
+        K = perm.shape[0]
+        maxdist = np.zeros(K, np.float32)
+        stats = np.zeros((K, self.num_voxels), np.float32) if want_stats else None
+        _check(self._lib.bpg_fl_plan_permutation_phase(self.handle, map_index, _ptr(perm), K, _ptr(maxdist),
+                                                       _ptr(stats)), self.ctx.handle)
+        return maxdist, stats
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check(self._lib.bpg_fl_plan_last_timing(self.handle, ctypes.byref(t)))
+        return t
+
+    def close(self):
+        if self.handle:
+            self._lib.bpg_fl_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -15,65 +15,15 @@
 
 #include "bpg_internal.h"
 #include "bpg_kernels.h"
-
-struct bpg_ctx {
-    int device = 0;
-    std::string err;
-    std::string name;
-    std::string arch;
-    size_t lds_per_block = 0;  // shared memory one block may use
-};
+#include "bpg_runtime.h"
 
 namespace {
 
-// device buffer that only grows
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t ensure(size_t want) {
-        if (want <= n && p) return hipSuccess;
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-            n = 0;
-        }
-        if (want == 0) want = 1;
-        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
-        if (e == hipSuccess) n = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-// pinned host staging: the per-phase uploads and downloads run as true
-// asynchronous DMA, and the host fills / reads them while kernels run
-template <typename T>
-struct HBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t ensure(size_t want) {
-        if (want <= n && p) return hipSuccess;
-        if (p) {
-            (void)hipHostFree(p);
-            p = nullptr;
-            n = 0;
-        }
-        if (want == 0) want = 1;
-        const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault);
-        if (e == hipSuccess) n = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
+using bpg::DBuf;
+using bpg::HBuf;
+using bpg::quantize;
+using bpg::empty_max;
+using bpg::elapsed;
 
 int pow2ceil(int x) {
     int r = 1;
@@ -131,7 +81,8 @@ struct bpg_plan {
     DBuf<float> partial, maxout;
     DBuf<float> stats;
     DBuf<unsigned> bits;
-    DBuf<unsigned> parent, sizes, maxsize, labels, overflow;
+    DBuf<unsigned> maxsize, labels;
+    bpg::ClusterScratch cl;
     DBuf<float> tfce, sqrt_size;
     DBuf<int> nlev;
     DBuf<unsigned> unsafe;
@@ -151,26 +102,11 @@ struct bpg_plan {
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
-        parent.release(); sizes.release(); maxsize.release(); labels.release(); overflow.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
+        cl.release(); maxsize.release(); labels.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
         d_obs.release(); sorted.release(); P.release(); glm_out.release();
     }
 };
-
-#define BPG_HIP(ctx, call)                                                         \
-    do {                                                                           \
-        const hipError_t e_ = (call);                                              \
-        if (e_ != hipSuccess) {                                                    \
-            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
-            return e_ == hipErrorOutOfMemory ? BPG_ERR_NOMEM : BPG_ERR_HIP;        \
-        }                                                                          \
-    } while (0)
-
-#define BPG_FAIL(ctx, code, msg) \
-    do {                         \
-        (ctx)->err = (msg);      \
-        return (code);           \
-    } while (0)
 
 extern "C" int bpg_create(int device, bpg_ctx** out) {
     if (!out) return BPG_ERR_ARG;
@@ -261,12 +197,8 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
     if (d.precision == BPG_PREC_SPLIT_BF16 && max_pfit > 16) pl->d.precision = BPG_PREC_FP32;
     // compaction: in-mask voxels in linear order (mask == 1.0f, K2:1586)
     pl->G = (size_t)d.W * d.H * d.D;
-    std::vector<int> grid2comp(pl->G, -1);
-    for (size_t g = 0; g < pl->G; g++)
-        if (d.mask[g] == 1.0f) {
-            grid2comp[g] = (int)pl->comp2grid.size();
-            pl->comp2grid.push_back((unsigned)g);
-        }
+    std::vector<int> grid2comp;
+    bpg::compact_mask(d.mask, pl->G, pl->comp2grid, grid2comp);
     pl->Vc = (int)pl->comp2grid.size();
     pl->Vpad = std::max(32, (pl->Vc + 255) / 256 * 256);
     pl->nwords = pl->Vpad / 32;
@@ -283,23 +215,7 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
     // cluster inference: compact indices of the 13 lower-index 26-neighbours
     // of every in-mask voxel (face neighbours first), -1 outside the mask
     std::vector<int> nbr;
-    if (d.inference != BPG_VOXEL) {
-        static const int off[bpg::BPG_NBR][3] = {
-            {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},                                   // faces
-            {-1, -1, 0}, {1, -1, 0},                                              // dz = 0 edges
-            {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1},    // dz = -1
-            {-1, 1, -1}, {0, 1, -1}, {1, 1, -1}};
-        nbr.assign((size_t)std::max(pl->Vc, 1) * bpg::BPG_NBR, -1);
-        for (int c = 0; c < pl->Vc; c++) {
-            const size_t g = pl->comp2grid[c];
-            const int x = (int)(g % d.W), y = (int)((g / d.W) % d.H), z = (int)(g / ((size_t)d.W * d.H));
-            for (int j = 0; j < bpg::BPG_NBR; j++) {
-                const int xx = x + off[j][0], yy = y + off[j][1], zz = z + off[j][2];
-                if (xx < 0 || yy < 0 || zz < 0 || xx >= d.W || yy >= d.H || zz >= d.D) continue;
-                nbr[(size_t)c * bpg::BPG_NBR + j] = grid2comp[xx + (size_t)yy * d.W + (size_t)zz * d.W * d.H];
-            }
-        }
-    }
+    if (d.inference != BPG_VOXEL) bpg::build_neighbours(d.W, d.H, d.D, pl->comp2grid, grid2comp, nbr);
     if (pl->Yc.ensure(yc.size()) || (!nbr.empty() && pl->d_nbr.ensure(nbr.size()))) {
         ctx->err = "device allocation of the study failed";
         return fail(BPG_ERR_NOMEM);
@@ -327,30 +243,6 @@ extern "C" void bpg_plan_destroy(bpg_plan* pl) {
 extern "C" long bpg_plan_num_voxels(const bpg_plan* pl) { return pl ? pl->Vc : -1; }
 
 namespace {
-
-// (int)(T * 10000.0f) as KM:299, made total (saturating, NaN -> INT_MIN).
-int quantize(float t) {
-    const float x = t * 10000.0f;
-    if (std::isnan(x)) return INT_MIN;
-    if (x >= 2147483648.0f) return INT_MAX;
-    if (x <= -2147483648.0f) return INT_MIN;
-    return (int)x;
-}
-
-// The empty-mask value of LIB:10971: the int buffer starts from the bit
-// pattern of -1e6f and is returned as (float)max / 10000.0f.
-float empty_max() {
-    const float init = -1000000.0f;
-    int m;
-    std::memcpy(&m, &init, sizeof m);
-    return (float)((float)m / 10000.0f);
-}
-
-float elapsed(hipEvent_t a, hipEvent_t b) {
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    return ms;
-}
 
 }  // namespace
 
@@ -504,47 +396,15 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     unsigned* host = reinterpret_cast<unsigned*>(pl->h_max.p);
     const float* hmax = pl->h_max.p;
     if (cluster) {
-        // batches of maps, each with a compact parent / size array
-        const size_t budget = (size_t)8 << 30;
-        const size_t per_map = (size_t)std::max(Vc, 1) * 2 * sizeof(unsigned);
-        long B = (long)std::max<size_t>(1, budget / per_map);
-        B = std::min<long>(B, K);
-        BPG_HIP(ctx, pl->parent.ensure((size_t)B * std::max(Vc, 1)));
-        BPG_HIP(ctx, pl->sizes.ensure((size_t)B * std::max(Vc, 1)));
         BPG_HIP(ctx, pl->maxsize.ensure(K));
-        BPG_HIP(ctx, hipMemsetAsync(pl->maxsize.p, 0, K * sizeof(unsigned), st));
-        for (long k0 = 0; k0 < K; k0 += B) {
-            bpg::CclArgs cc{};
-            cc.bits = pl->bits.p + (size_t)k0 * pl->nwords;
-            cc.nbr = pl->d_nbr.p;
-            cc.parent = pl->parent.p;
-            cc.sizes = pl->sizes.p;
-            cc.maxsize = pl->maxsize.p + k0;
-            cc.labels = nullptr;
-            cc.values = mass ? pl->stats.p + (size_t)k0 * Vc : nullptr;
-            cc.Vc = Vc; cc.nwords = pl->nwords;
-            cc.nmaps = (int)std::min<long>(B, K - k0);
-            cc.connectivity = d.connectivity;
-            if (bpg::ccl_lds_fits(pl->nwords, ctx->lds_per_block)) {
-                // whole maps in LDS; the few with too many active voxels
-                // (listed by the kernel) take the global path
-                BPG_HIP(ctx, pl->overflow.ensure((size_t)cc.nmaps + 1));
-                BPG_HIP(ctx, hipMemsetAsync(pl->overflow.p + cc.nmaps, 0, sizeof(unsigned), st));
-                BPG_HIP(ctx, bpg::launch_ccl_lds(cc, ctx->lds_per_block, pl->overflow.p, pl->overflow.p + cc.nmaps,
-                                                 st));
-                unsigned n_over = 0;
-                BPG_HIP(ctx, hipMemcpyAsync(&n_over, pl->overflow.p + cc.nmaps, sizeof(unsigned),
-                                            hipMemcpyDeviceToHost, st));
-                BPG_HIP(ctx, hipStreamSynchronize(st));
-                if (n_over > 0) {
-                    cc.map_ids = pl->overflow.p;
-                    cc.nmaps = (int)n_over;
-                    BPG_HIP(ctx, bpg::launch_ccl(cc, st));
-                }
-            } else {
-                BPG_HIP(ctx, bpg::launch_ccl(cc, st));
-            }
-        }
+        bpg::CclArgs cc{};
+        cc.bits = pl->bits.p;
+        cc.nbr = pl->d_nbr.p;
+        cc.values = mass ? pl->stats.p : nullptr;
+        cc.Vc = Vc; cc.nwords = pl->nwords;
+        cc.connectivity = d.connectivity;
+        const int rc = bpg::cluster_max(ctx, pl->cl, cc, K, pl->maxsize.p, st);
+        if (rc != BPG_OK) return rc;
         BPG_HIP(ctx, hipEventRecord(pl->ev[4], st));
         BPG_HIP(ctx, hipMemcpyAsync(host, pl->maxsize.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     } else if (tfce && Vc > 0) {
@@ -573,8 +433,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         const size_t budget = (size_t)8 << 30;
         const size_t per_map = (size_t)Vc * 3 * sizeof(unsigned);
         const long B = std::min<long>(K, (long)std::max<size_t>(1, budget / per_map));
-        BPG_HIP(ctx, pl->parent.ensure((size_t)B * Vc));
-        BPG_HIP(ctx, pl->sizes.ensure((size_t)B * Vc));
+        BPG_HIP(ctx, pl->cl.parent.ensure((size_t)B * Vc));
+        BPG_HIP(ctx, pl->cl.sizes.ensure((size_t)B * Vc));
         BPG_HIP(ctx, pl->tfce.ensure((size_t)B * Vc));
         if (pl->sqrt_size.n < (size_t)Vc + 1) {
             // sqrt(n) for every possible component size, correctly rounded
@@ -589,8 +449,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             bpg::CclArgs cc{};
             cc.bits = pl->bits.p;
             cc.nbr = pl->d_nbr.p;
-            cc.parent = pl->parent.p;
-            cc.sizes = pl->sizes.p;
+            cc.parent = pl->cl.parent.p;
+            cc.sizes = pl->cl.sizes.p;
             cc.tfce = pl->tfce.p;
             cc.sqrt_size = pl->sqrt_size.p;
             cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = nb;
@@ -877,8 +737,8 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
     BPG_HIP(ctx, pl->P.ensure(Vc));
     if (cluster) {
         BPG_HIP(ctx, pl->bits.ensure(pl->nwords));
-        BPG_HIP(ctx, pl->parent.ensure(Vc));
-        BPG_HIP(ctx, pl->sizes.ensure(Vc));
+        BPG_HIP(ctx, pl->cl.parent.ensure(Vc));
+        BPG_HIP(ctx, pl->cl.sizes.ensure(Vc));
         BPG_HIP(ctx, pl->labels.ensure(Vc));
         BPG_HIP(ctx, pl->maxsize.ensure(1));
     }
@@ -899,7 +759,7 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
             BPG_HIP(ctx, hipMemsetAsync(pl->maxsize.p, 0, sizeof(unsigned), st));
             bpg::CclArgs cc{};
             cc.bits = pl->bits.p; cc.nbr = pl->d_nbr.p;
-            cc.parent = pl->parent.p; cc.sizes = pl->sizes.p; cc.maxsize = pl->maxsize.p;
+            cc.parent = pl->cl.parent.p; cc.sizes = pl->cl.sizes.p; cc.maxsize = pl->maxsize.p;
             cc.labels = pl->labels.p;
             cc.values = d.inference == BPG_CLUSTER_MASS ? obs : nullptr;
             cc.Vc = Vc; cc.nwords = pl->nwords; cc.nmaps = 1;
@@ -909,7 +769,7 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
         bpg::PvalArgs pa{};
         pa.obs = obs; pa.sorted_max = pl->sorted.p;
         pa.labels = cluster ? pl->labels.p : nullptr;
-        pa.sizes = cluster ? pl->sizes.p : nullptr;
+        pa.sizes = cluster ? pl->cl.sizes.p : nullptr;
         pa.K = (int)K[c]; pa.Vc = Vc; pa.inference = d.inference; pa.cdt = d.cdt; pa.P = pl->P.p;
         BPG_HIP(ctx, bpg::launch_pvalues(pa, st));
         BPG_HIP(ctx, hipMemcpyAsync(h.data(), pl->P.p, Vc * sizeof(float), hipMemcpyDeviceToHost, st));

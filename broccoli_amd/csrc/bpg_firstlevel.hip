@@ -1,0 +1,458 @@
+// bpg_firstlevel.hip -- gfx950 kernels of the first-level permutation test
+// (BROCCOLI_LIB::ApplyPermutationTestFirstLevel, LIB:16454-16558).
+//
+// Set-up, once per study, in FP64: the regression of the full design
+// (LIB:12144-12194), three rounds of AR(4) estimation, normalised smoothing
+// of the estimates and whitening (LIB:16363-16452).  Then one fused kernel
+// per batch of permutations: inverse whitening of the permuted whitened
+// volumes (kernelWhitening.cpp:480-535), the per-voxel GLM statistic
+// (kernelStatistics3.cpp:2107-2190, kernelStatistics5.cpp:1943-2060) and the
+// per-permutation max / supra-threshold bitmask, without writing the
+// permuted volumes anywhere.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bpg_device.h"
+#include "bpg_fl_kernels.h"
+
+namespace bpg {
+namespace fl {
+
+// Scalar-memory views: wave-uniform addresses in the constant address space
+// are read with scalar loads into SGPRs.
+typedef const __attribute__((address_space(4))) double* cdouble_p;
+typedef const __attribute__((address_space(4))) unsigned* cuint_p;
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_regress(const float* __restrict__ Y, const double* __restrict__ Xp,
+                                                 const double* __restrict__ X, int T, int p, int Vc, int Vpad,
+                                                 double* __restrict__ e) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Vpad) return;
+    if (v >= Vc) {
+        for (int t = 0; t < T; t++) e[(size_t)t * Vpad + v] = 0.0;
+        return;
+    }
+    double beta[MAXP];
+#pragma unroll
+    for (int r = 0; r < MAXP; r++) beta[r] = 0.0;
+    for (int t = 0; t < T; t++) {
+        const double y = (double)Y[(size_t)t * Vpad + v];
+#pragma unroll
+        for (int r = 0; r < MAXP; r++)
+            if (r < p) beta[r] = fma(Xp[(size_t)r * T + t], y, beta[r]);
+    }
+    for (int t = 0; t < T; t++) {
+        double acc = (double)Y[(size_t)t * Vpad + v];
+#pragma unroll
+        for (int r = 0; r < MAXP; r++)
+            if (r < p) acc = fma(-X[(size_t)t * p + r], beta[r], acc);
+        e[(size_t)t * Vpad + v] = acc;
+    }
+}
+
+hipError_t launch_regress(const float* Y, const double* Xp, const double* X, int T, int p, int Vc, int Vpad,
+                          double* e, hipStream_t s) {
+    hipLaunchKernelGGL(k_regress, dim3((Vpad + 255) / 256), dim3(256), 0, s, Y, Xp, X, T, p, Vc, Vpad, e);
+    return hipGetLastError();
+}
+
+// Solves M a = r for the 4x4 system by elimination with pivoting by
+// compare-and-swap (all indices compile-time, so everything stays in
+// registers); returns det(M).
+__device__ __forceinline__ double solve4(double (&A)[4][5], double (&sol)[4]) {
+    double det = 1.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int r = c + 1; r < 4; r++) {
+            const bool sw = fabs(A[r][c]) > fabs(A[c][c]);
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const double lo = A[c][j], hi = A[r][j];
+                A[c][j] = sw ? hi : lo;
+                A[r][j] = sw ? lo : hi;
+            }
+            det = sw ? -det : det;
+        }
+        det *= A[c][c];
+        const double piv = A[c][c];
+#pragma unroll
+        for (int r = c + 1; r < 4; r++) {
+            const double f = piv != 0.0 ? A[r][c] / piv : 0.0;
+#pragma unroll
+            for (int j = 0; j < 5; j++) A[r][j] = fma(-f, A[c][j], A[r][j]);
+        }
+    }
+#pragma unroll
+    for (int r = 3; r >= 0; r--) {
+        double acc = A[r][4];
+#pragma unroll
+        for (int j = r + 1; j < 4; j++) acc = fma(-A[r][j], sol[j], acc);
+        sol[r] = A[r][r] != 0.0 ? acc / A[r][r] : 0.0;
+    }
+    return det;
+}
+
+// kernelWhitening.cpp:103-235: lag sums c_k over t >= invalid + k, divided
+// by T - 1 - k - invalid; r_k = c_k / c0; the Toeplitz system of 1 and
+// r_k + 0.001 solved and scaled by det / (det + 0.001) (the adjugate over
+// det + 0.001 of Invert_4x4, kernelWhitening.cpp:59-98).
+__global__ __launch_bounds__(256) void k_ar4(const double* __restrict__ src, int T, int invalid, int Vc, int Vpad,
+                                             double* __restrict__ ar) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Vpad) return;
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    if (v < Vc) {
+        double c[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;  // v_{t-1}, .., v_{t-4}
+        for (int t = invalid; t < T; t++) {
+            const double x = src[(size_t)t * Vpad + v];
+            const int n = t - invalid;  // earlier samples available
+            c[0] = fma(x, x, c[0]);
+            if (n >= 1) c[1] = fma(x, h0, c[1]);
+            if (n >= 2) c[2] = fma(x, h1, c[2]);
+            if (n >= 3) c[3] = fma(x, h2, c[3]);
+            if (n >= 4) c[4] = fma(x, h3, c[4]);
+            h3 = h2;
+            h2 = h1;
+            h1 = h0;
+            h0 = x;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) c[k] /= ((double)T - 1.0 - (double)k - (double)invalid);
+        if (c[0] != 0.0) {
+            const double r1 = c[1] / c[0], r2 = c[2] / c[0], r3 = c[3] / c[0], r4 = c[4] / c[0];
+            const double a = r1 + 0.001, b = r2 + 0.001, d = r3 + 0.001;
+            double A[4][5] = {{1.0, a, b, d, r1}, {a, 1.0, a, b, r2}, {b, a, 1.0, a, r3}, {d, b, a, 1.0, r4}};
+            double sol[4];
+            const double det = solve4(A, sol);
+            const double scale = det / (det + 0.001);
+#pragma unroll
+            for (int k = 0; k < 4; k++) out[k] = sol[k] * scale;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) ar[(size_t)k * Vpad + v] = out[k];
+}
+
+hipError_t launch_ar4(const double* src, int T, int invalid, int Vc, int Vpad, double* ar, hipStream_t s) {
+    hipLaunchKernelGGL(k_ar4, dim3((Vpad + 255) / 256), dim3(256), 0, s, src, T, invalid, Vc, Vpad, ar);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ ar, const unsigned* __restrict__ c2g,
+                                                 int Vc, int Vpad, size_t G, double* __restrict__ g) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Vc) return;
+    const size_t i = c2g[v];
+#pragma unroll
+    for (int k = 0; k < 4; k++) g[(size_t)k * G + i] = ar[(size_t)k * Vpad + v];
+}
+
+hipError_t launch_scatter(const double* ar, const unsigned* comp2grid, int Vc, int Vpad, size_t G, double* g,
+                          hipStream_t s) {
+    if (Vc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((Vc + 255) / 256), dim3(256), 0, s, ar, comp2grid, Vc, Vpad, G, g);
+    return hipGetLastError();
+}
+
+// kernelConvolution.cpp:52-91 / 531-569 / 1201-1240 without the
+// certainty factors (applied by the caller's input and the rods pass)
+__global__ __launch_bounds__(256) void k_conv(const double* __restrict__ in, double* __restrict__ out, int nf,
+                                              int W, int H, int D, int axis, Taps taps) {
+    const size_t G = (size_t)W * H * D;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nf * G) return;
+    const size_t f = idx / G, gi = idx - f * G;
+    const int x = (int)(gi % W), y = (int)((gi / W) % H), z = (int)(gi / ((size_t)W * H));
+    const int pos = axis == 0 ? x : axis == 1 ? y : z;
+    const int len = axis == 0 ? W : axis == 1 ? H : D;
+    const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)W : (size_t)W * H;
+    const double* base = in + f * G;
+    double acc = 0.0;
+#pragma unroll
+    for (int o = -4; o <= 4; o++) {
+        const int q = pos + o;
+        if (q >= 0 && q < len) acc = fma(base[gi + (ptrdiff_t)o * (ptrdiff_t)stride], (double)taps.f[4 - o], acc);
+    }
+    out[idx] = acc;
+}
+
+hipError_t launch_conv(const double* in, double* out, int nf, int W, int H, int D, int axis, const Taps& taps,
+                       hipStream_t s) {
+    const size_t total = (size_t)nf * W * H * D;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_conv, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, nf, W, H, D, axis,
+                       taps);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_rods_accumulate(const double* __restrict__ g,
+                                                         const double* __restrict__ smask,
+                                                         const unsigned* __restrict__ c2g, int Vc, int Vpad, int W,
+                                                         int H, int D, Taps taps, double* __restrict__ total) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 4 * Vc) return;
+    const int k = idx / Vc, v = idx - k * Vc;
+    const size_t G = (size_t)W * H * D;
+    const size_t gi = c2g[v];
+    const int z = (int)(gi / ((size_t)W * H));
+    const size_t plane = (size_t)W * H;
+    const double* base = g + (size_t)k * G;
+    double acc = 0.0;
+#pragma unroll
+    for (int o = -4; o <= 4; o++) {
+        const int q = z + o;
+        if (q >= 0 && q < D) acc = fma(base[gi + (ptrdiff_t)o * (ptrdiff_t)plane], (double)taps.f[4 - o], acc);
+    }
+    // in-mask voxels: the final multiplication by the mask is by 1
+    total[(size_t)k * Vpad + v] += acc / smask[gi];
+}
+
+hipError_t launch_rods_accumulate(const double* g, const double* smask, const unsigned* comp2grid, int Vc,
+                                  int Vpad, int W, int H, int D, const Taps& taps, double* total, hipStream_t s) {
+    if (Vc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rods_accumulate, dim3((4 * Vc + 255) / 256), dim3(256), 0, s, g, smask, comp2grid, Vc,
+                       Vpad, W, H, D, taps, total);
+    return hipGetLastError();
+}
+
+// kernelWhitening.cpp:371-423
+__global__ __launch_bounds__(256) void k_whiten(const double* __restrict__ e, const double* __restrict__ ar, int T,
+                                                int Vc, int Vpad, double* __restrict__ wh, float* __restrict__ w32) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Vpad) return;
+    const double a1 = ar[v], a2 = ar[(size_t)Vpad + v], a3 = ar[2 * (size_t)Vpad + v], a4 = ar[3 * (size_t)Vpad + v];
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;
+    for (int t = 0; t < T; t++) {
+        const double x = v < Vc ? e[(size_t)t * Vpad + v] : 0.0;
+        double o = x;
+        o = fma(-a1, h0, o);
+        o = fma(-a2, h1, o);
+        o = fma(-a3, h2, o);
+        o = fma(-a4, h3, o);
+        h3 = h2;
+        h2 = h1;
+        h1 = h0;
+        h0 = x;
+        wh[(size_t)t * Vpad + v] = o;
+        if (w32) w32[(size_t)t * Vpad + v] = (float)o;
+    }
+}
+
+hipError_t launch_whiten(const double* e, const double* ar, int T, int Vc, int Vpad, double* wh, float* w32,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_whiten, dim3((Vpad + 255) / 256), dim3(256), 0, s, e, ar, T, Vc, Vpad, wh, w32);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// the permutation kernel
+// ---------------------------------------------------------------------------
+// Block: 8 waves over one 64-voxel tile at a time (lane = voxel) and a
+// group of 64 permutations (each wave 4 rounds of 2); blocks of one voxel
+// chunk walk its tiles.  The tile's whitened volumes ([T][64] floats) sit
+// in LDS when T <= LDS_T_MAX; permutation indices and the design are
+// wave-uniform scalar loads.  Each thread keeps, per permutation, 4 AR
+// history values, pq projections, |x|^2 and sum x in FP64.
+
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+    return x;
+}
+
+template <int PQ, bool USE_LDS>
+__global__ __launch_bounds__(512) void k_fl_perm(PermArgs a) {
+    extern __shared__ float tile[];
+    __shared__ float smax[WAVES * ROUNDS * NP];
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const int lid = xcd_logical_block((int)(gridDim.x >> 3));
+    if (lid >= n_blocks) return;
+    const int vchunk = lid / a.n_pgroups;
+    const int pgroup = lid - vchunk * a.n_pgroups;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile0 = vchunk * a.tiles_per_chunk;
+    const int tile1 = min(tile0 + a.tiles_per_chunk, a.n_vtiles);
+    const cdouble_p Q = (cdouble_p)a.Q;
+    const cdouble_p A = (cdouble_p)a.A;
+    const cdouble_p U = (cdouble_p)a.u;
+    const double invT = 1.0 / (double)a.T, invTm1 = 1.0 / ((double)a.T - 1.0);
+    float* my = smax + wave * (ROUNDS * NP);
+    if (lane == 0)
+        for (int i = 0; i < ROUNDS * NP; i++) my[i] = -INFINITY;
+
+    for (int tl = tile0; tl < tile1; tl++) {
+        const int v = tl * TILE + lane;
+        if (USE_LDS) {
+            __syncthreads();
+            const int n = a.T * TILE;
+            for (int i = threadIdx.x; i < n; i += WAVES * 64)
+                tile[i] = a.w32[(size_t)(i >> 6) * a.Vpad + (size_t)tl * TILE + (i & 63)];
+            __syncthreads();
+        }
+        const double a1 = a.ar[v], a2 = a.ar[(size_t)a.Vpad + v];
+        const double a3 = a.ar[2 * (size_t)a.Vpad + v], a4 = a.ar[3 * (size_t)a.Vpad + v];
+        const bool in_mask = v < a.Vc;
+        for (int r = 0; r < ROUNDS; r++) {
+            const int kbase = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP;
+            if (kbase >= a.K) break;
+            cuint_p prow[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = min(kbase + j, a.K - 1);
+                prow[j] = (cuint_p)(a.perm + (size_t)k * a.Tp);
+            }
+            double z[NP][PQ], h[NP][4], s[NP], m[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+#pragma unroll
+                for (int q = 0; q < PQ; q++) z[j][q] = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) h[j][q] = 0.0;
+                s[j] = 0.0;
+                m[j] = 0.0;
+            }
+            for (int t8 = 0; t8 < a.Tp; t8 += 8) {
+                unsigned pk[NP][4];  // 8 indices of each permutation
+#pragma unroll
+                for (int j = 0; j < NP; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) pk[j][q] = prow[j][(t8 >> 1) + q];
+#pragma unroll
+                for (int tt = 0; tt < 8; tt++) {
+                    const int t = t8 + tt;
+                    if (t >= a.T) break;
+                    double qv[PQ];
+#pragma unroll
+                    for (int q = 0; q < PQ; q++) qv[q] = Q[(size_t)t * PQ + q];
+#pragma unroll
+                    for (int j = 0; j < NP; j++) {
+                        const unsigned idx = (pk[j][tt >> 1] >> ((tt & 1) * 16)) & 0xFFFFu;
+                        const float w = USE_LDS ? tile[idx * TILE + lane] : a.w32[(size_t)idx * a.Vpad + v];
+                        double x = (double)w;
+                        x = fma(a4, h[j][3], x);
+                        x = fma(a3, h[j][2], x);
+                        x = fma(a2, h[j][1], x);
+                        x = fma(a1, h[j][0], x);
+                        h[j][3] = h[j][2];
+                        h[j][2] = h[j][1];
+                        h[j][1] = h[j][0];
+                        h[j][0] = x;
+                        s[j] = fma(x, x, s[j]);
+                        m[j] += x;
+#pragma unroll
+                        for (int q = 0; q < PQ; q++) z[j][q] = fma(qv[q], x, z[j][q]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = kbase + j;
+                const bool valid = k < a.K;
+                double ss = 0.0, uz = 0.0;
+#pragma unroll
+                for (int q = 0; q < PQ; q++) {
+                    ss = fma(z[j][q], z[j][q], ss);
+                    uz = fma(U[q], z[j][q], uz);
+                }
+                const double sse = s[j] - ss;
+                const double me = m[j] - uz;
+                const double var = (sse - me * me * invT) * invTm1;
+                double stat;
+                if (!a.ftest) {
+                    double d = 0.0;
+#pragma unroll
+                    for (int q = 0; q < PQ; q++) d = fma(A[q], z[j][q], d);
+                    stat = d / sqrt(var);
+                } else {
+                    double acc = 0.0;
+                    for (int i = 0; i < a.nrows; i++) {
+                        double d = 0.0;
+#pragma unroll
+                        for (int q = 0; q < PQ; q++) d = fma(A[(size_t)i * PQ + q], z[j][q], d);
+                        acc = fma(d, d, acc);
+                    }
+                    stat = acc / (var * (double)a.nrows);
+                }
+                const float st = (float)stat;
+                const bool act = in_mask && valid;
+                if (a.stats && act) a.stats[(size_t)k * a.Vc + v] = st;
+                if (a.bits) {
+                    const unsigned long long bal = __ballot(act && st > a.cdt);
+                    if (lane == 0 && valid) {
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl] = (unsigned)bal;
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl + 1] = (unsigned)(bal >> 32);
+                    }
+                }
+                const float mx = wave_max(act ? st : -INFINITY);
+                if (lane == 0) my[r * NP + j] = fmaxf(my[r * NP + j], mx);
+            }
+        }
+    }
+    if (lane == 0) {
+        for (int r = 0; r < ROUNDS; r++)
+            for (int j = 0; j < NP; j++) {
+                const int k = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP + j;
+                if (k < a.K) a.partial[(size_t)vchunk * a.K + k] = my[r * NP + j];
+            }
+    }
+}
+
+int fl_width(int p) {
+    static const int widths[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 25};
+    for (int w : widths)
+        if (p <= w) return w;
+    return -1;
+}
+
+template <int PQ>
+hipError_t launch_perm_w(const PermArgs& a, hipStream_t s) {
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
+    if (a.T <= LDS_T_MAX) {
+        const size_t lds = (size_t)a.T * TILE * sizeof(float);
+        static bool attr = false;
+        if (!attr) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fl_perm<PQ, true>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)(LDS_T_MAX * TILE * sizeof(float)));
+            if (e != hipSuccess) return e;
+            attr = true;
+        }
+        hipLaunchKernelGGL((k_fl_perm<PQ, true>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+    } else {
+        hipLaunchKernelGGL((k_fl_perm<PQ, false>), dim3(grid), dim3(WAVES * 64), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_perm(PermArgs a, hipStream_t s) {
+    if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
+    switch (a.pq) {
+        case 1: return launch_perm_w<1>(a, s);
+        case 2: return launch_perm_w<2>(a, s);
+        case 3: return launch_perm_w<3>(a, s);
+        case 4: return launch_perm_w<4>(a, s);
+        case 5: return launch_perm_w<5>(a, s);
+        case 6: return launch_perm_w<6>(a, s);
+        case 7: return launch_perm_w<7>(a, s);
+        case 8: return launch_perm_w<8>(a, s);
+        case 10: return launch_perm_w<10>(a, s);
+        case 12: return launch_perm_w<12>(a, s);
+        case 14: return launch_perm_w<14>(a, s);
+        case 16: return launch_perm_w<16>(a, s);
+        case 20: return launch_perm_w<20>(a, s);
+        case 25: return launch_perm_w<25>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fl
+}  // namespace bpg

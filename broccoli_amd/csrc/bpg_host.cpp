@@ -240,6 +240,96 @@ int build_map_model(const bpg_study_desc& d, int map, MapModel& m, std::string& 
     return BPG_OK;
 }
 
+// First-level model: X = Q1 R1 (thin QR in FP64), z = Q1' x gives
+// c'beta = (R1^-T c)' z and c'(X'X)^-1 c = |R1^-T c|^2, so the t statistic
+// row is d / |d| with d = R1^-T c.  For F, D = C R1^-1 (rows R1^-T C_i) and
+// C(X'X)^-1C' = D D' = L L' (Cholesky), so (C beta)'(C(X'X)^-1C')^-1(C beta)
+// = |L^-1 D z|^2 and the rows are L^-1 D.
+int build_fl_model(int T, int p, int nc, int ftest, const float* X, const float* C, int pq, FlModel& m,
+                   std::string& err) {
+    if (pq < p) { err = "internal: design width"; return BPG_ERR_ARG; }
+    const std::vector<double> Xd = [&] {
+        std::vector<double> r((size_t)T * p);
+        for (size_t i = 0; i < r.size(); i++) r[i] = (double)X[i];  // X[t + r*T]
+        return r;
+    }();
+    std::vector<double> Q1, R1;
+    if (!thin_qr(T, p, Xd, Q1, R1)) { err = "the design matrix is rank deficient"; return BPG_ERR_ARG; }
+    m = FlModel{};
+    m.T = T; m.p = p; m.pq = pq; m.ftest = ftest != 0;
+    m.Q.assign((size_t)T * pq, 0.0);
+    m.u.assign(pq, 0.0);
+    for (int t = 0; t < T; t++)
+        for (int j = 0; j < p; j++) {
+            m.Q[(size_t)t * pq + j] = Q1[t + (size_t)j * T];
+            m.u[j] += Q1[t + (size_t)j * T];
+        }
+    // pinv(X) = R1^-1 Q1'
+    std::vector<double> Ri((size_t)p * p, 0.0);  // column-major R1^-1
+    for (int c = 0; c < p; c++) {
+        for (int r = p - 1; r >= 0; r--) {
+            double acc = (r == c) ? 1.0 : 0.0;
+            for (int j = r + 1; j < p; j++) acc -= R1[r + (size_t)j * p] * Ri[j + (size_t)c * p];
+            Ri[r + (size_t)c * p] = acc / R1[r + (size_t)r * p];
+        }
+    }
+    m.Xp.assign((size_t)p * T, 0.0);
+    m.X.assign((size_t)T * p, 0.0);
+    for (int t = 0; t < T; t++)
+        for (int r = 0; r < p; r++) {
+            double acc = 0.0;
+            for (int j = 0; j < p; j++) acc += Ri[r + (size_t)j * p] * Q1[t + (size_t)j * T];
+            m.Xp[(size_t)r * T + t] = acc;
+            m.X[(size_t)t * p + r] = Xd[t + (size_t)r * T];
+        }
+    std::vector<std::vector<double>> D(nc, std::vector<double>(p));
+    for (int c = 0; c < nc; c++) {
+        std::vector<double> cc(p);
+        for (int r = 0; r < p; r++) cc[r] = (double)C[r + (size_t)c * p];
+        solve_rt(p, R1, cc.data(), D[c].data());
+    }
+    if (!ftest) {
+        m.nmaps = nc;
+        for (int c = 0; c < nc; c++) {
+            double n2 = 0.0;
+            for (double v : D[c]) n2 += v * v;
+            if (!(n2 > 0.0)) { err = "a contrast is zero"; return BPG_ERR_ARG; }
+            std::vector<double> row(pq, 0.0);
+            for (int j = 0; j < p; j++) row[j] = D[c][j] / std::sqrt(n2);
+            m.A.push_back(row);
+            m.nrows.push_back(1);
+        }
+    } else {
+        m.nmaps = 1;
+        std::vector<double> L((size_t)nc * nc, 0.0);  // row-major lower
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j <= i; j++) {
+                double g = 0.0;
+                for (int q = 0; q < p; q++) g += D[i][q] * D[j][q];
+                for (int k = 0; k < j; k++) g -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
+                if (i == j) {
+                    if (!(g > 1e-12 * std::max(1.0, std::fabs(g)))) {
+                        err = "the F contrasts are linearly dependent";
+                        return BPG_ERR_ARG;
+                    }
+                    L[(size_t)i * nc + i] = std::sqrt(g);
+                } else {
+                    L[(size_t)i * nc + j] = g / L[(size_t)j * nc + j];
+                }
+            }
+        std::vector<double> rows((size_t)nc * pq, 0.0);
+        for (int i = 0; i < nc; i++)
+            for (int q = 0; q < p; q++) {
+                double acc = D[i][q];
+                for (int k = 0; k < i; k++) acc -= L[(size_t)i * nc + k] * rows[(size_t)k * pq + q];
+                rows[(size_t)i * pq + q] = acc / L[(size_t)i * nc + i];
+            }
+        m.A.push_back(rows);
+        m.nrows.push_back(nc);
+    }
+    return BPG_OK;
+}
+
 int build_obs_model(const bpg_study_desc& d, ObsModel& o, std::string& err) {
     const int N = d.N;
     const bool mean = d.test == BPG_GROUP_MEAN;
@@ -451,6 +541,27 @@ extern "C" int bpg_gen_perm_correlation(int N, long K, int compat, unsigned long
             if (seen.insert(key_of(cur.data(), N)).second) break;
         }
         std::copy(cur.begin(), cur.end(), perm + (size_t)k * N);
+    }
+    return BPG_OK;
+}
+
+extern "C" int bpg_gen_perm_first_level(int T, long K, int compat, unsigned long long seed,
+                                        unsigned short* perm) {
+    if (T <= 1 || T > 65535 || K <= 0 || !perm) return BPG_ERR_ARG;
+    if ((double)K + 1.0 > distinct_count(2, T, 0)) return BPG_ERR_ARG;
+    std::vector<unsigned short> cur(T);
+    for (int i = 0; i < T; i++) cur[i] = (unsigned short)i;
+    std::unordered_set<std::string> seen;
+    seen.insert(key_of(cur.data(), T));  // the identity is never a row
+    SplitMix64 rng{seed};
+    if (compat && seed) compat_stream().seed((unsigned)seed);
+    for (long k = 0; k < K; k++) {
+        for (;;) {
+            if (compat) compat_shuffle(cur.data(), T);
+            else seeded_shuffle(cur.data(), T, rng);
+            if (seen.insert(key_of(cur.data(), T)).second) break;
+        }
+        std::copy(cur.begin(), cur.end(), perm + (size_t)k * T);
     }
     return BPG_OK;
 }

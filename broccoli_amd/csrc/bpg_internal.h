@@ -42,7 +42,25 @@ struct ObsModel {
     std::vector<double> sc;   // t: c(X'X)^-1c' per contrast; F: (C(X'X)^-1C')^-1
 };
 
+// First-level model (kernelStatistics3.cpp:2107-2190,
+// kernelStatistics5.cpp:1943-2060) in FP64: an orthonormal basis Q of
+// col(X), Q'1 and, per statistical map, the rows A with t = A_0 z / sd and
+// F = |A z|^2 / (var nrows) for z = Q'x.  Also pinv(X) and X for the
+// regression of the full design (LIB:12144-12194).
+struct FlModel {
+    int T = 0, p = 0, pq = 0, nmaps = 1;
+    bool ftest = false;
+    std::vector<double> Q;               // [t*pq + j], zero beyond p
+    std::vector<double> u;               // [pq]
+    std::vector<std::vector<double>> A;  // per map: [i*pq + j]
+    std::vector<int> nrows;              // per map
+    std::vector<double> Xp;              // [r*T + t] = pinv(X)(r, t)
+    std::vector<double> X;               // [t*p + r]
+};
+
 int householder_qr(int m, int n, const double* A, double* Q, double* R);
+int build_fl_model(int T, int p, int nc, int ftest, const float* X, const float* C, int pq, FlModel& m,
+                   std::string& err);
 int build_map_model(const bpg_study_desc& d, int map, MapModel& out, std::string& err);
 int build_obs_model(const bpg_study_desc& d, ObsModel& out, std::string& err);
 

@@ -156,3 +156,70 @@ def studies_batch(n: int = 64, scale: float = 1.0, seed: int | None = None, K: i
         st["V"] = int(mask.sum())
         out.append(st)
     return out
+
+
+# ------------------------------------------------------------- first level
+# Single-subject fMRI data for the first-level permutation test: volumes are
+# (T, D, H, W) arrays, i.e. volumes[x + y*W + z*W*H + t*W*H*D].
+
+def ar_series(coefs, T: int, shape, seed: int = 0, rng=None) -> np.ndarray:
+    """AR(4) noise v_t = e_t + sum_k a_k v_{t-k} from zero history, per
+    voxel; coefs is 4 values or a (4, *shape) field."""
+    rng = rng if rng is not None else np.random.default_rng(seed)
+    a = np.asarray(coefs, np.float64)
+    if a.ndim == 1:
+        a = a.reshape((4,) + (1,) * len(shape))
+    v = np.zeros((T,) + tuple(shape))
+    for t in range(T):
+        acc = rng.standard_normal(shape)
+        for k in range(1, 5):
+            if t - k >= 0:
+                acc = acc + a[k - 1] * v[t - k]
+        v[t] = acc
+    return v
+
+
+def first_level_design(T: int, p: int = 5, seed: int = 0) -> np.ndarray:
+    """(T, p) design: intercept, linear trend, a block task (on/off blocks of
+    10), a sinusoid, then Gaussian nuisance columns (e.g. motion)."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(T, dtype=np.float64)
+    cols = [np.ones(T), (t - t.mean()) / T, ((t // 10) % 2).astype(np.float64), np.sin(2 * np.pi * t / 37.0)]
+    while len(cols) < p:
+        cols.append(rng.standard_normal(T) * 0.5)
+    return np.stack(cols[:p], axis=1).astype(np.float32)
+
+
+def first_level_study(seed: int = 0, W: int = 12, H: int = 14, D: int = 10, T: int = 64, p: int = 5,
+                      ar=(0.3, 0.1, 0.0, 0.0), noise_only: bool = False, voxel=(3.0, 3.0, 3.0),
+                      ar_gradient: float = 0.0) -> dict:
+    """A synthetic single-subject study: ellipsoid brain mask, AR(4) noise
+    (AR1 varying by `ar_gradient` across x), activation 0.6 * task in a
+    sphere unless noise_only (and p >= 3), baseline 100.  Contrast: the task
+    column (the intercept when p == 1)."""
+    rng = np.random.default_rng(seed)
+    mask = ellipsoid_mask(W, H, D, ((W - 1) / 2, (H - 1) / 2, (D - 1) / 2), (W * 0.45, H * 0.45, D * 0.45))
+    X = first_level_design(T, p, seed + 1)
+    a = np.zeros((4, D, H, W))
+    for k in range(4):
+        a[k] = ar[k]
+    if ar_gradient:
+        a[0] += ar_gradient * (np.arange(W) / max(W - 1, 1) - 0.5)[None, None, :]
+    Y = ar_series(a, T, (D, H, W), rng=rng)
+    task = min(2, p - 1)  # the block-task column (the intercept when p == 1)
+    if not noise_only and task == 2:
+        act = sphere(W, H, D, (W * 0.3, H * 0.5, D * 0.5), max(2.0, W * 0.15))
+        Y += 0.6 * X[:, 2][:, None, None, None] * act[None]
+    Y += 100.0
+    C = np.zeros((1, p), np.float32)
+    C[0, task] = 1.0
+    return {"Y": Y.astype(np.float32), "mask": mask, "X": X, "C": C, "voxel": tuple(voxel), "W": W, "H": H,
+            "D": D, "T": T, "p": p}
+
+
+def first_level_workload(seed: int = 20260006) -> dict:
+    """The first-level benchmark study: 64x64x33 EPI grid (3.75 x 3.75 x 4
+    mm voxels), ellipsoid brain mask, T = 240, p = 12, AR(4) noise whose AR1
+    varies across the brain, block-task activation."""
+    return first_level_study(seed=seed, W=64, H=64, D=33, T=240, p=12, ar=(0.35, 0.1, 0.05, 0.0),
+                             voxel=(3.75, 3.75, 4.0), ar_gradient=0.15)

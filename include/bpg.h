@@ -1,5 +1,6 @@
 /*
- * bpg.h -- C ABI of the MI355X second-level permutation-test GLM.
+ * bpg.h -- C ABI of the MI355X permutation-test GLM: the second-level
+ * (group) test and the first-level (single-subject) test.
  *
  * This is the thin layer between the BROCCOLI_LIB drop-in class
  * (include/broccoli_lib.h) and the CDNA4 HIP kernels.  Plain pointers and
@@ -52,6 +53,7 @@ extern "C" {
 /* arithmetic of the permutation contraction */
 #define BPG_PREC_FP32 0       /* FP32 VALU contraction                         */
 #define BPG_PREC_SPLIT_BF16 1 /* hi/lo BF16 MFMA with FP32 accumulation        */
+#define BPG_PREC_FP64 2       /* FP64 VALU (the first-level permutation kernel) */
 
 /* limits of the reference (RGL:630-636 p <= 25, RGL:726-732 F rows <= 10) */
 #define BPG_MAX_REGRESSORS 25
@@ -198,6 +200,76 @@ double bpg_max_permutations(int test, int design, int N, int n2);
  * xtxxt[s + r*N] = pinv(X)(r,s); ctxtxc[c] (t) or [c + cc*nc] (F). */
 int bpg_model_precompute(int N, int p, int nc, int ftest, const float* X,
                          const float* C, float* xtxxt, float* ctxtxc);
+
+/* ======================================================================
+ * First-level (single-subject) permutation test:
+ * BROCCOLI_LIB::ApplyPermutationTestFirstLevel (LIB:16454-16558), a private
+ * member that PerformFirstLevelAnalysisWrapper runs when
+ * SetPermuteFirstLevel(true) (HDR:143, LIB:8996-9075).  The volumes are the
+ * subject's preprocessed fMRI data; the design is the full first-level
+ * design (c_X_GLM, LIB:8795-8837: task regressors, detrending and motion
+ * regressors), T time points.
+ *   volumes[x + y*W + z*W*H + t*W*H*D]   mask[x + y*W + z*W*H] (== 1.0f)
+ *   X[t + r*T]   C[r + c*p]   perm[t + k*T] (0-based uint16)
+ *   AR estimates ar[k*W*H*D + i], k = 0..3 for AR1..AR4
+ * ====================================================================== */
+typedef struct bpg_fl_plan bpg_fl_plan;
+
+typedef struct {
+    int test;               /* BPG_TTEST / BPG_FTEST                          */
+    int inference;          /* BPG_VOXEL / BPG_CLUSTER_EXTENT /
+                               BPG_CLUSTER_MASS (the reference leaves the
+                               TFCE distribution unset, LIB:16524-16530)      */
+    int connectivity;       /* 26 (the reference's) or 6                      */
+    int W, H, D, T;         /* EPI grid and time points (10 <= T <= 65535)    */
+    int p;                  /* regressors, <= BPG_MAX_REGRESSORS              */
+    int nc;                 /* t: contrasts; F: contrast rows (<= 10)         */
+    const float* volumes;   /* host, T*W*H*D                                  */
+    const float* mask;      /* host, W*H*D                                    */
+    const float* X;         /* host, T*p                                      */
+    const float* C;         /* host, p*nc                                     */
+    float voxel_size[3];    /* EPI voxel size (mm): SetEPIVoxelSizeX/Y/Z,
+                               HDR:214-216                                    */
+    float ar_smoothing_fwhm;/* mm, SetARSmoothingAmount (HDR:177); the
+                               reference's default is 8 (LIB:294)            */
+    float cdt;              /* cluster-defining threshold                     */
+} bpg_fl_desc;
+
+/* Time-point permutations (GeneratePermutationMatrixFirstLevel,
+ * LIB:16877-16922): K rows, each a cumulative random_shuffle of 0..T-1
+ * distinct from the identity and every earlier row (the identity is never
+ * a row).  compat / seed as the generators above. */
+int bpg_gen_perm_first_level(int T, long K, int compat, unsigned long long seed, unsigned short* perm);
+
+/* Uploads the in-mask voxels and runs, on the device in FP64, the
+ * regression of the full design (PerformRegression, LIB:12144-12194) and the
+ * whitening prior to the permutations (PerformWhiteningPriorPermutations,
+ * LIB:16363-16452: three rounds of AR(4) estimation, normalised smoothing of
+ * the estimates, accumulation and whitening).  Synchronous. */
+int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_plan** out);
+void bpg_fl_plan_destroy(bpg_fl_plan* plan);
+long bpg_fl_plan_num_voxels(const bpg_fl_plan* plan);
+/* The total AR(4) estimates (d_AR1..4_Estimates after LIB:16436-16445):
+ * host grid, 4*W*H*D, 0 outside the mask. */
+int bpg_fl_plan_ar_estimates(bpg_fl_plan* plan, float* ar);
+/* The whitened volumes the permutations draw from: host, T*W*H*D, 0
+ * outside the mask. */
+int bpg_fl_plan_whitened(bpg_fl_plan* plan, float* volumes);
+
+/* ---- the permutation loop of one statistical map (LIB:16486-16540): for
+ *      each of the K permutations, inverse whitening of the permuted
+ *      whitened volumes (GeneratePermutedVolumesFirstLevel,
+ *      kernelWhitening.cpp:480-535), the first-level statistic
+ *      (kernelStatistics3.cpp:2107-2190 / kernelStatistics5.cpp:1943-2060)
+ *      and the max statistic (CalculateMaxAtomic) or largest cluster
+ *      (ClusterizeOpenCLPermutation).  map: contrast (t) or 0 (F).
+ *      maxdist: host, K.  stats (optional): host, K*V_mask compact
+ *      statistic values. */
+int bpg_fl_plan_permutation_phase(bpg_fl_plan* plan, int map, const unsigned short* perm, long K,
+                                  float* maxdist, float* stats);
+/* setup_ms = transform_ms (regression + whitening), contract_ms = the
+ * permutation kernel, cluster_ms, phase_ms (host wall time) */
+int bpg_fl_plan_last_timing(const bpg_fl_plan* plan, bpg_timing* out);
 
 #ifdef __cplusplus
 }

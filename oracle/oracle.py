@@ -272,3 +272,120 @@ def transformed32(st: dict, c: int = 0) -> np.ndarray:
         lib().orc_transform32(st["W"], st["H"], st["D"], N, P(Rf), P(np.ascontiguousarray(st["mask"], np.float32)),
                               P(Y))
     return Y
+
+
+# ------------------------------------------------------------- first level
+# Volumes are numpy arrays of shape (T, D, H, W) (index x + y*W + z*W*H +
+# t*W*H*D), masks (D, H, W), designs (T, p), contrasts (nc, p), AR fields
+# (4, D, H, W), permutations (K, T).
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def _xcol(X):
+    return np.asfortranarray(X, np.float64).ravel(order="F").copy()
+
+
+def fl_filters(fwhm, vx, vy, vz, size=9):
+    f = [np.zeros(size, np.float32) for _ in range(3)]
+    lib().orc_fl_filters(size, c_float(fwhm), c_float(vx), c_float(vy), c_float(vz), P(f[0]), P(f[1]), P(f[2]))
+    return f
+
+
+def fl_smooth(vol, filters, certainty=None, scert=None):
+    D, H, W = vol.shape
+    out = np.zeros(vol.shape)
+    fx, fy, fz = filters
+    cert = None if certainty is None else np.ascontiguousarray(certainty, np.float32)
+    sc = None if scert is None else _f64(scert)
+    lib().orc_fl_smooth(W, H, D, P(_f64(vol)), P(cert), P(sc), P(fx), P(fy), P(fz), P(out))
+    return out
+
+
+def fl_regress(Y, mask, X):
+    T, D, H, W = Y.shape
+    out = np.zeros(Y.shape)
+    lib().orc_fl_regress(W, H, D, T, X.shape[1], P(_f64(Y)), P(np.ascontiguousarray(mask, np.float32)),
+                         P(_xcol(X)), P(out))
+    return out
+
+
+def fl_ar4(data, mask, invalid=0):
+    T, D, H, W = data.shape
+    out = np.zeros((4, D, H, W))
+    lib().orc_fl_ar4(W, H, D, T, P(_f64(data)), P(np.ascontiguousarray(mask, np.float32)), invalid, P(out))
+    return out
+
+
+def fl_whiten(data, ar, mask):
+    T, D, H, W = data.shape
+    out = np.zeros(data.shape)
+    lib().orc_fl_whiten(W, H, D, T, P(_f64(data)), P(_f64(ar)), P(np.ascontiguousarray(mask, np.float32)), P(out))
+    return out
+
+
+def fl_color(white, ar, mask, perm):
+    T, D, H, W = white.shape
+    out = np.zeros(white.shape)
+    lib().orc_fl_color(W, H, D, T, P(_f64(white)), P(_f64(ar)), P(np.ascontiguousarray(mask, np.float32)),
+                       P(np.ascontiguousarray(perm, np.uint16)), P(out))
+    return out
+
+
+def fl_prepare(Y, mask, X, fwhm=8.0, voxel=(3.0, 3.0, 3.0)):
+    """(whitened volumes, AR estimates) before the permutations."""
+    T, D, H, W = Y.shape
+    wh = np.zeros(Y.shape)
+    ar = np.zeros((4, D, H, W))
+    Xf = np.asfortranarray(X, np.float32).ravel(order="F").copy()
+    lib().orc_fl_prepare(W, H, D, T, X.shape[1], P(np.ascontiguousarray(Y, np.float32)),
+                         P(np.ascontiguousarray(mask, np.float32)), P(Xf), c_float(fwhm), c_float(voxel[0]),
+                         c_float(voxel[1]), c_float(voxel[2]), P(wh), P(ar))
+    return wh, ar
+
+
+def fl_stat(test, x, mask, X, C, contrast=0):
+    T, D, H, W = x.shape
+    C = np.atleast_2d(C)
+    out = np.zeros((D, H, W))
+    lib().orc_fl_stat(test, W, H, D, T, X.shape[1], C.shape[0], P(_f64(x)), P(np.ascontiguousarray(mask, np.float32)),
+                      P(_xcol(X)), P(_f64(C).ravel()), contrast, P(out))
+    return out
+
+
+def gen_perm_first_level(T, K, seed=1):
+    out = np.zeros((K, T), np.uint16)
+    lib().orc_srand(seed)
+    rc = lib().orc_gen_perm_first_level(T, c_long(K), P(out))
+    assert rc == 0
+    return out
+
+
+def fl_null(test, inference, white, ar, mask, X, C, perm, contrast=0, cdt=0.0, connectivity=26,
+            want_stats=False):
+    """Null distribution of one map (K values) and, when asked, the
+    statistic maps (K, D, H, W)."""
+    T, D, H, W = white.shape
+    C = np.atleast_2d(C)
+    K = perm.shape[0]
+    md = np.zeros(K, np.float32)
+    stats = np.zeros((K, D, H, W)) if want_stats else None
+    Xf = np.asfortranarray(X, np.float32).ravel(order="F").copy()
+    Cf = np.ascontiguousarray(C, np.float32).ravel()
+    rc = lib().orc_fl_null(test, inference, W, H, D, T, X.shape[1], C.shape[0], P(_f64(white)), P(_f64(ar)),
+                           P(np.ascontiguousarray(mask, np.float32)), P(Xf), P(Cf), contrast, c_long(K),
+                           P(np.ascontiguousarray(perm, np.uint16)), c_float(cdt), connectivity, P(md), P(stats))
+    assert rc == 0
+    return (md, stats) if want_stats else md
+
+
+def ref_gen_perm_first_level(T, K, seed=1):
+    """The reference's own generator (oracle/_ref), or None when not built."""
+    r = ref_lib()
+    if r is None:
+        return None
+    out = np.zeros((K, T), np.uint16)
+    lib().orc_srand(seed)  # one process-global rand() stream
+    r.ref_gen_perm_first_level(T, c_long(K), P(out))
+    return out

@@ -8,6 +8,7 @@
 //   - GenerateSignMatrixSecondLevel            (LIB:17069-17122)
 //   - GeneratePermutationMatrixSecondLevelTwoSample (LIB:16925-17011)
 //   - GeneratePermutationMatrixSecondLevelCorrelation (LIB:17014-17066)
+//   - GeneratePermutationMatrixFirstLevel      (LIB:16877-16922)
 //   - Clusterize, the host 26-connected BFS     (LIB:18163-18297)
 //   - pinv, the free function                  (LIB:16561-16586)
 // These members are private in broccoli_lib.h (HDR:469-547); the harness
@@ -72,6 +73,16 @@ int ref_gen_perm_correlation(int N, long K, unsigned short* perm) {
     b->SetNumberOfGroupPermutations(kc);
     b->SetPermutationMatrices(mats);
     b->GeneratePermutationMatrixSecondLevelCorrelation(0);
+    return 0;
+}
+
+// Time-point permutations of the first-level test: K rows of T indices.
+int ref_gen_perm_first_level(int T, long K, unsigned short* perm) {
+    BROCCOLI_LIB* b = new BROCCOLI_LIB();
+    b->SetEPITimepoints((size_t)T);
+    b->SetNumberOfPermutations((size_t)K);
+    b->SetPermutationMatrix(perm);
+    b->GeneratePermutationMatrixFirstLevel();
     return 0;
 }
 

@@ -10,7 +10,8 @@ unmodified, see oracle/Makefile) can be built.  The fixture holds:
   permutation matrix (1-based there, stored 0-based here);
 - outputs of the reference's own routines on given inputs:
   GenerateSignMatrixSecondLevel, GeneratePermutationMatrixSecondLevel
-  {TwoSample,Correlation} (after srand(1)), the host Clusterize (26-connected
+  {TwoSample,Correlation} and GeneratePermutationMatrixFirstLevel (after
+  srand(1)), the host Clusterize (26-connected
   BFS, labels 1..n), and pinv.
 
 Nothing here runs on the GPU box; the tests only read the .npz.
@@ -85,6 +86,12 @@ def main():
         oracle.lib().orc_srand(1)
         ref.ref_gen_perm_correlation(N, c_long(K), P(a))
         out[f"correlation_N{N}_K{K}"] = a.reshape(K, N)
+    # first-level time-point permutations (LIB:16877-16922)
+    for T, K in [(12, 100), (40, 300), (160, 200)]:
+        a = np.zeros(T * K, np.uint16)
+        oracle.lib().orc_srand(1)
+        ref.ref_gen_perm_first_level(T, c_long(K), P(a))
+        out[f"firstlevel_T{T}_K{K}"] = a.reshape(K, T)
     # host Clusterize (LIB:18163-18297) on smoothed noise volumes
     rng = np.random.default_rng(20260009)
     from scipy.ndimage import gaussian_filter

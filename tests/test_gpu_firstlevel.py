@@ -1,0 +1,183 @@
+"""Parity of the first-level permutation test on the GPU (through the C ABI)
+with the FP64 oracle (oracle/fl_oracle.c).
+
+Reference: LIB:16454-16558 (ApplyPermutationTestFirstLevel), with the
+whitening of LIB:16363-16452 and the kernels of kernelWhitening.cpp and
+kernelStatistics3/5.cpp.  Accuracy gate as for the second level:
+|stat - stat_fp64| <= 1e-4 (1 + |stat|) for every permuted statistic of every
+in-mask voxel; cluster sizes and masses bit-identical on the GPU's own
+statistic values; the set-up (AR estimates, whitened volumes) to FP64
+round-off.
+"""
+import numpy as np
+import pytest
+
+from broccoli_amd import capi, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def fl_study(seed=0, **kw):
+    st = synth.first_level_study(seed=seed, **kw)
+    return st
+
+
+def in_mask(st):
+    return st["mask"] == 1.0
+
+
+def oracle_null(st, perm, test=0, inference=0, contrast=0, want_stats=True):
+    wh, ar = oracle.fl_prepare(st["Y"], st["mask"], st["X"], st.get("fwhm", 8.0), st["voxel"])
+    return oracle.fl_null(test, inference, wh, ar, st["mask"], st["X"], st["C"], perm, contrast=contrast,
+                          cdt=st.get("cdt", 0.0), connectivity=st.get("connectivity", 26), want_stats=want_stats)
+
+
+def check_stats(stats_gpu, stats_ref, m):
+    ref = stats_ref[:, m]
+    err = np.abs(stats_gpu.astype(np.float64) - ref)
+    assert np.all(err <= TOL * (1.0 + np.abs(ref))), f"max rel err {np.max(err / (1 + np.abs(ref)))}"
+    return float(np.max(err / (1 + np.abs(ref))))
+
+
+def check_max(md, stats_gpu):
+    """The null distribution is the quantised max of the GPU's statistics."""
+    want = np.trunc(stats_gpu.max(axis=1).astype(np.float32) * np.float32(1e4)).astype(np.float32) / np.float32(1e4)
+    np.testing.assert_array_equal(md, want)
+
+
+def test_setup_matches_the_oracle(ctx):
+    st = fl_study(seed=1)
+    print("first-level set-up, grid", st["Y"].shape, flush=True)
+    wh, ar = oracle.fl_prepare(st["Y"], st["mask"], st["X"], 8.0, st["voxel"])
+    m = in_mask(st)
+    with capi.FlPlan(ctx, st) as plan:
+        ar_g = plan.ar_estimates()
+        wh_g = plan.whitened()
+        assert plan.num_voxels == int(m.sum())
+    np.testing.assert_allclose(ar_g[:, m], ar[:, m], rtol=1e-6, atol=1e-7)
+    assert np.all(ar_g[:, ~m] == 0)
+    np.testing.assert_allclose(wh_g[:, m], wh[:, m], rtol=1e-6, atol=1e-5)
+
+
+@pytest.mark.parametrize("seed,T,p", [(2, 64, 5), (3, 120, 8), (4, 48, 1)])
+def test_permuted_t_statistics_match_fp64(ctx, seed, T, p):
+    st = fl_study(seed=seed, T=T, p=p, ar_gradient=0.2)
+    if p == 1:
+        st["C"] = np.ones((1, 1), np.float32)
+    K = 70
+    perm = capi.gen_perm_first_level(T, K, seed=seed)
+    print(f"first-level t seed {seed} T {T} p {p} K {K}", flush=True)
+    md_ref, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+        tm = plan.timing()
+    assert tm.precision_used == capi.PREC_FP64
+    check_stats(stats, stats_ref, in_mask(st))
+    check_max(md, stats)
+    np.testing.assert_allclose(md, md_ref, atol=2e-4, rtol=1e-5)
+
+
+def test_f_statistics_and_second_contrast(ctx):
+    st = fl_study(seed=5, T=80, p=6)
+    C = np.zeros((2, 6), np.float32)
+    C[0, 2] = 1.0
+    C[1, 3] = 1.0
+    C[1, 4] = -1.0
+    st["C"] = C
+    K = 40
+    perm = capi.gen_perm_first_level(80, K, seed=9)
+    m = in_mask(st)
+    print("first-level F and t contrast 1", flush=True)
+    # t of contrast 1
+    _, stats_ref = oracle_null(st, perm, test=0, contrast=1)
+    with capi.FlPlan(ctx, st) as plan:
+        _, stats = plan.permutation_phase(1, perm, want_stats=True)
+    check_stats(stats, stats_ref, m)
+    # F over both rows
+    st["test"] = capi.FTEST
+    _, stats_ref = oracle_null(st, perm, test=1)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+    check_stats(stats, stats_ref, m)
+    check_max(md, stats)
+
+
+def test_identity_permutation_refits_residuals(ctx):
+    st = fl_study(seed=6, T=60)
+    ident = np.arange(60, dtype=np.uint16)[None, :]
+    print("first-level identity permutation", flush=True)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, ident, want_stats=True)
+    # inverse whitening rebuilds the residuals, which are orthogonal to X
+    assert np.max(np.abs(stats)) < 1e-3
+
+
+def test_long_series_read_from_global_memory(ctx):
+    # T above the LDS tile limit: the whitened volumes are read from HBM
+    st = fl_study(seed=7, W=7, H=6, D=5, T=640, p=4)
+    K = 24
+    perm = capi.gen_perm_first_level(640, K, seed=2)
+    print("first-level T=640", flush=True)
+    _, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+    check_stats(stats, stats_ref, in_mask(st))
+    check_max(md, stats)
+
+
+@pytest.mark.parametrize("p", [14, 25])
+def test_wide_designs(ctx, p):
+    st = fl_study(seed=8, T=90, p=p)
+    K = 20
+    perm = capi.gen_perm_first_level(90, K, seed=4)
+    print(f"first-level p={p}", flush=True)
+    _, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+    check_stats(stats, stats_ref, in_mask(st))
+    check_max(md, stats)
+
+
+@pytest.mark.parametrize("inference,conn", [(capi.CLUSTER_EXTENT, 26), (capi.CLUSTER_MASS, 26),
+                                            (capi.CLUSTER_EXTENT, 6)])
+def test_cluster_inference_on_the_gpu_statistics(ctx, inference, conn):
+    st = fl_study(seed=10, W=16, H=16, D=12, T=60)
+    st["inference"] = inference
+    st["connectivity"] = conn
+    st["cdt"] = 1.5
+    K = 30
+    perm = capi.gen_perm_first_level(60, K, seed=5)
+    m = in_mask(st)
+    print(f"first-level cluster inference {inference} conn {conn}", flush=True)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+    G = st["mask"].size
+    for k in range(K):
+        g = np.zeros(G, np.float32)
+        g[np.flatnonzero(m.ravel())] = stats[k]
+        g = g.reshape(st["mask"].shape)
+        lab, _, best = oracle.ccl(g, st["mask"], st["cdt"], conn)
+        if inference == capi.CLUSTER_EXTENT:
+            assert md[k] == np.float32(best), k
+        else:
+            _, bm = oracle.cluster_mass(g, st["mask"], st["cdt"], lab)
+            assert md[k] == np.float32(bm) / np.float32(1e4), k
+
+
+def test_invalid_inputs_are_refused(ctx):
+    st = fl_study(seed=11, T=40)
+    st2 = dict(st, inference=capi.TFCE)
+    with pytest.raises(capi.BpgError) as e:
+        capi.FlPlan(ctx, st2)
+    assert e.value.code == -5
+    with capi.FlPlan(ctx, st) as plan:
+        bad = np.tile(np.arange(40, dtype=np.uint16), (2, 1))
+        bad[1, 3] = 40
+        with pytest.raises(capi.BpgError):
+            plan.permutation_phase(0, bad)
+        # the plan still works afterwards
+        md, _ = plan.permutation_phase(0, capi.gen_perm_first_level(40, 4, seed=1))
+        assert np.all(np.isfinite(md))

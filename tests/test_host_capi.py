@@ -66,6 +66,24 @@ def test_compat_correlation_generator_is_the_reference_stream(key):
     assert np.array_equal(capi.gen_perm_correlation(N, K, compat=True, seed=1), GOLD[key])
 
 
+@pytest.mark.parametrize("key", [k for k in GOLD.files if k.startswith("firstlevel_")])
+def test_compat_first_level_generator_is_the_reference_stream(key):
+    T, K = [int(t[1:]) for t in key.split("_")[1:]]
+    assert np.array_equal(capi.gen_perm_first_level(T, K, compat=True, seed=1), GOLD[key])
+    assert np.array_equal(oracle.gen_perm_first_level(T, K, seed=1), GOLD[key])
+
+
+def test_first_level_generator_invariants():
+    a = capi.gen_perm_first_level(50, 400, compat=False, seed=3)
+    assert np.array_equal(a, capi.gen_perm_first_level(50, 400, compat=False, seed=3))
+    ident = np.arange(50)
+    assert all(np.array_equal(np.sort(r), ident) for r in a)
+    assert len({r.tobytes() for r in a}) == 400
+    assert not any(np.array_equal(r, ident) for r in a)
+    with pytest.raises(capi.BpgError):
+        capi.gen_perm_first_level(3, 6, compat=False, seed=1)  # 3! - 1 = 5 non-identity rows
+
+
 @pytest.mark.parametrize("seed", [1, 2, 12345, 2**31 - 1, 4000000000])
 def test_private_rand_stream_is_glibc_rand(seed):
     """The compat stream restates glibc's rand(): compare long draws with the
