@@ -113,7 +113,8 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector
 # rocprofv3 summaries of the dominant kernel per workload: HBM traffic per
 # launch from the FETCH_SIZE / WRITE_SIZE passes, scaled by a copy of known
 # size (the counters cannot be read from inside this process)
-PROFILES = {2: ROOT / "profiles" / "k_contract_bf16_two_sample_glm_10k.md"}
+PROFILES = {2: ROOT / "profiles" / "k_contract_bf16_two_sample_glm_10k.md",
+            6: ROOT / "profiles" / "k_fl_perm_first_level_10k.md"}
 
 
 def profiled_traffic(workload):
@@ -150,8 +151,13 @@ def run_first_level(args):
     tm = plan.timing()
     kern_ms = float(np.mean(kern))
     # per permutation, voxel and time point: the AR(4) recursion (4 FMA),
-    # the projections on the p design columns (p FMA), |x|^2 (1 FMA), sum x
-    flops = float(T) * (2 * 4 + 2 * p + 2 + 1) * K * V
+    # the projections on the p design columns (p FMA), |x|^2 (1 FMA), and
+    # sum x when the constant is outside col(X)
+    X = st["X"].astype(np.float64)
+    ones = np.ones(T)
+    const_in_span = np.linalg.norm(ones - X @ np.linalg.lstsq(X, ones, rcond=None)[0]) < 1e-6 * np.sqrt(T)
+    flops = float(T) * (2 * 4 + 2 * p + 2 + (0 if const_in_span else 1)) * K * V
+    traffic, traffic_src = profiled_traffic(6)
     achieved = flops / (kern_ms * 1e-3) / 1e12
     res = {
         "metric": "perm·voxels/s (first-level max-T)", "value": K * V / dt, "unit": "perm·voxels/s", "n_gpus": 1,
@@ -162,13 +168,16 @@ def run_first_level(args):
         "setup_ms": plan.timing().transform_ms,
         "phase_breakdown_ms": {"permutation_kernel": kern_ms, "library_call": tm.phase_ms},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "k_fl_perm",
-                     "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": 4.0 * T * V + 2.0 * K * T + 4.0 * K,
+                     "kernel": "k_fl_perm", "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms},
     }
     plan.close()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_first_level(st, perm, args.cpu_seconds)
     ctx.close()
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
     print(json.dumps(res))
 
 

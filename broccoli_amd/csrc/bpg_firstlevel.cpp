@@ -349,6 +349,7 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
     a.bits = cluster ? pl->bits.p : nullptr;
     a.cdt = d.cdt;
     a.T = T; a.Tp = Tp; a.pq = m.pq; a.nrows = m.nrows[map]; a.ftest = m.ftest ? 1 : 0;
+    a.mean = m.const_in_span ? 0 : 1;
     a.K = (int)K; a.Vc = Vc; a.Vpad = pl->Vpad; a.nwords = pl->nwords;
     a.n_vtiles = n_vtiles; a.n_vchunks = n_vchunks; a.tiles_per_chunk = tpc; a.n_pgroups = n_pgroups;
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));

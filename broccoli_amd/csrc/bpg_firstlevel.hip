@@ -267,7 +267,7 @@ __device__ __forceinline__ float wave_max(float x) {
     return x;
 }
 
-template <int PQ, bool USE_LDS>
+template <int PQ, bool USE_LDS, bool MEAN>
 __global__ __launch_bounds__(512) void k_fl_perm(PermArgs a) {
     extern __shared__ float tile[];
     __shared__ float smax[WAVES * ROUNDS * NP];
@@ -319,7 +319,33 @@ __global__ __launch_bounds__(512) void k_fl_perm(PermArgs a) {
                 s[j] = 0.0;
                 m[j] = 0.0;
             }
-            for (int t8 = 0; t8 < a.Tp; t8 += 8) {
+            // one time step of every permutation of the thread
+            auto step = [&](int t, const unsigned (&idx)[NP]) {
+                double qv[PQ];
+#pragma unroll
+                for (int q = 0; q < PQ; q++) qv[q] = Q[(size_t)t * PQ + q];
+#pragma unroll
+                for (int j = 0; j < NP; j++) {
+                    const float w = USE_LDS ? tile[idx[j] * TILE + lane] : a.w32[(size_t)idx[j] * a.Vpad + v];
+                    double x = (double)w;
+                    x = fma(a4, h[j][3], x);
+                    x = fma(a3, h[j][2], x);
+                    x = fma(a2, h[j][1], x);
+                    x = fma(a1, h[j][0], x);
+                    h[j][3] = h[j][2];
+                    h[j][2] = h[j][1];
+                    h[j][1] = h[j][0];
+                    h[j][0] = x;
+                    s[j] = fma(x, x, s[j]);
+                    if (MEAN) m[j] += x;
+#pragma unroll
+                    for (int q = 0; q < PQ; q++) z[j][q] = fma(qv[q], x, z[j][q]);
+                }
+            };
+            // whole groups of 8 time points: no exits inside, so the AR
+            // history rotates by register renaming (8 is a multiple of 4)
+            const int Tfull = a.T & ~7;
+            for (int t8 = 0; t8 < Tfull; t8 += 8) {
                 unsigned pk[NP][4];  // 8 indices of each permutation
 #pragma unroll
                 for (int j = 0; j < NP; j++)
@@ -327,30 +353,17 @@ __global__ __launch_bounds__(512) void k_fl_perm(PermArgs a) {
                     for (int q = 0; q < 4; q++) pk[j][q] = prow[j][(t8 >> 1) + q];
 #pragma unroll
                 for (int tt = 0; tt < 8; tt++) {
-                    const int t = t8 + tt;
-                    if (t >= a.T) break;
-                    double qv[PQ];
+                    unsigned idx[NP];
 #pragma unroll
-                    for (int q = 0; q < PQ; q++) qv[q] = Q[(size_t)t * PQ + q];
-#pragma unroll
-                    for (int j = 0; j < NP; j++) {
-                        const unsigned idx = (pk[j][tt >> 1] >> ((tt & 1) * 16)) & 0xFFFFu;
-                        const float w = USE_LDS ? tile[idx * TILE + lane] : a.w32[(size_t)idx * a.Vpad + v];
-                        double x = (double)w;
-                        x = fma(a4, h[j][3], x);
-                        x = fma(a3, h[j][2], x);
-                        x = fma(a2, h[j][1], x);
-                        x = fma(a1, h[j][0], x);
-                        h[j][3] = h[j][2];
-                        h[j][2] = h[j][1];
-                        h[j][1] = h[j][0];
-                        h[j][0] = x;
-                        s[j] = fma(x, x, s[j]);
-                        m[j] += x;
-#pragma unroll
-                        for (int q = 0; q < PQ; q++) z[j][q] = fma(qv[q], x, z[j][q]);
-                    }
+                    for (int j = 0; j < NP; j++) idx[j] = (pk[j][tt >> 1] >> ((tt & 1) * 16)) & 0xFFFFu;
+                    step(t8 + tt, idx);
                 }
+            }
+            for (int t = Tfull; t < a.T; t++) {
+                unsigned idx[NP];
+#pragma unroll
+                for (int j = 0; j < NP; j++) idx[j] = (prow[j][t >> 1] >> ((t & 1) * 16)) & 0xFFFFu;
+                step(t, idx);
             }
 #pragma unroll
             for (int j = 0; j < NP; j++) {
@@ -363,7 +376,8 @@ __global__ __launch_bounds__(512) void k_fl_perm(PermArgs a) {
                     uz = fma(U[q], z[j][q], uz);
                 }
                 const double sse = s[j] - ss;
-                const double me = m[j] - uz;
+                // without the constant in col(X) the residuals need not sum to 0
+                const double me = MEAN ? m[j] - uz : 0.0;
                 const double var = (sse - me * me * invT) * invTm1;
                 double stat;
                 if (!a.ftest) {
@@ -412,25 +426,30 @@ int fl_width(int p) {
     return -1;
 }
 
-template <int PQ>
-hipError_t launch_perm_w(const PermArgs& a, hipStream_t s) {
+template <int PQ, bool MEAN>
+hipError_t launch_perm_m(const PermArgs& a, hipStream_t s) {
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
     if (a.T <= LDS_T_MAX) {
         const size_t lds = (size_t)a.T * TILE * sizeof(float);
         static bool attr = false;
         if (!attr) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fl_perm<PQ, true>),
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fl_perm<PQ, true, MEAN>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)(LDS_T_MAX * TILE * sizeof(float)));
             if (e != hipSuccess) return e;
             attr = true;
         }
-        hipLaunchKernelGGL((k_fl_perm<PQ, true>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+        hipLaunchKernelGGL((k_fl_perm<PQ, true, MEAN>), dim3(grid), dim3(WAVES * 64), lds, s, a);
     } else {
-        hipLaunchKernelGGL((k_fl_perm<PQ, false>), dim3(grid), dim3(WAVES * 64), 0, s, a);
+        hipLaunchKernelGGL((k_fl_perm<PQ, false, MEAN>), dim3(grid), dim3(WAVES * 64), 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <int PQ>
+hipError_t launch_perm_w(const PermArgs& a, hipStream_t s) {
+    return a.mean ? launch_perm_m<PQ, true>(a, s) : launch_perm_m<PQ, false>(a, s);
 }
 
 hipError_t launch_perm(PermArgs a, hipStream_t s) {

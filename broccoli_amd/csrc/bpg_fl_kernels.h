@@ -50,6 +50,7 @@ hipError_t launch_whiten(const double* e, const double* ar, int T, int Vc, int V
 //   x_t = sum_{i<=min(t,4)} a_i x_{t-i} + w[perm_k[t]][v]      (inverse whitening)
 //   z = Q' x, s = |x|^2, m = sum x                              (FP64)
 //   SSE = s - |z|^2, var = (SSE - (m - u'z)^2 / T) / (T - 1)
+//   (the mean term only when the constant is outside col(X): mean = 1)
 //   t = (A_0 z) / sqrt(var),  F = sum_i (A_i z)^2 / (var * nrows)
 // Q is an orthonormal basis of col(X) (T x pq, zero-padded columns), u =
 // Q'1, A the statistic rows.
@@ -64,7 +65,7 @@ struct PermArgs {
     float* stats;                  // [K][Vc] or null
     unsigned* bits;                // [K][nwords] stat > cdt, or null
     float cdt;
-    int T, Tp, pq, nrows, ftest, K, Vc, Vpad, nwords;
+    int T, Tp, pq, nrows, ftest, mean, K, Vc, Vpad, nwords;
     int n_vtiles, n_vchunks, tiles_per_chunk, n_pgroups;
 };
 // pq must be one of the compiled widths (fl_width)

@@ -264,6 +264,12 @@ int build_fl_model(int T, int p, int nc, int ftest, const float* X, const float*
             m.Q[(size_t)t * pq + j] = Q1[t + (size_t)j * T];
             m.u[j] += Q1[t + (size_t)j * T];
         }
+    {
+        // |P_X 1|^2 = |Q1' 1|^2 equals T exactly when 1 is in col(X)
+        double uu = 0.0;
+        for (int j = 0; j < p; j++) uu += m.u[j] * m.u[j];
+        m.const_in_span = (double)T - uu <= 1e-9 * (double)T;
+    }
     // pinv(X) = R1^-1 Q1'
     std::vector<double> Ri((size_t)p * p, 0.0);  // column-major R1^-1
     for (int c = 0; c < p; c++) {

@@ -50,6 +50,7 @@ struct ObsModel {
 struct FlModel {
     int T = 0, p = 0, pq = 0, nmaps = 1;
     bool ftest = false;
+    bool const_in_span = false;          // 1 in col(X): the residuals sum to 0
     std::vector<double> Q;               // [t*pq + j], zero beyond p
     std::vector<double> u;               // [pq]
     std::vector<std::vector<double>> A;  // per map: [i*pq + j]

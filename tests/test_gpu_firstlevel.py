@@ -80,6 +80,21 @@ def test_permuted_t_statistics_match_fp64(ctx, seed, T, p):
     np.testing.assert_allclose(md, md_ref, atol=2e-4, rtol=1e-5)
 
 
+def test_design_without_an_intercept(ctx):
+    # the residuals need not sum to zero: Welford's mean term is kept
+    st = fl_study(seed=12, T=61, p=5)
+    st["X"] = np.ascontiguousarray(st["X"][:, 1:])
+    st["C"] = np.ascontiguousarray(st["C"][:, 1:])
+    K = 30
+    perm = capi.gen_perm_first_level(61, K, seed=6)
+    print("first-level design without intercept, T=61", flush=True)
+    _, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+    check_stats(stats, stats_ref, in_mask(st))
+    check_max(md, stats)
+
+
 def test_f_statistics_and_second_contrast(ctx):
     st = fl_study(seed=5, T=80, p=6)
     C = np.zeros((2, 6), np.float32)
