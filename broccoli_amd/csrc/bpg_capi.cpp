@@ -260,6 +260,35 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const bpg::MapModel& m = pl->models[map];
     if (m.sign_operand ? !sign : !perm) BPG_FAIL(ctx, BPG_ERR_ARG, "missing permutation / sign matrix");
     const int N = d.N;
+    if (pl->Vc == 0) {
+        // empty mask: nothing to launch; the null is the reference's empty
+        // max (phase_finish) or an empty largest cluster
+        if (!m.sign_operand)
+            for (size_t i = 0; i < (size_t)K * N; i++)
+                if (perm[i] >= N) BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
+        BPG_HIP(ctx, pl->h_max.ensure(K));
+        BPG_HIP(ctx, pl->h_unsafe.ensure(1));
+        const bool clu = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
+        for (long k = 0; k < K; k++) {
+            if (clu) reinterpret_cast<unsigned*>(pl->h_max.p)[k] = 0u;
+            else pl->h_max.p[k] = -INFINITY;
+        }
+        *pl->h_unsafe.p = 0;
+        auto& ps = pl->pending;
+        ps = bpg_plan::Pending{};
+        ps.active = true;
+        ps.map = map;
+        ps.K = K;
+        ps.cluster = clu;
+        ps.mass = d.inference == BPG_CLUSTER_MASS;
+        ps.tfce = d.inference == BPG_TFCE;
+        ps.have_times = true;
+        ps.pfit = m.pfit;
+        ps.perm = perm;
+        ps.sign = sign;
+        ps.t0 = std::chrono::steady_clock::now();
+        return BPG_OK;
+    }
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = pl->stream;
@@ -672,9 +701,18 @@ extern "C" int bpg_plan_glm(bpg_plan* pl, float* betas, float* maps, float* resi
     if (!pl) return BPG_ERR_ARG;
     bpg_ctx* ctx = pl->ctx;
     const bpg::ObsModel& o = pl->obs;
+    const int Vc = pl->Vc, N = pl->d.N;
+    if (Vc == 0) {
+        // empty mask: every output is 0 (out-of-mask voxels are 0)
+        if (maps) std::fill(maps, maps + (size_t)pl->nmaps * pl->G, 0.0f);
+        if (betas) std::fill(betas, betas + (size_t)o.p * pl->G, 0.0f);
+        if (residuals) std::fill(residuals, residuals + (size_t)N * pl->G, 0.0f);
+        if (resvar) std::fill(resvar, resvar + pl->G, 0.0f);
+        pl->have_obs = true;
+        return BPG_OK;
+    }
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = pl->stream;
-    const int Vc = pl->Vc, N = pl->d.N;
     BPG_HIP(ctx, pl->oX.ensure(o.X.size()));
     BPG_HIP(ctx, pl->oXp.ensure(o.Xp.size()));
     BPG_HIP(ctx, pl->oC.ensure(o.C.size()));

@@ -220,7 +220,7 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
         float* dst = yc.data() + (size_t)t * pl->Vpad;
         for (int v = 0; v < pl->Vc; v++) dst[v] = src[pl->comp2grid[v]];
     }
-    {
+    if (pl->Vc > 0) {
         const int rc = run_setup(pl, yc.data());
         if (rc != BPG_OK) return fail(rc);
     }
@@ -266,6 +266,10 @@ extern "C" long bpg_fl_plan_num_voxels(const bpg_fl_plan* pl) { return pl ? pl->
 extern "C" int bpg_fl_plan_ar_estimates(bpg_fl_plan* pl, float* ar) {
     if (!pl || !ar) return BPG_ERR_ARG;
     bpg_ctx* ctx = pl->ctx;
+    if (pl->Vc == 0) {
+        std::fill(ar, ar + 4 * pl->G, 0.0f);
+        return BPG_OK;
+    }
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<double> h(4 * (size_t)pl->Vpad);
     BPG_HIP(ctx, hipMemcpy(h.data(), pl->ar.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -278,8 +282,12 @@ extern "C" int bpg_fl_plan_ar_estimates(bpg_fl_plan* pl, float* ar) {
 extern "C" int bpg_fl_plan_whitened(bpg_fl_plan* pl, float* volumes) {
     if (!pl || !volumes) return BPG_ERR_ARG;
     bpg_ctx* ctx = pl->ctx;
-    BPG_HIP(ctx, hipSetDevice(ctx->device));
     const int T = pl->d.T;
+    if (pl->Vc == 0) {
+        std::fill(volumes, volumes + (size_t)T * pl->G, 0.0f);
+        return BPG_OK;
+    }
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<float> h((size_t)T * pl->Vpad);
     BPG_HIP(ctx, hipMemcpy(h.data(), pl->w32.p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
     std::fill(volumes, volumes + (size_t)T * pl->G, 0.0f);
@@ -319,6 +327,14 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
         for (int t = T; t < Tp; t++) dst[t] = 0;
     }
     if (worst >= T) BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
+    if (Vc == 0) {
+        // empty mask: nothing to launch (the reference's empty max, or an
+        // empty largest cluster)
+        for (long k = 0; k < K; k++) maxdist[k] = cluster ? 0.0f : bpg::empty_max();
+        pl->timing = bpg_timing{};
+        pl->timing.precision_used = BPG_PREC_FP64;
+        return BPG_OK;
+    }
 
     const int n_vtiles = pl->Vpad / bpg::fl::TILE;
     const int tpc = (n_vtiles + bpg::fl::MAX_VCHUNKS - 1) / bpg::fl::MAX_VCHUNKS;

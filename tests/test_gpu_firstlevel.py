@@ -196,3 +196,30 @@ def test_invalid_inputs_are_refused(ctx):
         # the plan still works afterwards
         md, _ = plan.permutation_phase(0, capi.gen_perm_first_level(40, 4, seed=1))
         assert np.all(np.isfinite(md))
+
+
+@pytest.mark.parametrize("inference", [capi.VOXEL, capi.CLUSTER_MASS])
+def test_empty_and_single_voxel_masks(ctx, inference):
+    st = fl_study(seed=13, T=40)
+    st["inference"] = inference
+    st["cdt"] = 1.0
+    perm = capi.gen_perm_first_level(40, 10, seed=8)
+    print(f"first-level empty / single-voxel mask, inference {inference}", flush=True)
+    empty = dict(st, mask=np.zeros_like(st["mask"]))
+    with capi.FlPlan(ctx, empty) as plan:
+        assert plan.num_voxels == 0
+        md, _ = plan.permutation_phase(0, perm)
+        assert np.all(plan.ar_estimates() == 0)
+    if inference == capi.VOXEL:
+        s = np.frombuffer(np.float32(-1e6).tobytes(), np.int32)[0]
+        assert np.all(md == np.float32(np.float32(s) / np.float32(10000)))
+    else:
+        assert np.all(md == 0)
+    one = np.zeros_like(st["mask"])
+    one[5, 7, 6] = 1.0
+    single = dict(st, mask=one)
+    _, stats_ref = oracle_null(single, perm)
+    with capi.FlPlan(ctx, single) as plan:
+        assert plan.num_voxels == 1
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+    check_stats(stats, stats_ref, one == 1.0)

@@ -407,3 +407,49 @@ def test_cluster_extent_large_components_take_the_global_path(ctx):
             assert md[k] == float(best), (cdt, k)
         if cdt < 0:
             assert np.all(md == st["V"])
+
+
+def _sentinel():
+    # the empty-mask max of LIB:10971: the bit pattern of -1e6f read as int, / 1e4
+    s = np.frombuffer(np.float32(-1e6).tobytes(), np.int32)[0]
+    return np.float32(np.float32(s) / np.float32(10000))
+
+
+@pytest.mark.parametrize("inference", [capi.VOXEL, capi.CLUSTER_EXTENT])
+def test_empty_mask(ctx, inference):
+    st = synth.study(2, scale=0.1)
+    st["mask"] = np.zeros_like(st["mask"])
+    st["inference"] = inference
+    st["cdt"] = 2.0
+    K = 12
+    perms = synth.permutations(st, K, seed=3)
+    print(f"empty mask, inference {inference}", flush=True)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        assert plan.num_voxels == 0
+        md, _ = plan.permutation_phase(0, **perms)
+        maps = plan.observed_maps()
+        P = plan.pvalues([md])
+    want = _sentinel() if inference == capi.VOXEL else np.float32(0)
+    assert np.all(md == want)
+    assert np.all(maps == 0) and np.all(P == 0)
+    # the oracle's whole path agrees
+    _, _, md64 = oracle.randomise(dict(st), perms, K, precision=oracle.F64)
+    assert np.array_equal(md64[0], md)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "split_bf16"])
+def test_single_voxel_mask(ctx, prec):
+    st = synth.study(2, scale=0.1)
+    mask = np.zeros_like(st["mask"])
+    mask[mask.shape[0] // 2, mask.shape[1] // 2, mask.shape[2] // 2] = 1.0
+    st["mask"] = mask
+    st["inference"] = capi.VOXEL
+    K = 40
+    perms = synth.permutations(st, K, seed=4)
+    print(f"single-voxel mask {prec}", flush=True)
+    with capi.Plan(ctx, st, prec) as plan:
+        assert plan.num_voxels == 1
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    assert (np.abs(stats - ref) / (1.0 + np.abs(ref))).max() <= TOL
+    check_maxT(md, ref, stats)
