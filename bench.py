@@ -108,13 +108,14 @@ WORKLOADS = {
        "(8 mm smoothing), 10000 time-point permutations, voxel max-T",
 }
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector
+PEAK_FP32_MATRIX_TFLOPS = 157.3  # MI355X FP32 matrix
 
 
 # rocprofv3 summaries of the dominant kernel per workload: HBM traffic per
 # launch from the FETCH_SIZE / WRITE_SIZE passes, scaled by a copy of known
 # size (the counters cannot be read from inside this process)
 PROFILES = {2: ROOT / "profiles" / "k_contract_bf16_two_sample_glm_10k.md",
-            6: ROOT / "profiles" / "k_fl_perm_first_level_10k.md"}
+            6: ROOT / "profiles" / "k_fl_perm_mx_first_level_10k.md"}
 
 
 def profiled_traffic(workload):
@@ -132,24 +133,30 @@ def run_first_level(args):
     K = args.K
     perm = capi.gen_perm_first_level(T, K, compat=False, seed=20260006)
     ctx = capi.Context(0)
-    t0 = time.perf_counter()
-    plan = capi.FlPlan(ctx, st)
-    setup_wall = time.perf_counter() - t0
+
+    def timed(precision, steps, warmup):
+        t0 = time.perf_counter()
+        plan = capi.FlPlan(ctx, dict(st, precision=precision))
+        setup_wall = time.perf_counter() - t0
+        for _ in range(warmup):
+            plan.permutation_phase(0, perm)
+        hip_sync()
+        t0 = time.perf_counter()
+        kern = []
+        for _ in range(steps):
+            md, _ = plan.permutation_phase(0, perm)
+            kern.append(plan.timing().contract_ms)
+        hip_sync()
+        return plan, md, (time.perf_counter() - t0) / steps, float(np.mean(kern)), setup_wall
+
+    plan64, md64, dt64, kern64, _ = timed("fp64", max(2, args.steps // 2), 1)
+    plan64.close()
+    plan, md, dt, kern_ms, setup_wall = timed("fp32", args.steps, args.warmup)
     V = plan.num_voxels
-    print(f"# first-level study ready: V={V} T={T} p={p} K={K} set-up {plan.timing().transform_ms:.1f} ms "
-          f"(wall {setup_wall*1e3:.0f} ms)", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        plan.permutation_phase(0, perm)
-    hip_sync()
-    t0 = time.perf_counter()
-    kern = []
-    for _ in range(args.steps):
-        md, _ = plan.permutation_phase(0, perm)
-        kern.append(plan.timing().contract_ms)
-    hip_sync()
-    dt = (time.perf_counter() - t0) / args.steps
     tm = plan.timing()
-    kern_ms = float(np.mean(kern))
+    print(f"# first-level study: V={V} T={T} p={p} K={K} set-up {tm.transform_ms:.1f} ms (wall {setup_wall*1e3:.0f} ms); "
+          f"fp32 form {dt*1e3:.2f} ms/step (kernel {kern_ms:.2f}), fp64 form {dt64*1e3:.2f} ms/step "
+          f"(kernel {kern64:.2f})", file=sys.stderr, flush=True)
     # per permutation, voxel and time point: the AR(4) recursion (4 FMA),
     # the projections on the p design columns (p FMA), |x|^2 (1 FMA), and
     # sum x when the constant is outside col(X)
@@ -163,14 +170,22 @@ def run_first_level(args):
         "metric": "perm·voxels/s (first-level max-T)", "value": K * V / dt, "unit": "perm·voxels/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": WORKLOADS[6], "V_mask": V, "T": T, "p": p, "K": K},
-        "setup_ms": plan.timing().transform_ms,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS[6], "V_mask": V, "T": T, "p": p, "K": K, "precision": "fp32"},
+        "setup_ms": tm.transform_ms,
         "phase_breakdown_ms": {"permutation_kernel": kern_ms, "library_call": tm.phase_ms},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
+        "permutations_repeated_in_fp64": tm.unsafe_waves,
+        "roofline": {"bound": "mfma+valu", "achieved": achieved, "peak": PEAK_FP32_MATRIX_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_MATRIX_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": 4.0 * T * V + 2.0 * K * T + 4.0 * K,
-                     "kernel": "k_fl_perm", "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms},
+                     "kernel": "k_fl_perm_mx", "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms,
+                     "note": "the recursion and |x|^2 on the FP32 VALU, the projections on the FP32 matrix cores "
+                             "(peak 157.3 TFLOP/s), over the measured launch time"},
+        "fp64_form": {"value": K * V / dt64, "ms_per_step": dt64 * 1e3, "kernel_ms": kern64,
+                      "achieved_tflops": flops / (kern64 * 1e-3) / 1e12,
+                      "frac_of_fp64_peak": flops / (kern64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                      "maxT_equal_to_fp32_form": int(np.sum(md64 == md))},
+        "speedup_vs_fp64": dt64 / dt,
     }
     plan.close()
     if not args.no_cpu:
@@ -299,7 +314,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--K", type=int, default=10000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

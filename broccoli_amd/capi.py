@@ -50,7 +50,8 @@ class StudyDesc(ctypes.Structure):
 
 
 class FlDesc(ctypes.Structure):
-    _fields_ = [("test", c_int), ("inference", c_int), ("connectivity", c_int), ("W", c_int), ("H", c_int),
+    _fields_ = [("test", c_int), ("inference", c_int), ("precision", c_int), ("connectivity", c_int),
+                ("W", c_int), ("H", c_int),
                 ("D", c_int), ("T", c_int), ("p", c_int), ("nc", c_int), ("volumes", c_void_p),
                 ("mask", c_void_p), ("X", c_void_p), ("C", c_void_p), ("voxel_size", c_float * 3),
                 ("ar_smoothing_fwhm", c_float), ("cdt", c_float)]
@@ -350,7 +351,8 @@ class FlPlan:
 
     study: dict with Y (T, D, H, W) float32, mask (D, H, W), X (T, p), C
     (nc, p), voxel (3 sizes in mm); optional test (TTEST), inference
-    (VOXEL), cdt, connectivity (26), fwhm (AR smoothing, 8 mm).
+    (VOXEL), precision ("fp32": FP32 with the matrix cores and FP64 repeats,
+    or "fp64"), cdt, connectivity (26), fwhm (AR smoothing, 8 mm).
     """
 
     def __init__(self, ctx: Context, study: dict):
@@ -366,6 +368,8 @@ class FlPlan:
         d = FlDesc()
         d.test = s.get("test", TTEST)
         d.inference = s.get("inference", VOXEL)
+        prec = s.get("precision", "fp32")
+        d.precision = {"fp32": PREC_FP32, "fp64": PREC_FP64}[prec] if isinstance(prec, str) else prec
         d.connectivity = s.get("connectivity", 26)
         d.W, d.H, d.D, d.T = W, H, D, T
         d.p, d.nc = s["X"].shape[1], C.shape[0]

@@ -35,6 +35,9 @@ struct bpg_fl_plan {
     DBuf<float> w32;           // [T][Vpad] whitened volumes
     DBuf<double> ar;           // [4][Vpad] total AR estimates
     DBuf<double> Q, A, u;      // model
+    DBuf<float> Q32, u32;      // rotated model of the FP32 form ([Tq][16], [16])
+    DBuf<unsigned char> flags; // [K] permutations to repeat in FP64
+    HBuf<unsigned char> h_flags;
     DBuf<int> d_nbr;
     DBuf<unsigned short> perm; // [K][Tp]
     DBuf<float> partial, maxout, stats;
@@ -47,6 +50,7 @@ struct bpg_fl_plan {
         w32.release(); ar.release(); Q.release(); A.release(); u.release(); d_nbr.release(); perm.release();
         partial.release(); maxout.release(); stats.release(); bits.release(); maxsize.release(); cl.release();
         h_perm.release(); h_max.release();
+        Q32.release(); u32.release(); flags.release(); h_flags.release();
     }
 };
 
@@ -70,6 +74,49 @@ void smoothing_filters(float fwhm, const float* vox, float (*f)[9]) {
             sum += f[a][i];
         }
         for (int i = 0; i < 9; i++) f[a][i] /= sum;
+    }
+}
+
+// The FP32 form's basis for statistical map `map`: Q M with M orthogonal
+// (p x p) whose first r columns are the statistic rows A_i (orthonormal), so
+// z' = M'z has z'_i = A_i z for i < r and |z'| = |z|; completed by
+// Gram-Schmidt on the unit vectors.  q32[t*16 + j] (rows to Tq, zeros
+// beyond T and p), u32 = M'u.
+void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<float>& q32, std::vector<float>& u32) {
+    const int p = m.p, r = m.nrows[map];
+    const std::vector<double>& A = m.A[map];
+    std::vector<std::vector<double>> cols;
+    for (int i = 0; i < r; i++) {
+        std::vector<double> v(p);
+        for (int j = 0; j < p; j++) v[j] = A[(size_t)i * m.pq + j];
+        cols.push_back(v);
+    }
+    for (int e = 0; e < p && (int)cols.size() < p; e++) {
+        std::vector<double> v(p, 0.0);
+        v[e] = 1.0;
+        for (int pass = 0; pass < 2; pass++)
+            for (const auto& q : cols) {
+                double d = 0.0;
+                for (int j = 0; j < p; j++) d += q[j] * v[j];
+                for (int j = 0; j < p; j++) v[j] -= d * q[j];
+            }
+        double n2 = 0.0;
+        for (double x : v) n2 += x * x;
+        if (n2 < 1e-6) continue;
+        for (double& x : v) x /= std::sqrt(n2);
+        cols.push_back(v);
+    }
+    q32.assign((size_t)Tq * 16, 0.0f);
+    u32.assign(16, 0.0f);
+    for (int i = 0; i < (int)cols.size(); i++) {
+        double uu = 0.0;
+        for (int j = 0; j < p; j++) uu += m.u[j] * cols[i][j];
+        u32[i] = (float)uu;
+        for (int t = 0; t < m.T; t++) {
+            double acc = 0.0;
+            for (int j = 0; j < p; j++) acc += m.Q[(size_t)t * m.pq + j] * cols[i][j];
+            q32[(size_t)t * 16 + i] = (float)acc;
+        }
     }
 }
 
@@ -161,6 +208,8 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
     *out = nullptr;
     const bpg_fl_desc& d = *desc;
     if (d.test != BPG_TTEST && d.test != BPG_FTEST) BPG_FAIL(ctx, BPG_ERR_ARG, "test must be t or F");
+    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_FP64)
+        BPG_FAIL(ctx, BPG_ERR_ARG, "precision must be BPG_PREC_FP32 or BPG_PREC_FP64");
     if (d.inference == BPG_TFCE)
         BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED,
                  "TFCE is not available for the first-level permutation test (the reference leaves its "
@@ -236,6 +285,13 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
         ctx->err = "upload of the model failed";
         return fail(BPG_ERR_HIP);
     }
+    if (m.pq <= 16) {
+        const int Tq = (d.T + 3) & ~3;
+        if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16)) {
+            ctx->err = "device allocation failed";
+            return fail(BPG_ERR_NOMEM);
+        }
+    }
     if (d.inference != BPG_VOXEL) {
         std::vector<int> nbr;
         bpg::build_neighbours(d.W, d.H, d.D, pl->comp2grid, grid2comp, nbr);
@@ -297,9 +353,12 @@ extern "C" int bpg_fl_plan_whitened(bpg_fl_plan* pl, float* volumes) {
     return BPG_OK;
 }
 
-extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K,
-                                             float* maxdist, float* stats) {
-    if (!pl) return BPG_ERR_ARG;
+namespace {
+
+// One permutation phase in the FP32 form (when allowed and the study fits
+// it) or the FP64 form.  *fp32_used tells which ran.
+int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float* maxdist, float* stats,
+             bool allow_fp32, bool* fp32_used) {
     bpg_ctx* ctx = pl->ctx;
     const bpg_fl_desc& d = pl->d;
     const bpg::FlModel& m = pl->model;
@@ -333,6 +392,7 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
         for (long k = 0; k < K; k++) maxdist[k] = cluster ? 0.0f : bpg::empty_max();
         pl->timing = bpg_timing{};
         pl->timing.precision_used = BPG_PREC_FP64;
+        *fp32_used = false;
         return BPG_OK;
     }
 
@@ -349,8 +409,20 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
         BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
         BPG_HIP(ctx, pl->maxsize.ensure(K));
     }
+    const bool mx = allow_fp32 && d.precision == BPG_PREC_FP32 && m.pq <= 16 && T <= bpg::fl::MX_T_MAX &&
+                    m.nrows[map] <= BPG_FL_MAX_FROWS;
+    *fp32_used = mx;
     const std::vector<double>& Arows = m.A[map];
     BPG_HIP(ctx, hipMemcpyAsync(pl->A.p, Arows.data(), Arows.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    std::vector<float> q32, u32;
+    if (mx) {
+        rotated_basis(m, map, (T + 3) & ~3, q32, u32);
+        BPG_HIP(ctx, hipMemcpyAsync(pl->Q32.p, q32.data(), q32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->u32.p, u32.data(), 16 * sizeof(float), hipMemcpyHostToDevice, st));
+        BPG_HIP(ctx, pl->flags.ensure(K));
+        BPG_HIP(ctx, pl->h_flags.ensure(K));
+        BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
+    }
     BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, pl->h_perm.p, (size_t)K * Tp * sizeof(unsigned short),
                                 hipMemcpyHostToDevice, st));
     bpg::fl::PermArgs a{};
@@ -368,9 +440,15 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
     a.mean = m.const_in_span ? 0 : 1;
     a.K = (int)K; a.Vc = Vc; a.Vpad = pl->Vpad; a.nwords = pl->nwords;
     a.n_vtiles = n_vtiles; a.n_vchunks = n_vchunks; a.tiles_per_chunk = tpc; a.n_pgroups = n_pgroups;
+    if (mx) {
+        a.Q32 = pl->Q32.p;
+        a.u32 = pl->u32.p;
+        a.flags = pl->flags.p;
+    }
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
-    BPG_HIP(ctx, bpg::fl::launch_perm(a, st));
+    BPG_HIP(ctx, mx ? bpg::fl::launch_perm_mx(a, st) : bpg::fl::launch_perm(a, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
+    if (mx) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
     if (cluster) {
         bpg::CclArgs cc{};
         cc.bits = pl->bits.p;
@@ -411,8 +489,45 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
     tm.contract_launches = 1;
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = d.p;
-    tm.slots_per_perm = m.pq;
-    tm.precision_used = BPG_PREC_FP64;
+    tm.slots_per_perm = mx ? 16 : m.pq;
+    tm.precision_used = mx ? BPG_PREC_FP32 : BPG_PREC_FP64;
+    tm.fp32_permutations = mx ? K : 0;
+    return BPG_OK;
+}
+
+}  // namespace
+
+extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K,
+                                             float* maxdist, float* stats) {
+    if (!pl) return BPG_ERR_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool fp32 = false;
+    int rc = fl_phase(pl, map, perm, K, maxdist, stats, true, &fp32);
+    if (rc != BPG_OK || !fp32) return rc;
+    // permutations with an ill-conditioned sample: repeated in FP64, their
+    // results replace the FP32 ones
+    std::vector<long> S;
+    for (long k = 0; k < K; k++)
+        if (pl->h_flags.p[k]) S.push_back(k);
+    if (S.empty()) return BPG_OK;
+    const bpg_timing keep = pl->timing;
+    const int T = pl->d.T, Vc = pl->Vc;
+    const long Ks = (long)S.size();
+    std::vector<unsigned short> sp((size_t)Ks * T);
+    for (long i = 0; i < Ks; i++) std::copy(perm + S[i] * T, perm + (S[i] + 1) * T, sp.begin() + i * T);
+    std::vector<float> smd(Ks), sst(stats ? (size_t)Ks * Vc : 0);
+    bool unused = false;
+    rc = fl_phase(pl, map, sp.data(), Ks, smd.data(), stats ? sst.data() : nullptr, false, &unused);
+    if (rc != BPG_OK) return rc;
+    for (long i = 0; i < Ks; i++) {
+        maxdist[S[i]] = smd[i];
+        if (stats) std::copy(sst.begin() + i * Vc, sst.begin() + (i + 1) * Vc, stats + S[i] * Vc);
+    }
+    pl->timing = keep;
+    pl->timing.unsafe_waves = (int)Ks;
+    pl->timing.fp32_permutations = K - Ks;
+    pl->timing.phase_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return BPG_OK;
 }
 

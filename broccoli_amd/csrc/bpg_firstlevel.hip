@@ -419,6 +419,243 @@ __global__ __launch_bounds__(512) void k_fl_perm(PermArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// the FP32 form: the recursion, |x|^2 and sum x in FP32 on the vector ALUs
+// (the reference's arithmetic), the projections z = Q'x on the FP32 matrix
+// cores.  Per group of 4 time points and permutation, each lane holds x for
+// its voxel at t..t+3 in 4 registers; two permlane32 and two permlane16
+// swaps turn them into the A operands of 4 MFMA 16x16x4 (16 voxels x 4 time
+// points each), B = Q[t..t+3][0..15] (4 rows, one coalesced load), so the
+// 64 voxels x 16 columns of z accumulate in 4 x 4 registers.  The basis is
+// rotated so that its first column is the t contrast direction (first r
+// columns span the F rows): the numerator is column 0 (or the sum of
+// squares of the first r columns).  Samples with SSE < |x|^2 / 64 flag
+// their permutation for an FP64 repeat.
+// ---------------------------------------------------------------------------
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// sum over the 16 lanes of a row, in every lane of the row (DPP row
+// rotations, no LDS)
+__device__ __forceinline__ float sum16(float x) {
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xF, 0xF, false));  // row_ror:8
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x124, 0xF, 0xF, false));  // row_ror:4
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x122, 0xF, 0xF, false));  // row_ror:2
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x121, 0xF, 0xF, false));  // row_ror:1
+    return x;
+}
+
+__device__ __forceinline__ void transpose4(float (&X)[4]) {
+    // [X_k] row blocks of 16 lanes -> X_g[16k + i] = X_k[16g + i] (the
+    // permlane32 swap exchanges the upper half of its first operand with
+    // the lower half of its second; permlane16 odd rows of the first with
+    // even rows of the second)
+    auto p02 = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[0]), __float_as_uint(X[2]), false, false);
+    auto p13 = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[1]), __float_as_uint(X[3]), false, false);
+    auto q01 = __builtin_amdgcn_permlane16_swap(p02[0], p13[0], false, false);
+    auto q23 = __builtin_amdgcn_permlane16_swap(p02[1], p13[1], false, false);
+    X[0] = __uint_as_float(q01[0]);
+    X[1] = __uint_as_float(q01[1]);
+    X[2] = __uint_as_float(q23[0]);
+    X[3] = __uint_as_float(q23[1]);
+}
+
+template <bool MEAN>
+__global__ __launch_bounds__(512) void k_fl_perm_mx(PermArgs a) {
+    extern __shared__ float tile[];  // [T][64] whitened tile
+    __shared__ float smax[WAVES * ROUNDS * NP];
+    __shared__ float red[WAVES][3][TILE];
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const int lid = xcd_logical_block((int)(gridDim.x >> 3));
+    if (lid >= n_blocks) return;
+    const int vchunk = lid / a.n_pgroups;
+    const int pgroup = lid - vchunk * a.n_pgroups;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile0 = vchunk * a.tiles_per_chunk;
+    const int tile1 = min(tile0 + a.tiles_per_chunk, a.n_vtiles);
+    // this lane's column c = lane % 16 of the rotated basis
+    const int c = lane & 15;
+    const float uc = a.u32[c];
+    const float invT = 1.0f / (float)a.T, invTm1 = 1.0f / ((float)a.T - 1.0f);
+    float* my = smax + wave * (ROUNDS * NP);
+    if (lane == 0)
+        for (int i = 0; i < ROUNDS * NP; i++) my[i] = -INFINITY;
+
+    for (int tl = tile0; tl < tile1; tl++) {
+        const int v = tl * TILE + lane;
+        __syncthreads();
+        {
+            const int n = a.T * TILE;
+            for (int i = threadIdx.x; i < n; i += WAVES * 64)
+                tile[i] = a.w32[(size_t)(i >> 6) * a.Vpad + (size_t)tl * TILE + (i & 63)];
+        }
+        __syncthreads();
+        const float a1 = (float)a.ar[v], a2 = (float)a.ar[(size_t)a.Vpad + v];
+        const float a3 = (float)a.ar[2 * (size_t)a.Vpad + v], a4 = (float)a.ar[3 * (size_t)a.Vpad + v];
+        const bool in_mask = v < a.Vc;
+        for (int r = 0; r < ROUNDS; r++) {
+            const int kbase = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP;
+            if (kbase >= a.K) break;
+            cuint_p prow[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = min(kbase + j, a.K - 1);
+                prow[j] = (cuint_p)(a.perm + (size_t)k * a.Tp);
+            }
+            f32x4 acc[NP][4];
+            float h[NP][4], s[NP], m[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+#pragma unroll
+                for (int g = 0; g < 4; g++) acc[j][g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int q = 0; q < 4; q++) h[j][q] = 0.0f;
+                s[j] = 0.0f;
+                m[j] = 0.0f;
+            }
+            // 4 time points from t0 (n valid): recursion, swaps, 4 MFMAs per
+            // permutation
+            auto block4 = [&](int t0, const unsigned (&idx)[NP][4], int n) {
+                const float b = a.Q32[(size_t)t0 * 16 + lane];  // B[k][col] = Q[t0 + lane/16][lane%16]
+#pragma unroll
+                for (int j = 0; j < NP; j++) {
+                    float X[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (k < n) {
+                            float x = tile[idx[j][k] * TILE + lane];
+                            x = fmaf(a4, h[j][3], x);
+                            x = fmaf(a3, h[j][2], x);
+                            x = fmaf(a2, h[j][1], x);
+                            x = fmaf(a1, h[j][0], x);
+                            h[j][3] = h[j][2];
+                            h[j][2] = h[j][1];
+                            h[j][1] = h[j][0];
+                            h[j][0] = x;
+                            s[j] = fmaf(x, x, s[j]);
+                            if (MEAN) m[j] += x;
+                            X[k] = x;
+                        } else {
+                            X[k] = 0.0f;
+                        }
+                    }
+                    transpose4(X);
+#pragma unroll
+                    for (int g = 0; g < 4; g++) acc[j][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(X[g], b, acc[j][g], 0, 0, 0);
+                }
+            };
+            const int Tfull = a.T & ~7;
+            for (int t8 = 0; t8 < Tfull; t8 += 8) {
+                unsigned pk[NP][4];
+#pragma unroll
+                for (int j = 0; j < NP; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) pk[j][q] = prow[j][(t8 >> 1) + q];
+#pragma unroll
+                for (int hb = 0; hb < 2; hb++) {
+                    unsigned idx[NP][4];
+#pragma unroll
+                    for (int j = 0; j < NP; j++)
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const int tt = hb * 4 + k;
+                            idx[j][k] = (pk[j][tt >> 1] >> ((tt & 1) * 16)) & 0xFFFFu;
+                        }
+                    block4(t8 + hb * 4, idx, 4);
+                }
+            }
+            for (int t0 = Tfull; t0 < a.T; t0 += 4) {
+                const int n = min(4, a.T - t0);
+                unsigned idx[NP][4];
+#pragma unroll
+                for (int j = 0; j < NP; j++)
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int t = min(t0 + k, a.T - 1);
+                        idx[j][k] = (prow[j][t >> 1] >> ((t & 1) * 16)) & 0xFFFFu;
+                    }
+                block4(t0, idx, n);
+            }
+            // epilogue: per voxel |z|^2, the statistic rows and u'z from the
+            // 16 lanes that hold its columns, back to lane = voxel via LDS
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = kbase + j;
+                const bool valid = k < a.K;
+#pragma unroll
+                for (int g = 0; g < 4; g++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const float zc = acc[j][g][rr];
+                        const float z2 = zc * zc;
+                        const float ss = sum16(z2);
+                        const float uz = MEAN ? sum16(uc * zc) : 0.0f;
+                        // t: column 0 (held by the writing lane); F: the
+                        // first nrows columns
+                        const float num = a.ftest ? sum16(c < a.nrows ? z2 : 0.0f) : zc;
+                        const int vv = 16 * g + 4 * (lane >> 4) + rr;  // voxel of this row
+                        if (c == 0) {
+                            red[wave][0][vv] = ss;
+                            red[wave][1][vv] = num;
+                            red[wave][2][vv] = uz;
+                        }
+                    }
+                const float ss = red[wave][0][lane], num = red[wave][1][lane], uz = red[wave][2][lane];
+                const float sse = s[j] - ss;
+                const float me = MEAN ? m[j] - uz : 0.0f;
+                const float var = (sse - me * me * invT) * invTm1;
+                const float st = a.ftest ? num / (var * (float)a.nrows) : num / sqrtf(var);
+                const bool act = in_mask && valid;
+                // conditioning: a sample whose residual sum of squares is
+                // below 1/64 of |x|^2 needs the FP64 form
+                const bool bad = act && !(sse >= s[j] * (1.0f / 64.0f));
+                if (__ballot(bad) && lane == 0) a.flags[k] = 1;
+                if (a.stats && act) a.stats[(size_t)k * a.Vc + v] = st;
+                if (a.bits) {
+                    const unsigned long long bal = __ballot(act && st > a.cdt);
+                    if (lane == 0 && valid) {
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl] = (unsigned)bal;
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl + 1] = (unsigned)(bal >> 32);
+                    }
+                }
+                const float mx = wave_max(act ? st : -INFINITY);
+                if (lane == 0) my[r * NP + j] = fmaxf(my[r * NP + j], mx);
+            }
+        }
+    }
+    if (lane == 0) {
+        for (int r = 0; r < ROUNDS; r++)
+            for (int j = 0; j < NP; j++) {
+                const int k = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP + j;
+                if (k < a.K) a.partial[(size_t)vchunk * a.K + k] = my[r * NP + j];
+            }
+    }
+}
+
+hipError_t launch_perm_mx(PermArgs a, hipStream_t s) {
+    if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
+    if (a.T > MX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS) return hipErrorInvalidValue;
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
+    const size_t lds = (size_t)a.T * TILE * sizeof(float);
+    static bool attr[2] = {false, false};
+    const int mi = a.mean ? 1 : 0;
+    if (!attr[mi]) {
+        const void* f = a.mean ? reinterpret_cast<const void*>(&k_fl_perm_mx<true>)
+                               : reinterpret_cast<const void*>(&k_fl_perm_mx<false>);
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)(MX_T_MAX * TILE * sizeof(float)));
+        if (e != hipSuccess) return e;
+        attr[mi] = true;
+    }
+    if (a.mean)
+        hipLaunchKernelGGL((k_fl_perm_mx<true>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_fl_perm_mx<false>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+    return hipGetLastError();
+}
+
 int fl_width(int p) {
     static const int widths[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 25};
     for (int w : widths)

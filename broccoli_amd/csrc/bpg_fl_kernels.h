@@ -20,6 +20,10 @@ constexpr int PERMS_PER_BLOCK = WAVES * NP * ROUNDS;
 constexpr int MAX_VCHUNKS = 1024;
 // largest T whose whitened voxel tile ([T][64] floats) is staged in LDS
 constexpr int LDS_T_MAX = 600;
+// largest T of the FP32 / MFMA form ([T][64] tile in LDS next to the
+// 6.25 KB of per-wave reduction space)
+constexpr int MX_T_MAX = 560;
+#define BPG_FL_MAX_FROWS 10
 
 struct Taps {
     float f[9];
@@ -61,6 +65,9 @@ struct PermArgs {
     const double* Q;               // [T][pq]
     const double* A;               // [nrows][pq]
     const double* u;               // [pq]
+    const float* Q32;              // [Tq][16] FP32 rotated basis, rows to T rounded up to 4 zero (FP32 form)
+    const float* u32;              // [16] FP32 rotated Q'1 (FP32 form)
+    unsigned char* flags;          // [K] ill-conditioned permutations (FP32 form)
     float* partial;                // [n_vchunks][K] per-chunk maxima
     float* stats;                  // [K][Vc] or null
     unsigned* bits;                // [K][nwords] stat > cdt, or null
@@ -71,6 +78,11 @@ struct PermArgs {
 // pq must be one of the compiled widths (fl_width)
 int fl_width(int p);
 hipError_t launch_perm(PermArgs a, hipStream_t s);
+// The FP32 form (pq <= 16, T <= MX_T_MAX): the recursion, |x|^2 and sum x
+// in FP32, z on the FP32 matrix cores; flags[k] = 1 for a permutation with a
+// sample of SSE < |x|^2 / 64 (the caller clears flags and repeats those in
+// the FP64 form).
+hipError_t launch_perm_mx(PermArgs a, hipStream_t s);
 
 }  // namespace fl
 }  // namespace bpg
