@@ -351,8 +351,9 @@ class FlPlan:
 
     study: dict with Y (T, D, H, W) float32, mask (D, H, W), X (T, p), C
     (nc, p), voxel (3 sizes in mm); optional test (TTEST), inference
-    (VOXEL), precision ("fp32": FP32 with the matrix cores and FP64 repeats,
-    or "fp64"), cdt, connectivity (26), fwhm (AR smoothing, 8 mm).
+    (VOXEL), precision ("split_bf16" or "fp32": FP32 recursion with the
+    projections on the matrix cores and FP64 repeats; "fp64"), cdt,
+    connectivity (26), fwhm (AR smoothing, 8 mm).
     """
 
     def __init__(self, ctx: Context, study: dict):
@@ -368,8 +369,9 @@ class FlPlan:
         d = FlDesc()
         d.test = s.get("test", TTEST)
         d.inference = s.get("inference", VOXEL)
-        prec = s.get("precision", "fp32")
-        d.precision = {"fp32": PREC_FP32, "fp64": PREC_FP64}[prec] if isinstance(prec, str) else prec
+        prec = s.get("precision", "split_bf16")
+        d.precision = ({"split_bf16": PREC_SPLIT_BF16, "fp32": PREC_FP32, "fp64": PREC_FP64}[prec]
+                       if isinstance(prec, str) else prec)
         d.connectivity = s.get("connectivity", 26)
         d.W, d.H, d.D, d.T = W, H, D, T
         d.p, d.nc = s["X"].shape[1], C.shape[0]

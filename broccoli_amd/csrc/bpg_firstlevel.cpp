@@ -36,6 +36,7 @@ struct bpg_fl_plan {
     DBuf<double> ar;           // [4][Vpad] total AR estimates
     DBuf<double> Q, A, u;      // model
     DBuf<float> Q32, u32;      // rotated model of the FP32 form ([Tq][16], [16])
+    DBuf<unsigned short> Qb;   // rotated model of the split-BF16 form ([Tq32/32][2][64][8])
     DBuf<unsigned char> flags; // [K] permutations to repeat in FP64
     HBuf<unsigned char> h_flags;
     DBuf<int> d_nbr;
@@ -50,7 +51,7 @@ struct bpg_fl_plan {
         w32.release(); ar.release(); Q.release(); A.release(); u.release(); d_nbr.release(); perm.release();
         partial.release(); maxout.release(); stats.release(); bits.release(); maxsize.release(); cl.release();
         h_perm.release(); h_max.release();
-        Q32.release(); u32.release(); flags.release(); h_flags.release();
+        Q32.release(); u32.release(); Qb.release(); flags.release(); h_flags.release();
     }
 };
 
@@ -77,12 +78,12 @@ void smoothing_filters(float fwhm, const float* vox, float (*f)[9]) {
     }
 }
 
-// The FP32 form's basis for statistical map `map`: Q M with M orthogonal
-// (p x p) whose first r columns are the statistic rows A_i (orthonormal), so
-// z' = M'z has z'_i = A_i z for i < r and |z'| = |z|; completed by
-// Gram-Schmidt on the unit vectors.  q32[t*16 + j] (rows to Tq, zeros
-// beyond T and p), u32 = M'u.
-void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<float>& q32, std::vector<float>& u32) {
+// The basis of the FP32 and split-BF16 forms for statistical map `map`:
+// Q M with M orthogonal (p x p) whose first r columns are the statistic rows
+// A_i (orthonormal), so z' = M'z has z'_i = A_i z for i < r and |z'| = |z|;
+// completed by Gram-Schmidt on the unit vectors.  rot[t*16 + j] (FP64, rows
+// to Tq, zeros beyond T and p), u32 = M'u.
+void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<double>& rot, std::vector<float>& u32) {
     const int p = m.p, r = m.nrows[map];
     const std::vector<double>& A = m.A[map];
     std::vector<std::vector<double>> cols;
@@ -106,7 +107,7 @@ void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<float>& q
         for (double& x : v) x /= std::sqrt(n2);
         cols.push_back(v);
     }
-    q32.assign((size_t)Tq * 16, 0.0f);
+    rot.assign((size_t)Tq * 16, 0.0);
     u32.assign(16, 0.0f);
     for (int i = 0; i < (int)cols.size(); i++) {
         double uu = 0.0;
@@ -115,9 +116,40 @@ void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<float>& q
         for (int t = 0; t < m.T; t++) {
             double acc = 0.0;
             for (int j = 0; j < p; j++) acc += m.Q[(size_t)t * m.pq + j] * cols[i][j];
-            q32[(size_t)t * 16 + i] = (float)acc;
+            rot[(size_t)t * 16 + i] = acc;
         }
     }
+}
+
+// bf16 bits of f, round to nearest even
+unsigned short bf16_bits(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+float bf16_value(unsigned short b) {
+    const unsigned u = (unsigned)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// The B operands of the split-BF16 form: per block of 32 time points, hi then
+// lo, lane l holding rot[t0 + 8 (l / 16) + e][l % 16] for e = 0..7.
+void split_basis(const std::vector<double>& rot, int Tq32, std::vector<unsigned short>& qb) {
+    const int nb = Tq32 / 32;
+    qb.assign((size_t)nb * 2 * 64 * 8, 0);
+    for (int b = 0; b < nb; b++)
+        for (int l = 0; l < 64; l++)
+            for (int e = 0; e < 8; e++) {
+                const int t = 32 * b + 8 * (l / 16) + e;
+                const double q = (size_t)t * 16 + (l % 16) < rot.size() ? rot[(size_t)t * 16 + (l % 16)] : 0.0;
+                const unsigned short hb = bf16_bits((float)q);
+                const unsigned short lb = bf16_bits((float)(q - (double)bf16_value(hb)));
+                qb[(((size_t)b * 2 + 0) * 64 + l) * 8 + e] = hb;
+                qb[(((size_t)b * 2 + 1) * 64 + l) * 8 + e] = lb;
+            }
 }
 
 // The device set-up: regression, then three rounds of AR(4) estimation,
@@ -208,8 +240,8 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
     *out = nullptr;
     const bpg_fl_desc& d = *desc;
     if (d.test != BPG_TTEST && d.test != BPG_FTEST) BPG_FAIL(ctx, BPG_ERR_ARG, "test must be t or F");
-    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_FP64)
-        BPG_FAIL(ctx, BPG_ERR_ARG, "precision must be BPG_PREC_FP32 or BPG_PREC_FP64");
+    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_FP64 && d.precision != BPG_PREC_SPLIT_BF16)
+        BPG_FAIL(ctx, BPG_ERR_ARG, "precision must be BPG_PREC_SPLIT_BF16, BPG_PREC_FP32 or BPG_PREC_FP64");
     if (d.inference == BPG_TFCE)
         BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED,
                  "TFCE is not available for the first-level permutation test (the reference leaves its "
@@ -286,8 +318,8 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
         return fail(BPG_ERR_HIP);
     }
     if (m.pq <= 16) {
-        const int Tq = (d.T + 3) & ~3;
-        if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16)) {
+        const int Tq = (d.T + 3) & ~3, Tq32 = (d.T + 31) & ~31;
+        if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16) || pl->Qb.ensure((size_t)Tq32 / 32 * 2 * 64 * 8)) {
             ctx->err = "device allocation failed";
             return fail(BPG_ERR_NOMEM);
         }
@@ -355,10 +387,11 @@ extern "C" int bpg_fl_plan_whitened(bpg_fl_plan* pl, float* volumes) {
 
 namespace {
 
-// One permutation phase in the FP32 form (when allowed and the study fits
-// it) or the FP64 form.  *fp32_used tells which ran.
+// One permutation phase in the requested fast form (FP32 or split-BF16
+// projections; when allowed and the study fits it) or the FP64 form.
+// *fast_used tells which ran.
 int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float* maxdist, float* stats,
-             bool allow_fp32, bool* fp32_used) {
+             bool allow_fast, bool* fast_used) {
     bpg_ctx* ctx = pl->ctx;
     const bpg_fl_desc& d = pl->d;
     const bpg::FlModel& m = pl->model;
@@ -392,7 +425,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
         for (long k = 0; k < K; k++) maxdist[k] = cluster ? 0.0f : bpg::empty_max();
         pl->timing = bpg_timing{};
         pl->timing.precision_used = BPG_PREC_FP64;
-        *fp32_used = false;
+        *fast_used = false;
         return BPG_OK;
     }
 
@@ -409,15 +442,30 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
         BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
         BPG_HIP(ctx, pl->maxsize.ensure(K));
     }
-    const bool mx = allow_fp32 && d.precision == BPG_PREC_FP32 && m.pq <= 16 && T <= bpg::fl::MX_T_MAX &&
-                    m.nrows[map] <= BPG_FL_MAX_FROWS;
-    *fp32_used = mx;
+    // the fast forms (FP32 MFMA or split-BF16 MFMA projections) and their
+    // limits; the FP64 form otherwise and for the repeats
+    const bool fast = allow_fast && d.precision != BPG_PREC_FP64 && m.pq <= 16 && T <= bpg::fl::MX_T_MAX &&
+                      m.nrows[map] <= BPG_FL_MAX_FROWS;
+    const bool bx = fast && d.precision == BPG_PREC_SPLIT_BF16;
+    const bool mx = fast;
+    *fast_used = mx;
     const std::vector<double>& Arows = m.A[map];
     BPG_HIP(ctx, hipMemcpyAsync(pl->A.p, Arows.data(), Arows.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    std::vector<double> rot;
     std::vector<float> q32, u32;
+    std::vector<unsigned short> qb;
     if (mx) {
-        rotated_basis(m, map, (T + 3) & ~3, q32, u32);
-        BPG_HIP(ctx, hipMemcpyAsync(pl->Q32.p, q32.data(), q32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        const int Tq = (T + 3) & ~3, Tq32 = (T + 31) & ~31;
+        rotated_basis(m, map, Tq32, rot, u32);
+        if (bx) {
+            split_basis(rot, Tq32, qb);
+            BPG_HIP(ctx, hipMemcpyAsync(pl->Qb.p, qb.data(), qb.size() * sizeof(unsigned short), hipMemcpyHostToDevice,
+                                        st));
+        } else {
+            q32.assign((size_t)Tq * 16, 0.0f);
+            for (size_t i = 0; i < q32.size(); i++) q32[i] = (float)rot[i];
+            BPG_HIP(ctx, hipMemcpyAsync(pl->Q32.p, q32.data(), q32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        }
         BPG_HIP(ctx, hipMemcpyAsync(pl->u32.p, u32.data(), 16 * sizeof(float), hipMemcpyHostToDevice, st));
         BPG_HIP(ctx, pl->flags.ensure(K));
         BPG_HIP(ctx, pl->h_flags.ensure(K));
@@ -442,11 +490,12 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     a.n_vtiles = n_vtiles; a.n_vchunks = n_vchunks; a.tiles_per_chunk = tpc; a.n_pgroups = n_pgroups;
     if (mx) {
         a.Q32 = pl->Q32.p;
+        a.Qb = reinterpret_cast<const unsigned*>(pl->Qb.p);
         a.u32 = pl->u32.p;
         a.flags = pl->flags.p;
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
-    BPG_HIP(ctx, mx ? bpg::fl::launch_perm_mx(a, st) : bpg::fl::launch_perm(a, st));
+    BPG_HIP(ctx, bx ? bpg::fl::launch_perm_bx(a, st) : mx ? bpg::fl::launch_perm_mx(a, st) : bpg::fl::launch_perm(a, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
     if (mx) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
     if (cluster) {
@@ -490,7 +539,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = d.p;
     tm.slots_per_perm = mx ? 16 : m.pq;
-    tm.precision_used = mx ? BPG_PREC_FP32 : BPG_PREC_FP64;
+    tm.precision_used = bx ? BPG_PREC_SPLIT_BF16 : mx ? BPG_PREC_FP32 : BPG_PREC_FP64;
     tm.fp32_permutations = mx ? K : 0;
     return BPG_OK;
 }
@@ -501,9 +550,9 @@ extern "C" int bpg_fl_plan_permutation_phase(bpg_fl_plan* pl, int map, const uns
                                              float* maxdist, float* stats) {
     if (!pl) return BPG_ERR_ARG;
     const auto t0 = std::chrono::steady_clock::now();
-    bool fp32 = false;
-    int rc = fl_phase(pl, map, perm, K, maxdist, stats, true, &fp32);
-    if (rc != BPG_OK || !fp32) return rc;
+    bool fast = false;
+    int rc = fl_phase(pl, map, perm, K, maxdist, stats, true, &fast);
+    if (rc != BPG_OK || !fast) return rc;
     // permutations with an ill-conditioned sample: repeated in FP64, their
     // results replace the FP32 ones
     std::vector<long> S;

@@ -633,6 +633,237 @@ __global__ __launch_bounds__(512) void k_fl_perm_mx(PermArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// the split-BF16 form: as the FP32 form, but the projections run on the BF16
+// matrix cores (v_mfma_f32_16x16x32_bf16), which overlap the vector work.
+// Per 32 time points each lane splits its voxel's x into bf16 hi + lo (8
+// per 4 registers, |x - hi - lo| <= 2^-18 |x|); the permlane transposes
+// turn each register slice into the A operands of the 4 row groups, and 3
+// products (hi.hi, hi.lo, lo.hi) per group accumulate z.  B is the rotated
+// basis split the same way, laid out per lane on the host.  Samples with
+// SSE < |x|^2 / 8 flag their permutation for the FP64 form.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void transpose4u(unsigned (&X)[4]) {
+    auto p02 = __builtin_amdgcn_permlane32_swap(X[0], X[2], false, false);
+    auto p13 = __builtin_amdgcn_permlane32_swap(X[1], X[3], false, false);
+    auto q01 = __builtin_amdgcn_permlane16_swap(p02[0], p13[0], false, false);
+    auto q23 = __builtin_amdgcn_permlane16_swap(p02[1], p13[1], false, false);
+    X[0] = q01[0];
+    X[1] = q01[1];
+    X[2] = q23[0];
+    X[3] = q23[1];
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// (x0, x1) -> packed bf16 hi parts and bf16 remainders, round to nearest
+// even: one v_cvt_pk_bf16_f32 per pair for each
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned& hi, unsigned& lo) {
+    const bf16x2 h = __builtin_convertvector(f32x2{x0, x1}, bf16x2);
+    hi = __builtin_bit_cast(unsigned, h);
+    const float h0 = __uint_as_float(hi << 16), h1 = __uint_as_float(hi & 0xFFFF0000u);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{x0 - h0, x1 - h1}, bf16x2));
+}
+
+template <bool MEAN>
+__global__ __launch_bounds__(512) void k_fl_perm_bx(PermArgs a) {
+    extern __shared__ float tile[];  // [T][64] whitened tile
+    __shared__ float smax[WAVES * ROUNDS * NP];
+    __shared__ float red[WAVES][3][TILE];
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const int lid = xcd_logical_block((int)(gridDim.x >> 3));
+    if (lid >= n_blocks) return;
+    const int vchunk = lid / a.n_pgroups;
+    const int pgroup = lid - vchunk * a.n_pgroups;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile0 = vchunk * a.tiles_per_chunk;
+    const int tile1 = min(tile0 + a.tiles_per_chunk, a.n_vtiles);
+    const bf16x8* QB = reinterpret_cast<const bf16x8*>(a.Qb);
+    const int c = lane & 15;
+    const float uc = a.u32[c];
+    const float invT = 1.0f / (float)a.T, invTm1 = 1.0f / ((float)a.T - 1.0f);
+    float* my = smax + wave * (ROUNDS * NP);
+    if (lane == 0)
+        for (int i = 0; i < ROUNDS * NP; i++) my[i] = -INFINITY;
+
+    for (int tl = tile0; tl < tile1; tl++) {
+        const int v = tl * TILE + lane;
+        __syncthreads();
+        {
+            const int n = a.T * TILE;
+            for (int i = threadIdx.x; i < n; i += WAVES * 64)
+                tile[i] = a.w32[(size_t)(i >> 6) * a.Vpad + (size_t)tl * TILE + (i & 63)];
+        }
+        __syncthreads();
+        const float a1 = (float)a.ar[v], a2 = (float)a.ar[(size_t)a.Vpad + v];
+        const float a3 = (float)a.ar[2 * (size_t)a.Vpad + v], a4 = (float)a.ar[3 * (size_t)a.Vpad + v];
+        const bool in_mask = v < a.Vc;
+        for (int r = 0; r < ROUNDS; r++) {
+            const int kbase = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP;
+            if (kbase >= a.K) break;
+            cuint_p prow[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = min(kbase + j, a.K - 1);
+                prow[j] = (cuint_p)(a.perm + (size_t)k * a.Tp);
+            }
+            f32x4 acc[NP][4];
+            float h[NP][4], s[NP], m[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+#pragma unroll
+                for (int g = 0; g < 4; g++) acc[j][g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int q = 0; q < 4; q++) h[j][q] = 0.0f;
+                s[j] = 0.0f;
+                m[j] = 0.0f;
+            }
+            // 32 time points from t0, n of them valid
+            auto block32 = [&](int t0, int n) {
+                const bf16x8 bh = QB[((size_t)(t0 >> 5) * 2) * 64 + lane];
+                const bf16x8 bl = QB[((size_t)(t0 >> 5) * 2 + 1) * 64 + lane];
+                // one permutation at a time keeps 32 split registers live
+#pragma unroll
+                for (int j = 0; j < NP; j++) {
+                    unsigned hi[16], lo[16];  // [chunk * 4 + w]: bf16 pair of times 8 chunk + 2w, +1
+#pragma unroll
+                    for (int ch = 0; ch < 4; ch++) {
+                        unsigned pk[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int t = min(t0 + 8 * ch + 2 * q, a.Tp - 2);
+                            pk[q] = prow[j][t >> 1];
+                        }
+                        float xs[8];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            if (8 * ch + e < n) {
+                                const unsigned idx = (pk[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu;
+                                float x = tile[idx * TILE + lane];
+                                x = fmaf(a4, h[j][3], x);
+                                x = fmaf(a3, h[j][2], x);
+                                x = fmaf(a2, h[j][1], x);
+                                x = fmaf(a1, h[j][0], x);
+                                h[j][3] = h[j][2];
+                                h[j][2] = h[j][1];
+                                h[j][1] = h[j][0];
+                                h[j][0] = x;
+                                s[j] = fmaf(x, x, s[j]);
+                                if (MEAN) m[j] += x;
+                                xs[e] = x;
+                            } else {
+                                xs[e] = 0.0f;
+                            }
+                        }
+#pragma unroll
+                        for (int w = 0; w < 4; w++) split_pair(xs[2 * w], xs[2 * w + 1], hi[ch * 4 + w], lo[ch * 4 + w]);
+                    }
+                    // per register slice w: chunk-major -> row-group-major
+                    unsigned ah[4][4], al[4][4];  // [group][w]
+#pragma unroll
+                    for (int w = 0; w < 4; w++) {
+                        unsigned xh[4] = {hi[w], hi[4 + w], hi[8 + w], hi[12 + w]};
+                        unsigned xl[4] = {lo[w], lo[4 + w], lo[8 + w], lo[12 + w]};
+                        transpose4u(xh);
+                        transpose4u(xl);
+#pragma unroll
+                        for (int g = 0; g < 4; g++) {
+                            ah[g][w] = xh[g];
+                            al[g][w] = xl[g];
+                        }
+                    }
+#pragma unroll
+                    for (int g = 0; g < 4; g++) {
+                        typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                        const bf16x8 Ah = __builtin_bit_cast(bf16x8, u32x4{ah[g][0], ah[g][1], ah[g][2], ah[g][3]});
+                        const bf16x8 Al = __builtin_bit_cast(bf16x8, u32x4{al[g][0], al[g][1], al[g][2], al[g][3]});
+                        acc[j][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, bh, acc[j][g], 0, 0, 0);
+                        acc[j][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, bl, acc[j][g], 0, 0, 0);
+                        acc[j][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, bh, acc[j][g], 0, 0, 0);
+                    }
+                }
+            };
+            const int Tfull = a.T & ~31;
+            for (int t0 = 0; t0 < Tfull; t0 += 32) block32(t0, 32);
+            if (Tfull < a.T) block32(Tfull, a.T - Tfull);
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = kbase + j;
+                const bool valid = k < a.K;
+#pragma unroll
+                for (int g = 0; g < 4; g++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const float zc = acc[j][g][rr];
+                        const float z2 = zc * zc;
+                        const float ss = sum16(z2);
+                        const float uz = MEAN ? sum16(uc * zc) : 0.0f;
+                        const float num = a.ftest ? sum16(c < a.nrows ? z2 : 0.0f) : zc;
+                        const int vv = 16 * g + 4 * (lane >> 4) + rr;
+                        if (c == 0) {
+                            red[wave][0][vv] = ss;
+                            red[wave][1][vv] = num;
+                            red[wave][2][vv] = uz;
+                        }
+                    }
+                const float ss = red[wave][0][lane], num = red[wave][1][lane], uz = red[wave][2][lane];
+                const float sse = s[j] - ss;
+                const float me = MEAN ? m[j] - uz : 0.0f;
+                const float var = (sse - me * me * invT) * invTm1;
+                const float st = a.ftest ? num / (var * (float)a.nrows) : num / sqrtf(var);
+                const bool act = in_mask && valid;
+                // the split-BF16 z is good to about 1e-5 of |x|: samples with
+                // SSE below 1/8 of |x|^2 need the FP64 form
+                const bool bad = act && !(sse >= s[j] * 0.125f);
+                if (__ballot(bad) && lane == 0) a.flags[k] = 1;
+                if (a.stats && act) a.stats[(size_t)k * a.Vc + v] = st;
+                if (a.bits) {
+                    const unsigned long long bal = __ballot(act && st > a.cdt);
+                    if (lane == 0 && valid) {
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl] = (unsigned)bal;
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl + 1] = (unsigned)(bal >> 32);
+                    }
+                }
+                const float mx = wave_max(act ? st : -INFINITY);
+                if (lane == 0) my[r * NP + j] = fmaxf(my[r * NP + j], mx);
+            }
+        }
+    }
+    if (lane == 0) {
+        for (int r = 0; r < ROUNDS; r++)
+            for (int j = 0; j < NP; j++) {
+                const int k = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP + j;
+                if (k < a.K) a.partial[(size_t)vchunk * a.K + k] = my[r * NP + j];
+            }
+    }
+}
+
+hipError_t launch_perm_bx(PermArgs a, hipStream_t s) {
+    if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
+    if (a.T > MX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS) return hipErrorInvalidValue;
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
+    const size_t lds = (size_t)a.T * TILE * sizeof(float);
+    static bool attr[2] = {false, false};
+    const int mi = a.mean ? 1 : 0;
+    if (!attr[mi]) {
+        const void* f = a.mean ? reinterpret_cast<const void*>(&k_fl_perm_bx<true>)
+                               : reinterpret_cast<const void*>(&k_fl_perm_bx<false>);
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)(MX_T_MAX * TILE * sizeof(float)));
+        if (e != hipSuccess) return e;
+        attr[mi] = true;
+    }
+    if (a.mean)
+        hipLaunchKernelGGL((k_fl_perm_bx<true>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_fl_perm_bx<false>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_perm_mx(PermArgs a, hipStream_t s) {
     if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
     if (a.T > MX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS) return hipErrorInvalidValue;

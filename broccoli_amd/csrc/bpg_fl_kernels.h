@@ -66,7 +66,8 @@ struct PermArgs {
     const double* A;               // [nrows][pq]
     const double* u;               // [pq]
     const float* Q32;              // [Tq][16] FP32 rotated basis, rows to T rounded up to 4 zero (FP32 form)
-    const float* u32;              // [16] FP32 rotated Q'1 (FP32 form)
+    const float* u32;              // [16] FP32 rotated Q'1 (FP32 and split-BF16 forms)
+    const unsigned* Qb;            // [Tq32/32][2][64] lanes x 8 bf16: the rotated basis, hi then lo (split-BF16 form)
     unsigned char* flags;          // [K] ill-conditioned permutations (FP32 form)
     float* partial;                // [n_vchunks][K] per-chunk maxima
     float* stats;                  // [K][Vc] or null
@@ -83,6 +84,9 @@ hipError_t launch_perm(PermArgs a, hipStream_t s);
 // sample of SSE < |x|^2 / 64 (the caller clears flags and repeats those in
 // the FP64 form).
 hipError_t launch_perm_mx(PermArgs a, hipStream_t s);
+// The split-BF16 form (same limits): projections on the BF16 matrix cores
+// from hi/lo splits of x and of the basis; flags samples with SSE < |x|^2/8.
+hipError_t launch_perm_bx(PermArgs a, hipStream_t s);
 
 }  // namespace fl
 }  // namespace bpg

@@ -220,11 +220,13 @@ typedef struct {
     int inference;          /* BPG_VOXEL / BPG_CLUSTER_EXTENT /
                                BPG_CLUSTER_MASS (the reference leaves the
                                TFCE distribution unset, LIB:16524-16530)      */
-    int precision;          /* BPG_PREC_FP32: the recursion in FP32 (the
-                               reference's arithmetic) with the projections on
-                               the FP32 matrix cores, ill-conditioned
-                               permutations repeated in FP64 (p <= 16,
-                               T <= 480; other studies run FP64);
+    int precision;          /* the recursion in FP32 (the reference's
+                               arithmetic) with the projections on the matrix
+                               cores, ill-conditioned permutations repeated in
+                               FP64 (p <= 16, T <= 560; other studies run
+                               FP64): BPG_PREC_SPLIT_BF16 (BF16 hi/lo splits,
+                               samples with SSE < |x|^2/8 repeated) or
+                               BPG_PREC_FP32 (FP32 MFMA, SSE < |x|^2/64);
                                BPG_PREC_FP64: everything in FP64             */
     int connectivity;       /* 26 (the reference's) or 6                      */
     int W, H, D, T;         /* EPI grid and time points (10 <= T <= 65535)    */
@@ -275,8 +277,8 @@ int bpg_fl_plan_permutation_phase(bpg_fl_plan* plan, int map, const unsigned sho
                                   float* maxdist, float* stats);
 /* setup_ms = transform_ms (regression + whitening), contract_ms = the
  * permutation kernel, cluster_ms, phase_ms (host wall time);
- * precision_used = the form that ran; with BPG_PREC_FP32, unsafe_waves counts
- * the permutations with a sample of SSE < |x|^2 / 64, which were repeated in
+ * precision_used = the form that ran; with a fast form, unsafe_waves counts
+ * the permutations with an ill-conditioned sample, which were repeated in
  * FP64, and fp32_permutations the others. */
 int bpg_fl_plan_last_timing(const bpg_fl_plan* plan, bpg_timing* out);
 

@@ -62,7 +62,10 @@ def test_setup_matches_the_oracle(ctx):
     np.testing.assert_allclose(wh_g[:, m], wh[:, m], rtol=1e-6, atol=1e-5)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+PREC = {"split_bf16": capi.PREC_SPLIT_BF16, "fp32": capi.PREC_FP32, "fp64": capi.PREC_FP64}
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "fp32", "fp64"])
 @pytest.mark.parametrize("seed,T,p", [(2, 64, 5), (3, 120, 8), (4, 48, 1), (14, 242, 12)])
 def test_permuted_t_statistics_match_fp64(ctx, seed, T, p, prec):
     st = fl_study(seed=seed, T=T, p=p, ar_gradient=0.2)
@@ -76,14 +79,14 @@ def test_permuted_t_statistics_match_fp64(ctx, seed, T, p, prec):
     with capi.FlPlan(ctx, st) as plan:
         md, stats = plan.permutation_phase(0, perm, want_stats=True)
         tm = plan.timing()
-    assert tm.precision_used == (capi.PREC_FP32 if prec == "fp32" else capi.PREC_FP64)
+    assert tm.precision_used == PREC[prec]
     assert tm.unsafe_waves == 0
     check_stats(stats, stats_ref, in_mask(st))
     check_max(md, stats)
     np.testing.assert_allclose(md, md_ref, atol=2e-4, rtol=1e-5)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("prec", ["split_bf16", "fp32", "fp64"])
 def test_design_without_an_intercept(ctx, prec):
     # the residuals need not sum to zero: Welford's mean term is kept
     st = fl_study(seed=12, T=61, p=5)
@@ -100,7 +103,7 @@ def test_design_without_an_intercept(ctx, prec):
     check_max(md, stats)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("prec", ["split_bf16", "fp32", "fp64"])
 def test_f_statistics_and_second_contrast(ctx, prec):
     st = fl_study(seed=5, T=80, p=6)
     st["precision"] = prec
@@ -163,7 +166,9 @@ def test_wide_designs(ctx, p):
     check_max(md, stats)
 
 
-@pytest.mark.parametrize("inference,conn,prec", [(capi.CLUSTER_EXTENT, 26, "fp32"), (capi.CLUSTER_MASS, 26, "fp32"),
+@pytest.mark.parametrize("inference,conn,prec", [(capi.CLUSTER_EXTENT, 26, "split_bf16"),
+                                                 (capi.CLUSTER_MASS, 26, "split_bf16"),
+                                                 (capi.CLUSTER_EXTENT, 26, "fp32"), (capi.CLUSTER_MASS, 26, "fp32"),
                                                  (capi.CLUSTER_EXTENT, 6, "fp64"), (capi.CLUSTER_MASS, 26, "fp64")])
 def test_cluster_inference_on_the_gpu_statistics(ctx, inference, conn, prec):
     st = fl_study(seed=10, W=16, H=16, D=12, T=60)
@@ -233,11 +238,12 @@ def test_empty_and_single_voxel_masks(ctx, inference):
     check_stats(stats, stats_ref, one == 1.0)
 
 
-def test_ill_conditioned_permutations_repeat_in_fp64(ctx):
+@pytest.mark.parametrize("prec", ["split_bf16", "fp32"])
+def test_ill_conditioned_permutations_repeat_in_fp64(ctx, prec):
     # 16 regressors on 20 time points: some samples keep less than 1/64 of
-    # |x|^2 as residual, which the FP32 form hands to the FP64 form
+    # |x|^2 as residual, which the fast forms hand to the FP64 form
     st = fl_study(seed=15, T=20, p=16)
-    st["precision"] = "fp32"
+    st["precision"] = prec
     K = 24
     perm = capi.gen_perm_first_level(20, K, seed=7)
     print("first-level ill-conditioned, T=20 p=16", flush=True)
@@ -245,16 +251,17 @@ def test_ill_conditioned_permutations_repeat_in_fp64(ctx):
     with capi.FlPlan(ctx, st) as plan:
         md, stats = plan.permutation_phase(0, perm, want_stats=True)
         tm = plan.timing()
-    assert tm.precision_used == capi.PREC_FP32
+    assert tm.precision_used == PREC[prec]
     assert tm.unsafe_waves > 0 and tm.unsafe_waves + tm.fp32_permutations == K
     check_stats(stats, stats_ref, in_mask(st))
     check_max(md, stats)
 
 
-def test_fp32_form_falls_back_to_fp64_outside_its_limits(ctx):
+@pytest.mark.parametrize("prec", ["split_bf16", "fp32"])
+def test_fast_forms_fall_back_to_fp64_outside_their_limits(ctx, prec):
     for kw in ({"T": 90, "p": 25}, {"W": 7, "H": 6, "D": 5, "T": 600, "p": 4}):
         st = fl_study(seed=16, **kw)
-        st["precision"] = "fp32"
+        st["precision"] = prec
         perm = capi.gen_perm_first_level(kw["T"], 8, seed=1)
         print(f"first-level fallback {kw}", flush=True)
         with capi.FlPlan(ctx, st) as plan:
