@@ -314,8 +314,12 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     // buffers
     BPG_HIP(ctx, pl->yy.ensure(Vpad));
     BPG_HIP(ctx, pl->Qt.ensure((size_t)N * pfit));
-    BPG_HIP(ctx, pl->Qn.ensure((size_t)N * std::max(m.q, 1)));
-    BPG_HIP(ctx, pl->dscale.ensure(N));
+    // nuisance basis row-major, zero-padded to QB columns and NR rows
+    const int QB = bpg::transform_width(m.q);
+    const int NR = (N + 15) / 16 * 16;
+    if (QB < 0) BPG_FAIL(ctx, BPG_ERR_DESIGN, "more than 24 nuisance regressors");
+    BPG_HIP(ctx, pl->Qn.ensure((size_t)NR * std::max(QB, 1)));
+    BPG_HIP(ctx, pl->dscale.ensure(NR));
     if (bf) {
         BPG_HIP(ctx, pl->Yb.ensure((size_t)(Vpad / 32) * KK * 2 * 64 * 4));
         BPG_HIP(ctx, pl->Gb.ensure((size_t)n_sub * KK * 2 * 64 * 4));
@@ -335,20 +339,22 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
     // pinned staging (free at entry: every phase ends with a stream sync)
-    const size_t nQt = m.Qt.size(), nQn = m.q > 0 ? m.Qn.size() : 0;
-    BPG_HIP(ctx, pl->h_model.ensure(nQt + nQn + N));
+    const size_t nQt = m.Qt.size(), nQn = (size_t)NR * QB;
+    BPG_HIP(ctx, pl->h_model.ensure(nQt + nQn + NR));
     BPG_HIP(ctx, pl->h_max.ensure(K));
     if (m.sign_operand) BPG_HIP(ctx, pl->h_sign.ensure((size_t)K * N));
     else BPG_HIP(ctx, pl->h_perm.ensure((size_t)K * N));
     double* hm = pl->h_model.p;
     std::memcpy(hm, m.Qt.data(), nQt * sizeof(double));
-    if (nQn) std::memcpy(hm + nQt, m.Qn.data(), nQn * sizeof(double));
+    std::fill(hm + nQt, hm + nQt + nQn + NR, 0.0);
+    for (int j = 0; j < m.q; j++)
+        for (int s = 0; s < N; s++) hm[nQt + (size_t)s * QB + j] = m.Qn[(size_t)j * N + s];
     std::memcpy(hm + nQt + nQn, m.dscale.data(), N * sizeof(double));
 
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
     BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, hm, nQt * sizeof(double), hipMemcpyHostToDevice, st));
     if (nQn) BPG_HIP(ctx, hipMemcpyAsync(pl->Qn.p, hm + nQt, nQn * sizeof(double), hipMemcpyHostToDevice, st));
-    BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, hm + nQt + nQn, N * sizeof(double), hipMemcpyHostToDevice, st));
+    BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, hm + nQt + nQn, NR * sizeof(double), hipMemcpyHostToDevice, st));
     {
         bpg::TransformArgs ta{};
         ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;

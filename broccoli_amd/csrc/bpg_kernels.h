@@ -40,15 +40,17 @@ namespace bpg {
 
 struct TransformArgs {
     const float* Yc;       // [N][Vc] compact, subject-major
-    const double* Qn;      // [q][N] nuisance basis (column-major N x q)
-    const double* dscale;  // [N]
+    const double* Qn;      // [NR][transform_width(q)] nuisance basis, row-major, zero-padded
+    const double* dscale;  // [NR], zero beyond N (NR = N rounded up to 16)
     int N, q, Vc, Vpad;
-    int KK;                // BF16 k-steps of 16 subjects (0: no BF16 output)
-    int NS;                // FP32 operand rows (0: no FP32 output)
-    double* yy;            // [Vpad] |y*|^2 (padding voxels: 1)
+    int KK;                // BF16 k-steps of 16 subjects (with Yb)
+    int NS;                // FP32 operand rows, >= N (with Y32)
+    double* yy;            // [Vpad] |y*|^2 (padding voxels: NaN)
     unsigned* Yb;          // BF16 fragments, (Vpad/32)*KK*2*64 uint4
     float* Y32;            // [NS][Vpad]
 };
+// padded width of the nuisance basis for q columns (0 for none, -1 if q > 24)
+int transform_width(int q);
 hipError_t launch_transform(const TransformArgs& a, hipStream_t s);
 
 struct OperandArgs {

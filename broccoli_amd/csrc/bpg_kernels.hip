@@ -33,46 +33,85 @@ namespace bpg {
 // ---------------------------------------------------------------------------
 // y* = y - Qn (Qn' y)  (the projector I - W2 pinv(W2) of LIB:16684-16689,
 // applied in FP64), yy = |y*|^2 in FP64, operand row s = dscale_s * y*_s.
-// One thread per voxel.
+// One thread per voxel.  Qn is zero-padded to QB columns and, like dscale,
+// to a multiple of 16 rows, so the loops carry no guards and the scalar
+// loads of a row batch; subjects are read 8 at a time (8 loads in flight
+// per thread), and each group of 8 operand rows leaves as one 16-byte hi
+// fragment and one 16-byte lo fragment (a wave writes 512 contiguous bytes
+// per store).
+// (the indices are clamped into range and the values selected afterwards,
+// so the 8 loads issue together; Vc >= 1 whenever the kernel runs)
+__device__ __forceinline__ void load8(const TransformArgs& a, int v, bool in, int s0, double (&y)[8]) {
+    const int vc = min(v, a.Vc - 1);
+    float f[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) f[e] = a.Yc[(size_t)min(s0 + e, a.N - 1) * a.Vc + vc];
+#pragma unroll
+    for (int e = 0; e < 8; e++) y[e] = (in && s0 + e < a.N) ? (double)f[e] : 0.0;
+}
+
+template <int QB>
 __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.Vpad) return;
-    const int N = a.N;
     const bool in = v < a.Vc;
-    double w[24];
+    double w[QB > 0 ? QB : 1];
 #pragma unroll
-    for (int j = 0; j < 24; j++) w[j] = 0.0;
-    if (in && a.q > 0) {
-        for (int s = 0; s < N; s++) {
-            const double ys = (double)a.Yc[(size_t)s * a.Vc + v];
+    for (int j = 0; j < QB; j++) w[j] = 0.0;
+    if (QB > 0 && in) {
+        for (int s0 = 0; s0 < a.N; s0 += 8) {
+            double y[8];
+            load8(a, v, true, s0, y);
 #pragma unroll
-            for (int j = 0; j < 24; j++)
-                if (j < a.q) w[j] += a.Qn[s + (size_t)j * N] * ys;
+            for (int e = 0; e < 8; e++) {
+                const double* q = a.Qn + (size_t)(s0 + e) * QB;
+#pragma unroll
+                for (int j = 0; j < QB; j++) w[j] += q[j] * y[e];
+            }
         }
     }
-    double yy = 0.0;
-    const int vt = v >> 5, col = v & 31;
-    unsigned short* Yb16 = reinterpret_cast<unsigned short*>(a.Yb);
-    const int srows = (a.KK * 16 > a.NS) ? a.KK * 16 : a.NS;
-    for (int s = 0; s < srows; s++) {
-        double o = 0.0;
-        if (in && s < N) {
-            double ys = (double)a.Yc[(size_t)s * a.Vc + v];
+    // o[e] = dscale_s * y*_s for the 8 subjects from s0 (zero beyond N)
+    auto residual8 = [&](int s0, double (&o)[8], double& yy) {
+        load8(a, v, in, s0, o);
 #pragma unroll
-            for (int j = 0; j < 24; j++)
-                if (j < a.q) ys -= a.Qn[s + (size_t)j * N] * w[j];
+        for (int e = 0; e < 8; e++) {
+            const double* q = a.Qn + (size_t)(s0 + e) * QB;
+            double ys = o[e];
+#pragma unroll
+            for (int j = 0; j < QB; j++) ys -= q[j] * w[j];
             yy += ys * ys;
-            o = a.dscale[s] * ys;
+            o[e] = a.dscale[s0 + e] * ys;
         }
-        if (a.Y32 && s < a.NS) a.Y32[(size_t)s * a.Vpad + v] = (float)o;
-        if (a.Yb && s < a.KK * 16) {
-            const int kk = s >> 4, h = (s >> 3) & 1, e = s & 7;
-            const int lane = h * 32 + col;
-            const size_t base = (((size_t)vt * a.KK + kk) * 2) * 64 + lane;
-            const unsigned short hi = f2bf((float)o);
-            const unsigned short lo = f2bf((float)(o - (double)bf2f(hi)));
-            Yb16[base * 8 + e] = hi;
-            Yb16[(base + 64) * 8 + e] = lo;
+    };
+    double yy = 0.0;
+    if (a.Yb) {
+        const int vt = v >> 5, col = v & 31;
+        uint4* Yb4 = reinterpret_cast<uint4*>(a.Yb);
+        for (int kk = 0; kk < a.KK; kk++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                double o[8];
+                residual8(kk * 16 + h * 8, o, yy);
+                unsigned hi[4], lo[4];
+#pragma unroll
+                for (int e = 0; e < 8; e += 2) {
+                    const unsigned short h0 = f2bf((float)o[e]), h1 = f2bf((float)o[e + 1]);
+                    const unsigned short l0 = f2bf((float)(o[e] - (double)bf2f(h0)));
+                    const unsigned short l1 = f2bf((float)(o[e + 1] - (double)bf2f(h1)));
+                    hi[e / 2] = h0 | ((unsigned)h1 << 16);
+                    lo[e / 2] = l0 | ((unsigned)l1 << 16);
+                }
+                const size_t base = (((size_t)vt * a.KK + kk) * 2) * 64 + h * 32 + col;
+                Yb4[base] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+                Yb4[base + 64] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+            }
+    } else {
+        for (int s0 = 0; s0 < a.NS; s0 += 8) {
+            double o[8];
+            residual8(s0, o, yy);
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (s0 + e < a.NS) a.Y32[(size_t)(s0 + e) * a.Vpad + v] = (float)o[e];
         }
     }
     // padding voxels carry a NaN norm: their statistic is NaN, which the
@@ -80,10 +119,24 @@ __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
     a.yy[v] = in ? yy : __builtin_nan("");
 }
 
+int transform_width(int q) {
+    if (q <= 0) return 0;
+    if (q <= 4) return 4;
+    if (q <= 8) return 8;
+    if (q <= 16) return 16;
+    return q <= 24 ? 24 : -1;
+}
+
 hipError_t launch_transform(const TransformArgs& a, hipStream_t s) {
-    if (a.q > 24) return hipErrorInvalidValue;
     const int blocks = (a.Vpad + 255) / 256;
-    hipLaunchKernelGGL(k_transform, dim3(blocks), dim3(256), 0, s, a);
+    switch (transform_width(a.q)) {
+        case 0: hipLaunchKernelGGL(k_transform<0>, dim3(blocks), dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_transform<4>, dim3(blocks), dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_transform<8>, dim3(blocks), dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL(k_transform<16>, dim3(blocks), dim3(256), 0, s, a); break;
+        case 24: hipLaunchKernelGGL(k_transform<24>, dim3(blocks), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -97,11 +150,10 @@ hipError_t launch_transform(const TransformArgs& a, hipStream_t s) {
 // of one voxel and the statistic needs no cross-lane traffic.
 __global__ __launch_bounds__(256) void k_operand_bf16(OperandArgs a) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)a.n_sub * a.KK * 2 * 64;
+    const size_t total = (size_t)a.n_sub * a.KK * 64;
     if (t >= total) return;
     const int lane = (int)(t & 63);
-    const int part = (int)((t >> 6) & 1);
-    const size_t rest = t >> 7;
+    const size_t rest = t >> 6;
     const int kk = (int)(rest % a.KK);
     const int st = (int)(rest / a.KK);
     const int i = lane & 31, hl = lane >> 5;
@@ -112,28 +164,41 @@ __global__ __launch_bounds__(256) void k_operand_bf16(OperandArgs a) {
     const int reg = j % a.PP;
     const bool live = j < per * a.PP && k < a.K && reg < a.pfit;
     const int s0 = kk * 16 + hl * 8;
-    unsigned short out[8];
+    // gathers from clamped (in-range) indices, selected afterwards: the 8
+    // loads issue together (the host checked every index against N)
+    const size_t row = (size_t)min(k, a.K - 1) * a.N;
+    const size_t col = (size_t)min(reg, a.pfit - 1) * a.N;
+    double g[8];
+    if (a.sign) {
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int s = s0 + e;
-        double val = 0.0;
-        if (live && s < a.N) {
-            if (a.sign) val = (double)a.sign[(size_t)k * a.N + s];
-            else val = a.Qt[(size_t)a.perm[(size_t)k * a.N + s] + (size_t)reg * a.N];
-        }
-        const unsigned short hi = f2bf((float)val);
-        out[e] = part == 0 ? hi : f2bf((float)(val - (double)bf2f(hi)));
+        for (int e = 0; e < 8; e++) g[e] = (double)a.sign[row + min(s0 + e, a.N - 1)];
+    } else {
+        unsigned short ix[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) ix[e] = a.perm[row + min(s0 + e, a.N - 1)];
+#pragma unroll
+        for (int e = 0; e < 8; e++) g[e] = a.Qt[(size_t)ix[e] + col];
     }
-    uint4 pk;
-    pk.x = out[0] | ((unsigned)out[1] << 16);
-    pk.y = out[2] | ((unsigned)out[3] << 16);
-    pk.z = out[4] | ((unsigned)out[5] << 16);
-    pk.w = out[6] | ((unsigned)out[7] << 16);
-    reinterpret_cast<uint4*>(a.Gb)[t] = pk;
+    unsigned hi[4], lo[4];
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        double val[2];
+#pragma unroll
+        for (int d = 0; d < 2; d++) val[d] = (live && s0 + e + d < a.N) ? g[e + d] : 0.0;
+        const unsigned short h0 = f2bf((float)val[0]), h1 = f2bf((float)val[1]);
+        const unsigned short l0 = f2bf((float)(val[0] - (double)bf2f(h0)));
+        const unsigned short l1 = f2bf((float)(val[1] - (double)bf2f(h1)));
+        hi[e / 2] = h0 | ((unsigned)h1 << 16);
+        lo[e / 2] = l0 | ((unsigned)l1 << 16);
+    }
+    // fragment (st, kk): 64 lanes of hi, then 64 lanes of lo
+    uint4* G4 = reinterpret_cast<uint4*>(a.Gb) + (rest * 2) * 64 + lane;
+    G4[0] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    G4[64] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
 }
 
 hipError_t launch_operand_bf16(const OperandArgs& a, hipStream_t s) {
-    const size_t total = (size_t)a.n_sub * a.KK * 2 * 64;
+    const size_t total = (size_t)a.n_sub * a.KK * 64;
     const unsigned blocks = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(k_operand_bf16, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
