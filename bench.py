@@ -115,7 +115,7 @@ PEAK_FP32_MATRIX_TFLOPS = 157.3  # MI355X FP32 matrix
 # launch from the FETCH_SIZE / WRITE_SIZE passes, scaled by a copy of known
 # size (the counters cannot be read from inside this process)
 PROFILES = {2: ROOT / "profiles" / "k_contract_bf16_two_sample_glm_10k.md",
-            6: ROOT / "profiles" / "k_fl_perm_bx_first_level_10k.md"}
+            6: ROOT / "profiles" / "k_fl_perm_hx_first_level_10k.md"}
 
 
 def profiled_traffic(workload):
@@ -153,12 +153,15 @@ def run_first_level(args):
     plan64.close()
     plan32, md32, dt32, kern32, _ = timed("fp32", max(2, args.steps // 2), 1)
     plan32.close()
-    plan, md, dt, kern_ms, setup_wall = timed("split_bf16", args.steps, args.warmup)
+    planb, mdb, dtb, kernb, _ = timed("split_bf16", max(2, args.steps // 2), 1)
+    planb.close()
+    plan, md, dt, kern_ms, setup_wall = timed("split_f16", args.steps, args.warmup)
     V = plan.num_voxels
     tm = plan.timing()
     print(f"# first-level study: V={V} T={T} p={p} K={K} set-up {tm.transform_ms:.1f} ms (wall {setup_wall*1e3:.0f} ms); "
-          f"split-BF16 form {dt*1e3:.2f} ms/step (kernel {kern_ms:.2f}), fp32 form {dt32*1e3:.2f} "
-          f"(kernel {kern32:.2f}), fp64 form {dt64*1e3:.2f} ms/step (kernel {kern64:.2f})", file=sys.stderr, flush=True)
+          f"split-FP16 form {dt*1e3:.2f} ms/step (kernel {kern_ms:.2f}), split-BF16 form {dtb*1e3:.2f} (kernel "
+          f"{kernb:.2f}), fp32 form {dt32*1e3:.2f} (kernel {kern32:.2f}), fp64 form {dt64*1e3:.2f} ms/step "
+          f"(kernel {kern64:.2f})", file=sys.stderr, flush=True)
     # per permutation, voxel and time point: the AR(4) recursion (4 FMA),
     # the projections on the p design columns (p FMA), |x|^2 (1 FMA), and
     # sum x when the constant is outside col(X)
@@ -172,24 +175,26 @@ def run_first_level(args):
         "metric": "perm·voxels/s (first-level max-T)", "value": K * V / dt, "unit": "perm·voxels/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16", "accumulate": "fp32", "data": "synthetic",
-        "config": {"workload": WORKLOADS[6], "V_mask": V, "T": T, "p": p, "K": K, "precision": "split_bf16"},
+        "dtype": "f16", "accumulate": "fp32", "data": "synthetic",
+        "config": {"workload": WORKLOADS[6], "V_mask": V, "T": T, "p": p, "K": K, "precision": "split_f16"},
         "setup_ms": tm.transform_ms,
         "phase_breakdown_ms": {"permutation_kernel": kern_ms, "library_call": tm.phase_ms},
         "permutations_repeated_in_fp64": tm.unsafe_waves,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": 4.0 * T * V + 2.0 * K * T + 4.0 * K,
-                     "kernel": "k_fl_perm_bx", "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms,
-                     "note": "the recursion, |x|^2 and the BF16 splits on the vector ALUs (the bound), the "
-                             "projections on the BF16 matrix cores; algorithmic flop over the measured launch "
+                     "kernel": "k_fl_perm_hx", "algorithmic_flop_per_launch": flops, "avg_launch_ms": kern_ms,
+                     "note": "the recursion, |x|^2 and the FP16 splits on the vector ALUs (the bound), the "
+                             "projections on the FP16 matrix cores; algorithmic flop over the measured launch "
                              "time against the FP32 vector peak"},
+        "split_bf16_form": {"value": K * V / dtb, "ms_per_step": dtb * 1e3, "kernel_ms": kernb,
+                            "maxT_equal_to_split_f16_form": int(np.sum(mdb == md))},
         "fp32_form": {"value": K * V / dt32, "ms_per_step": dt32 * 1e3, "kernel_ms": kern32,
-                      "maxT_equal_to_split_bf16_form": int(np.sum(md32 == md))},
+                      "maxT_equal_to_split_f16_form": int(np.sum(md32 == md))},
         "fp64_form": {"value": K * V / dt64, "ms_per_step": dt64 * 1e3, "kernel_ms": kern64,
                       "achieved_tflops": flops / (kern64 * 1e-3) / 1e12,
                       "frac_of_fp64_peak": flops / (kern64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                      "maxT_equal_to_split_bf16_form": int(np.sum(md64 == md))},
+                      "maxT_equal_to_split_f16_form": int(np.sum(md64 == md))},
         "speedup_vs_fp64": dt64 / dt,
     }
     plan.close()

@@ -20,7 +20,8 @@ LIB_PATH = Path(os.environ.get("BPG_LIB", Path(__file__).resolve().with_name("li
 TTEST, FTEST, GROUP_MEAN = 0, 1, 2
 VOXEL, CLUSTER_EXTENT, CLUSTER_MASS, TFCE = 0, 1, 2, 3
 TWOSAMPLE, CORRELATION, MEAN_DESIGN = 0, 1, 2
-PREC_FP32, PREC_SPLIT_BF16, PREC_FP64 = 0, 1, 2
+PREC_FP32, PREC_SPLIT_BF16, PREC_FP64, PREC_SPLIT_F16 = 0, 1, 2, 3
+FL_PRECISIONS = {"split_f16": PREC_SPLIT_F16, "split_bf16": PREC_SPLIT_BF16, "fp32": PREC_FP32, "fp64": PREC_FP64}
 PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16}
 
 OK = 0
@@ -351,9 +352,9 @@ class FlPlan:
 
     study: dict with Y (T, D, H, W) float32, mask (D, H, W), X (T, p), C
     (nc, p), voxel (3 sizes in mm); optional test (TTEST), inference
-    (VOXEL), precision ("split_bf16" or "fp32": FP32 recursion with the
-    projections on the matrix cores and FP64 repeats; "fp64"), cdt,
-    connectivity (26), fwhm (AR smoothing, 8 mm).
+    (VOXEL), precision ("split_f16" (the default), "split_bf16" or "fp32":
+    FP32 recursion with the projections on the matrix cores and FP64
+    repeats; "fp64"), cdt, connectivity (26), fwhm (AR smoothing, 8 mm).
     """
 
     def __init__(self, ctx: Context, study: dict):
@@ -369,9 +370,8 @@ class FlPlan:
         d = FlDesc()
         d.test = s.get("test", TTEST)
         d.inference = s.get("inference", VOXEL)
-        prec = s.get("precision", "split_bf16")
-        d.precision = ({"split_bf16": PREC_SPLIT_BF16, "fp32": PREC_FP32, "fp64": PREC_FP64}[prec]
-                       if isinstance(prec, str) else prec)
+        prec = s.get("precision", "split_f16")
+        d.precision = FL_PRECISIONS[prec] if isinstance(prec, str) else prec
         d.connectivity = s.get("connectivity", 26)
         d.W, d.H, d.D, d.T = W, H, D, T
         d.p, d.nc = s["X"].shape[1], C.shape[0]

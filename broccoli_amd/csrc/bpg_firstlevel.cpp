@@ -37,6 +37,8 @@ struct bpg_fl_plan {
     DBuf<double> Q, A, u;      // model
     DBuf<float> Q32, u32;      // rotated model of the FP32 form ([Tq][16], [16])
     DBuf<unsigned short> Qb;   // rotated model of the split-BF16 form ([Tq32/32][2][64][8])
+    DBuf<unsigned short> Qh;   // rotated model of the split-FP16 form ([2][ng][T4][4][4])
+    DBuf<float> wscale;        // [Vpad] power-of-two scale of each whitened series
     DBuf<unsigned char> flags; // [K] permutations to repeat in FP64
     HBuf<unsigned char> h_flags;
     DBuf<int> d_nbr;
@@ -51,7 +53,7 @@ struct bpg_fl_plan {
         w32.release(); ar.release(); Q.release(); A.release(); u.release(); d_nbr.release(); perm.release();
         partial.release(); maxout.release(); stats.release(); bits.release(); maxsize.release(); cl.release();
         h_perm.release(); h_max.release();
-        Q32.release(); u32.release(); Qb.release(); flags.release(); h_flags.release();
+        Q32.release(); u32.release(); Qb.release(); Qh.release(); wscale.release(); flags.release(); h_flags.release();
     }
 };
 
@@ -152,6 +154,27 @@ void split_basis(const std::vector<double>& rot, int Tq32, std::vector<unsigned 
             }
 }
 
+// The A operands of the split-FP16 form: for time block c and 4-column
+// group g, qh[((c * 2 + hl) * ng + g) * 16 + 4 i + k] is the FP16 hi part
+// (hl = 0) or remainder (hl = 1), rounded to nearest even, of
+// HX_QSCALE * rot[(4c + k) * 16 + 4g + i] (zero beyond T).
+void split_basis_f16(const std::vector<double>& rot, int T, int ng, std::vector<unsigned short>& qh) {
+    const int T4 = (T + 3) / 4;
+    qh.assign((size_t)2 * ng * T4 * 16, 0);
+    for (int g = 0; g < ng; g++)
+        for (int c = 0; c < T4; c++)
+            for (int i = 0; i < 4; i++)
+                for (int k = 0; k < 4; k++) {
+                    const int t = 4 * c + k;
+                    const double q = t < T ? (double)bpg::fl::HX_QSCALE * rot[(size_t)t * 16 + 4 * g + i] : 0.0;
+                    const _Float16 h = (_Float16)q;
+                    const _Float16 l = (_Float16)(q - (double)h);
+                    const size_t o = ((size_t)c * 2 * ng + g) * 16 + 4 * i + k;
+                    std::memcpy(&qh[o], &h, 2);
+                    std::memcpy(&qh[o + (size_t)ng * 16], &l, 2);
+                }
+}
+
 // The device set-up: regression, then three rounds of AR(4) estimation,
 // normalised smoothing, accumulation and whitening (LIB:16363-16452).
 int run_setup(bpg_fl_plan* pl, const float* yc) {
@@ -182,7 +205,7 @@ int run_setup(bpg_fl_plan* pl, const float* yc) {
             !H(g0.ensure(4 * G), "allocation") || !H(g1.ensure(4 * G), "allocation") ||
             !H(smask.ensure(G), "allocation") || !H(m0.ensure(G), "allocation") ||
             !H(c2g.ensure(std::max(Vc, 1)), "allocation") || !H(pl->w32.ensure((size_t)T * Vpad), "allocation") ||
-            !H(pl->ar.ensure(4 * (size_t)Vpad), "allocation"))
+            !H(pl->ar.ensure(4 * (size_t)Vpad), "allocation") || !H(pl->wscale.ensure((size_t)Vpad), "allocation"))
             return;
         std::vector<double> maskd(G);
         for (size_t i = 0; i < G; i++) maskd[i] = (double)d.mask[i];
@@ -224,6 +247,7 @@ int run_setup(bpg_fl_plan* pl, const float* yc) {
                    "whitening"))
                 return;
         }
+        if (!H(bpg::fl::launch_wscale(pl->w32.p, T, Vc, Vpad, pl->wscale.p, st), "series scale")) return;
         if (!H(hipEventRecord(pl->ev[1], st), "event")) return;
         H(hipStreamSynchronize(st), "set-up");
     }();
@@ -240,8 +264,10 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
     *out = nullptr;
     const bpg_fl_desc& d = *desc;
     if (d.test != BPG_TTEST && d.test != BPG_FTEST) BPG_FAIL(ctx, BPG_ERR_ARG, "test must be t or F");
-    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_FP64 && d.precision != BPG_PREC_SPLIT_BF16)
-        BPG_FAIL(ctx, BPG_ERR_ARG, "precision must be BPG_PREC_SPLIT_BF16, BPG_PREC_FP32 or BPG_PREC_FP64");
+    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_FP64 && d.precision != BPG_PREC_SPLIT_BF16 &&
+        d.precision != BPG_PREC_SPLIT_F16)
+        BPG_FAIL(ctx, BPG_ERR_ARG,
+                 "precision must be BPG_PREC_SPLIT_F16, BPG_PREC_SPLIT_BF16, BPG_PREC_FP32 or BPG_PREC_FP64");
     if (d.inference == BPG_TFCE)
         BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED,
                  "TFCE is not available for the first-level permutation test (the reference leaves its "
@@ -319,7 +345,8 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
     }
     if (m.pq <= 16) {
         const int Tq = (d.T + 3) & ~3, Tq32 = (d.T + 31) & ~31;
-        if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16) || pl->Qb.ensure((size_t)Tq32 / 32 * 2 * 64 * 8)) {
+        if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16) || pl->Qb.ensure((size_t)Tq32 / 32 * 2 * 64 * 8) ||
+            pl->Qh.ensure((size_t)2 * 4 * (Tq / 4) * 16)) {
             ctx->err = "device allocation failed";
             return fail(BPG_ERR_NOMEM);
         }
@@ -446,8 +473,10 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     // limits; the FP64 form otherwise and for the repeats
     const bool fast = allow_fast && d.precision != BPG_PREC_FP64 && m.pq <= 16 && T <= bpg::fl::MX_T_MAX &&
                       m.nrows[map] <= BPG_FL_MAX_FROWS;
-    const bool bx = fast && d.precision == BPG_PREC_SPLIT_BF16;
+    const bool hx = fast && d.precision == BPG_PREC_SPLIT_F16 && T <= bpg::fl::HX_T_MAX;
+    const bool bx = fast && !hx && (d.precision == BPG_PREC_SPLIT_BF16 || d.precision == BPG_PREC_SPLIT_F16);
     const bool mx = fast;
+    const int ng = (m.pq + 3) / 4;
     *fast_used = mx;
     const std::vector<double>& Arows = m.A[map];
     BPG_HIP(ctx, hipMemcpyAsync(pl->A.p, Arows.data(), Arows.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -457,7 +486,11 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     if (mx) {
         const int Tq = (T + 3) & ~3, Tq32 = (T + 31) & ~31;
         rotated_basis(m, map, Tq32, rot, u32);
-        if (bx) {
+        if (hx) {
+            split_basis_f16(rot, T, ng, qb);
+            BPG_HIP(ctx, hipMemcpyAsync(pl->Qh.p, qb.data(), qb.size() * sizeof(unsigned short), hipMemcpyHostToDevice,
+                                        st));
+        } else if (bx) {
             split_basis(rot, Tq32, qb);
             BPG_HIP(ctx, hipMemcpyAsync(pl->Qb.p, qb.data(), qb.size() * sizeof(unsigned short), hipMemcpyHostToDevice,
                                         st));
@@ -491,11 +524,18 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     if (mx) {
         a.Q32 = pl->Q32.p;
         a.Qb = reinterpret_cast<const unsigned*>(pl->Qb.p);
+        a.Qh = reinterpret_cast<const unsigned*>(pl->Qh.p);
+        a.wscale = pl->wscale.p;
+        a.neg_one = -1.0f;
+        a.ng = ng;
         a.u32 = pl->u32.p;
         a.flags = pl->flags.p;
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
-    BPG_HIP(ctx, bx ? bpg::fl::launch_perm_bx(a, st) : mx ? bpg::fl::launch_perm_mx(a, st) : bpg::fl::launch_perm(a, st));
+    BPG_HIP(ctx, hx   ? bpg::fl::launch_perm_hx(a, st)
+                 : bx ? bpg::fl::launch_perm_bx(a, st)
+                 : mx ? bpg::fl::launch_perm_mx(a, st)
+                      : bpg::fl::launch_perm(a, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
     if (mx) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
     if (cluster) {
@@ -539,7 +579,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = d.p;
     tm.slots_per_perm = mx ? 16 : m.pq;
-    tm.precision_used = bx ? BPG_PREC_SPLIT_BF16 : mx ? BPG_PREC_FP32 : BPG_PREC_FP64;
+    tm.precision_used = hx ? BPG_PREC_SPLIT_F16 : bx ? BPG_PREC_SPLIT_BF16 : mx ? BPG_PREC_FP32 : BPG_PREC_FP64;
     tm.fp32_permutations = mx ? K : 0;
     return BPG_OK;
 }

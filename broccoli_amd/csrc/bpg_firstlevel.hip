@@ -11,6 +11,7 @@
 // permuted volumes anywhere.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 
 #include "bpg_device.h"
@@ -248,6 +249,27 @@ __global__ __launch_bounds__(256) void k_whiten(const double* __restrict__ e, co
 hipError_t launch_whiten(const double* e, const double* ar, int T, int Vc, int Vpad, double* wh, float* w32,
                          hipStream_t s) {
     hipLaunchKernelGGL(k_whiten, dim3((Vpad + 255) / 256), dim3(256), 0, s, e, ar, T, Vc, Vpad, wh, w32);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_wscale(const float* __restrict__ w32, int T, int Vc, int Vpad,
+                                                float* __restrict__ wscale) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Vpad) return;
+    float mx = 0.0f;
+    if (v < Vc)
+        for (int t = 0; t < T; t++) mx = fmaxf(mx, fabsf(w32[(size_t)t * Vpad + v]));
+    float sc = 1.0f;
+    if (mx > 0.0f && mx <= FLT_MAX) {
+        int e;
+        (void)frexpf(mx, &e);
+        sc = ldexpf(1.0f, 9 - max(e, -100));
+    }
+    wscale[v] = sc;
+}
+
+hipError_t launch_wscale(const float* w32, int T, int Vc, int Vpad, float* wscale, hipStream_t s) {
+    hipLaunchKernelGGL(k_wscale, dim3((Vpad + 255) / 256), dim3(256), 0, s, w32, T, Vc, Vpad, wscale);
     return hipGetLastError();
 }
 
@@ -841,27 +863,234 @@ __global__ __launch_bounds__(512) void k_fl_perm_bx(PermArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// the split-FP16 form: v_mfma_f32_4x4x4_16b_f16 is 16 independent 4x4x4
+// products; in block b = lane / 4, lane l supplies row l % 4 of A and
+// column l % 4 of B (4 values along k each) and receives column l % 4 of D
+// (rows in its 4 result registers).  With A = 4 basis columns over 4 time
+// points (the same in every block) and B = 4 time points of each lane's own
+// x, lane l receives z_{4g + i}, i = 0..3, of its own voxel: no lane
+// exchanges before or after.  Per 4 time points a lane splits its x into
+// FP16 hi + lo (two instructions per value), and 3 products (hi.hi, hi.lo,
+// lo.hi) per group of 4 columns accumulate z.  Each voxel's series is scaled
+// by a power of two (exact, and the statistic is scale-free) so that its
+// values sit well inside the FP16 range; a sample with |x|^2 >= 4e9 may
+// have left it and is flagged, as is one with SSE < |x|^2 / 32.
+// ---------------------------------------------------------------------------
+
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+
+// x -> FP16 hi (rounded toward zero) and the remainder, rounded toward zero:
+// |x - hi - lo| < 2^-20 |x| in the FP16 normal range.  m1 is -1 held in a
+// register, so that x - hi is one v_fma_mix_f32.
+__device__ __forceinline__ void split4_f16(const float (&x)[4], float m1, f16x4& hi, f16x4& lo) {
+    const auto h01 = __builtin_amdgcn_cvt_pkrtz(x[0], x[1]);
+    const auto h23 = __builtin_amdgcn_cvt_pkrtz(x[2], x[3]);
+    const auto l01 = __builtin_amdgcn_cvt_pkrtz(__builtin_fmaf((float)h01[0], m1, x[0]),
+                                                __builtin_fmaf((float)h01[1], m1, x[1]));
+    const auto l23 = __builtin_amdgcn_cvt_pkrtz(__builtin_fmaf((float)h23[0], m1, x[2]),
+                                                __builtin_fmaf((float)h23[1], m1, x[3]));
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    hi = __builtin_bit_cast(f16x4, u32x2{__builtin_bit_cast(unsigned, h01), __builtin_bit_cast(unsigned, h23)});
+    lo = __builtin_bit_cast(f16x4, u32x2{__builtin_bit_cast(unsigned, l01), __builtin_bit_cast(unsigned, l23)});
+}
+
+template <bool MEAN, int NG>
+__global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
+    extern __shared__ float tile[];  // [T][64] scaled whitened tile, then the [T4][2][NG][4] f16x4 basis
+    __shared__ float smax[WAVES * ROUNDS * NP];
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const int lid = xcd_logical_block((int)(gridDim.x >> 3));
+    if (lid >= n_blocks) return;
+    const int vchunk = lid / a.n_pgroups;
+    const int pgroup = lid - vchunk * a.n_pgroups;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile0 = vchunk * a.tiles_per_chunk;
+    const int tile1 = min(tile0 + a.tiles_per_chunk, a.n_vtiles);
+    const int T4 = (a.T + 3) >> 2;
+    f16x4* qs = reinterpret_cast<f16x4*>(tile + (size_t)a.T * TILE);
+    {
+        // the basis, once per block (the first barrier below orders it)
+        const int n = 2 * NG * T4 * 4 * 2;
+        unsigned* dst = reinterpret_cast<unsigned*>(qs);
+        for (int i = threadIdx.x; i < n; i += WAVES * 64) dst[i] = a.Qh[i];
+    }
+    const float m1 = a.neg_one;
+    const float invT = 1.0f / (float)a.T, invTm1 = 1.0f / ((float)a.T - 1.0f);
+    constexpr float iq = 1.0f / HX_QSCALE, iq2 = 1.0f / (HX_QSCALE * HX_QSCALE);
+    float* my = smax + wave * (ROUNDS * NP);
+    if (lane == 0)
+        for (int i = 0; i < ROUNDS * NP; i++) my[i] = -INFINITY;
+
+    for (int tl = tile0; tl < tile1; tl++) {
+        const int v = tl * TILE + lane;
+        __syncthreads();
+        {
+            // every thread stages values of voxel (threadIdx.x & 63) only
+            const float sc = a.wscale[(size_t)tl * TILE + (threadIdx.x & 63)];
+            const int n = a.T * TILE;
+            for (int i = threadIdx.x; i < n; i += WAVES * 64)
+                tile[i] = a.w32[(size_t)(i >> 6) * a.Vpad + (size_t)tl * TILE + (i & 63)] * sc;
+        }
+        __syncthreads();
+        const float a1 = (float)a.ar[v], a2 = (float)a.ar[(size_t)a.Vpad + v];
+        const float a3 = (float)a.ar[2 * (size_t)a.Vpad + v], a4 = (float)a.ar[3 * (size_t)a.Vpad + v];
+        const bool in_mask = v < a.Vc;
+        for (int r = 0; r < ROUNDS; r++) {
+            const int kbase = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP;
+            if (kbase >= a.K) break;
+            cuint_p prow[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = min(kbase + j, a.K - 1);
+                prow[j] = (cuint_p)(a.perm + (size_t)k * a.Tp);
+            }
+            f32x4 acc[NP][NG];
+            float h[NP][4], s[NP], m[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+#pragma unroll
+                for (int g = 0; g < NG; g++) acc[j][g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int q = 0; q < 4; q++) h[j][q] = 0.0f;
+                s[j] = 0.0f;
+                m[j] = 0.0f;
+            }
+            // time points 4c .. 4c + n - 1
+            auto chunk = [&](int c, int n) {
+                f16x4 qh[NG], ql[NG];
+#pragma unroll
+                for (int g = 0; g < NG; g++) {
+                    qh[g] = qs[((size_t)c * 2 * NG + g) * 4 + (lane & 3)];
+                    ql[g] = qs[((size_t)c * 2 * NG + NG + g) * 4 + (lane & 3)];
+                }
+#pragma unroll
+                for (int j = 0; j < NP; j++) {
+                    const unsigned pk[2] = {prow[j][2 * c], prow[j][2 * c + 1]};
+                    float xs[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if (e < n) {
+                            const unsigned idx = (pk[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu;
+                            float x = tile[idx * TILE + lane];
+                            x = fmaf(a4, h[j][3], x);
+                            x = fmaf(a3, h[j][2], x);
+                            x = fmaf(a2, h[j][1], x);
+                            x = fmaf(a1, h[j][0], x);
+                            h[j][3] = h[j][2];
+                            h[j][2] = h[j][1];
+                            h[j][1] = h[j][0];
+                            h[j][0] = x;
+                            s[j] = fmaf(x, x, s[j]);
+                            if (MEAN) m[j] += x;
+                            xs[e] = x;
+                        } else {
+                            xs[e] = 0.0f;
+                        }
+                    }
+                    f16x4 xh, xl;
+                    split4_f16(xs, m1, xh, xl);
+#pragma unroll
+                    for (int g = 0; g < NG; g++) {
+                        acc[j][g] = __builtin_amdgcn_mfma_f32_4x4x4f16(qh[g], xh, acc[j][g], 0, 0, 0);
+                        acc[j][g] = __builtin_amdgcn_mfma_f32_4x4x4f16(qh[g], xl, acc[j][g], 0, 0, 0);
+                        acc[j][g] = __builtin_amdgcn_mfma_f32_4x4x4f16(ql[g], xh, acc[j][g], 0, 0, 0);
+                    }
+                }
+            };
+            // two chunks per trip: the AR history stays in place between them
+            const int T4f = a.T >> 2;
+            int c = 0;
+            for (; c + 1 < T4f; c += 2) {
+                chunk(c, 4);
+                chunk(c + 1, 4);
+            }
+            if (c < T4f) chunk(c, 4);
+            if (T4f < T4) chunk(T4f, a.T - 4 * T4f);
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const int k = kbase + j;
+                const bool valid = k < a.K;
+                float ss = 0.0f, fn = 0.0f, uz = 0.0f;
+#pragma unroll
+                for (int g = 0; g < NG; g++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int col = 4 * g + i;
+                        const float z = acc[j][g][i];
+                        ss = fmaf(z, z, ss);
+                        if (col < a.nrows) fn = fmaf(z, z, fn);
+                        if (MEAN) uz = fmaf(a.u32[col], z, uz);
+                    }
+                const float sse = s[j] - ss * iq2;
+                const float me = MEAN ? m[j] - uz * iq : 0.0f;
+                const float var = (sse - me * me * invT) * invTm1;
+                const float st = a.ftest ? fn * iq2 / (var * (float)a.nrows) : acc[j][0][0] * iq / sqrtf(var);
+                const bool act = in_mask && valid;
+                const bool bad = act && !(sse >= s[j] * 0.03125f && s[j] < 4.0e9f);
+                if (__ballot(bad) && lane == 0) a.flags[k] = 1;
+                if (a.stats && act) a.stats[(size_t)k * a.Vc + v] = st;
+                if (a.bits) {
+                    const unsigned long long bal = __ballot(act && st > a.cdt);
+                    if (lane == 0 && valid) {
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl] = (unsigned)bal;
+                        a.bits[(size_t)k * a.nwords + 2 * (size_t)tl + 1] = (unsigned)(bal >> 32);
+                    }
+                }
+                const float mx = wave_max(act ? st : -INFINITY);
+                if (lane == 0) my[r * NP + j] = fmaxf(my[r * NP + j], mx);
+            }
+        }
+    }
+    if (lane == 0) {
+        for (int r = 0; r < ROUNDS; r++)
+            for (int j = 0; j < NP; j++) {
+                const int k = pgroup * PERMS_PER_BLOCK + r * (WAVES * NP) + wave * NP + j;
+                if (k < a.K) a.partial[(size_t)vchunk * a.K + k] = my[r * NP + j];
+            }
+    }
+}
+
+// Launch with `lds` bytes of dynamic LDS; the limit is raised on every call
+// (it is a property of the kernel on the current device).
+template <typename F>
+hipError_t launch_lds(F* kern, unsigned grid, size_t lds, const PermArgs& a, hipStream_t s) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WAVES * 64), lds, s, a);
+    return hipGetLastError();
+}
+
+template <bool MEAN>
+hipError_t launch_hx_m(const PermArgs& a, unsigned grid, size_t lds, hipStream_t s) {
+    switch (a.ng) {
+        case 1: return launch_lds(&k_fl_perm_hx<MEAN, 1>, grid, lds, a, s);
+        case 2: return launch_lds(&k_fl_perm_hx<MEAN, 2>, grid, lds, a, s);
+        case 3: return launch_lds(&k_fl_perm_hx<MEAN, 3>, grid, lds, a, s);
+        case 4: return launch_lds(&k_fl_perm_hx<MEAN, 4>, grid, lds, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_perm_hx(PermArgs a, hipStream_t s) {
+    if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
+    if (a.T > HX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS || a.ng != (a.pq + 3) / 4 || a.Tp < a.T)
+        return hipErrorInvalidValue;
+    const int n_blocks = a.n_vchunks * a.n_pgroups;
+    const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
+    const size_t lds = (size_t)a.T * TILE * sizeof(float) + (size_t)2 * a.ng * ((a.T + 3) / 4) * 4 * sizeof(f16x4);
+    return a.mean ? launch_hx_m<true>(a, grid, lds, s) : launch_hx_m<false>(a, grid, lds, s);
+}
+
 hipError_t launch_perm_bx(PermArgs a, hipStream_t s) {
     if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
     if (a.T > MX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS) return hipErrorInvalidValue;
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
     const size_t lds = (size_t)a.T * TILE * sizeof(float);
-    static bool attr[2] = {false, false};
-    const int mi = a.mean ? 1 : 0;
-    if (!attr[mi]) {
-        const void* f = a.mean ? reinterpret_cast<const void*>(&k_fl_perm_bx<true>)
-                               : reinterpret_cast<const void*>(&k_fl_perm_bx<false>);
-        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)(MX_T_MAX * TILE * sizeof(float)));
-        if (e != hipSuccess) return e;
-        attr[mi] = true;
-    }
-    if (a.mean)
-        hipLaunchKernelGGL((k_fl_perm_bx<true>), dim3(grid), dim3(WAVES * 64), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_fl_perm_bx<false>), dim3(grid), dim3(WAVES * 64), lds, s, a);
-    return hipGetLastError();
+    return a.mean ? launch_lds(&k_fl_perm_bx<true>, grid, lds, a, s) : launch_lds(&k_fl_perm_bx<false>, grid, lds, a, s);
 }
 
 hipError_t launch_perm_mx(PermArgs a, hipStream_t s) {
@@ -870,21 +1099,7 @@ hipError_t launch_perm_mx(PermArgs a, hipStream_t s) {
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
     const size_t lds = (size_t)a.T * TILE * sizeof(float);
-    static bool attr[2] = {false, false};
-    const int mi = a.mean ? 1 : 0;
-    if (!attr[mi]) {
-        const void* f = a.mean ? reinterpret_cast<const void*>(&k_fl_perm_mx<true>)
-                               : reinterpret_cast<const void*>(&k_fl_perm_mx<false>);
-        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)(MX_T_MAX * TILE * sizeof(float)));
-        if (e != hipSuccess) return e;
-        attr[mi] = true;
-    }
-    if (a.mean)
-        hipLaunchKernelGGL((k_fl_perm_mx<true>), dim3(grid), dim3(WAVES * 64), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_fl_perm_mx<false>), dim3(grid), dim3(WAVES * 64), lds, s, a);
-    return hipGetLastError();
+    return a.mean ? launch_lds(&k_fl_perm_mx<true>, grid, lds, a, s) : launch_lds(&k_fl_perm_mx<false>, grid, lds, a, s);
 }
 
 int fl_width(int p) {
@@ -899,16 +1114,7 @@ hipError_t launch_perm_m(const PermArgs& a, hipStream_t s) {
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
     if (a.T <= LDS_T_MAX) {
-        const size_t lds = (size_t)a.T * TILE * sizeof(float);
-        static bool attr = false;
-        if (!attr) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fl_perm<PQ, true, MEAN>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                     (int)(LDS_T_MAX * TILE * sizeof(float)));
-            if (e != hipSuccess) return e;
-            attr = true;
-        }
-        hipLaunchKernelGGL((k_fl_perm<PQ, true, MEAN>), dim3(grid), dim3(WAVES * 64), lds, s, a);
+        return launch_lds(&k_fl_perm<PQ, true, MEAN>, grid, (size_t)a.T * TILE * sizeof(float), a, s);
     } else {
         hipLaunchKernelGGL((k_fl_perm<PQ, false, MEAN>), dim3(grid), dim3(WAVES * 64), 0, s, a);
     }

@@ -54,6 +54,7 @@ extern "C" {
 #define BPG_PREC_FP32 0       /* FP32 VALU contraction                         */
 #define BPG_PREC_SPLIT_BF16 1 /* hi/lo BF16 MFMA with FP32 accumulation        */
 #define BPG_PREC_FP64 2       /* FP64 VALU (the first-level permutation kernel) */
+#define BPG_PREC_SPLIT_F16 3  /* hi/lo FP16 MFMA, first-level plans only       */
 
 /* limits of the reference (RGL:630-636 p <= 25, RGL:726-732 F rows <= 10) */
 #define BPG_MAX_REGRESSORS 25
@@ -224,10 +225,14 @@ typedef struct {
                                arithmetic) with the projections on the matrix
                                cores, ill-conditioned permutations repeated in
                                FP64 (p <= 16, T <= 560; other studies run
-                               FP64): BPG_PREC_SPLIT_BF16 (BF16 hi/lo splits,
-                               samples with SSE < |x|^2/8 repeated) or
-                               BPG_PREC_FP32 (FP32 MFMA, SSE < |x|^2/64);
-                               BPG_PREC_FP64: everything in FP64             */
+                               FP64): BPG_PREC_SPLIT_F16 (FP16 hi/lo splits of
+                               power-of-two scaled series, samples with
+                               SSE < |x|^2/32 repeated; T <= 480, the
+                               split-BF16 form above that),
+                               BPG_PREC_SPLIT_BF16 (BF16 hi/lo splits,
+                               SSE < |x|^2/8) or BPG_PREC_FP32 (FP32 MFMA,
+                               SSE < |x|^2/64); BPG_PREC_FP64: everything in
+                               FP64                                          */
     int connectivity;       /* 26 (the reference's) or 6                      */
     int W, H, D, T;         /* EPI grid and time points (10 <= T <= 65535)    */
     int p;                  /* regressors, <= BPG_MAX_REGRESSORS              */

@@ -62,10 +62,12 @@ def test_setup_matches_the_oracle(ctx):
     np.testing.assert_allclose(wh_g[:, m], wh[:, m], rtol=1e-6, atol=1e-5)
 
 
-PREC = {"split_bf16": capi.PREC_SPLIT_BF16, "fp32": capi.PREC_FP32, "fp64": capi.PREC_FP64}
+PREC = {"split_f16": capi.PREC_SPLIT_F16, "split_bf16": capi.PREC_SPLIT_BF16, "fp32": capi.PREC_FP32,
+        "fp64": capi.PREC_FP64}
+ALL = list(PREC)
 
 
-@pytest.mark.parametrize("prec", ["split_bf16", "fp32", "fp64"])
+@pytest.mark.parametrize("prec", ALL)
 @pytest.mark.parametrize("seed,T,p", [(2, 64, 5), (3, 120, 8), (4, 48, 1), (14, 242, 12)])
 def test_permuted_t_statistics_match_fp64(ctx, seed, T, p, prec):
     st = fl_study(seed=seed, T=T, p=p, ar_gradient=0.2)
@@ -86,7 +88,7 @@ def test_permuted_t_statistics_match_fp64(ctx, seed, T, p, prec):
     np.testing.assert_allclose(md, md_ref, atol=2e-4, rtol=1e-5)
 
 
-@pytest.mark.parametrize("prec", ["split_bf16", "fp32", "fp64"])
+@pytest.mark.parametrize("prec", ALL)
 def test_design_without_an_intercept(ctx, prec):
     # the residuals need not sum to zero: Welford's mean term is kept
     st = fl_study(seed=12, T=61, p=5)
@@ -103,7 +105,7 @@ def test_design_without_an_intercept(ctx, prec):
     check_max(md, stats)
 
 
-@pytest.mark.parametrize("prec", ["split_bf16", "fp32", "fp64"])
+@pytest.mark.parametrize("prec", ALL)
 def test_f_statistics_and_second_contrast(ctx, prec):
     st = fl_study(seed=5, T=80, p=6)
     st["precision"] = prec
@@ -166,7 +168,9 @@ def test_wide_designs(ctx, p):
     check_max(md, stats)
 
 
-@pytest.mark.parametrize("inference,conn,prec", [(capi.CLUSTER_EXTENT, 26, "split_bf16"),
+@pytest.mark.parametrize("inference,conn,prec", [(capi.CLUSTER_EXTENT, 26, "split_f16"),
+                                                 (capi.CLUSTER_MASS, 6, "split_f16"),
+                                                 (capi.CLUSTER_EXTENT, 26, "split_bf16"),
                                                  (capi.CLUSTER_MASS, 26, "split_bf16"),
                                                  (capi.CLUSTER_EXTENT, 26, "fp32"), (capi.CLUSTER_MASS, 26, "fp32"),
                                                  (capi.CLUSTER_EXTENT, 6, "fp64"), (capi.CLUSTER_MASS, 26, "fp64")])
@@ -238,7 +242,7 @@ def test_empty_and_single_voxel_masks(ctx, inference):
     check_stats(stats, stats_ref, one == 1.0)
 
 
-@pytest.mark.parametrize("prec", ["split_bf16", "fp32"])
+@pytest.mark.parametrize("prec", ["split_f16", "split_bf16", "fp32"])
 def test_ill_conditioned_permutations_repeat_in_fp64(ctx, prec):
     # 16 regressors on 20 time points: some samples keep less than 1/64 of
     # |x|^2 as residual, which the fast forms hand to the FP64 form
@@ -257,7 +261,7 @@ def test_ill_conditioned_permutations_repeat_in_fp64(ctx, prec):
     check_max(md, stats)
 
 
-@pytest.mark.parametrize("prec", ["split_bf16", "fp32"])
+@pytest.mark.parametrize("prec", ["split_f16", "split_bf16", "fp32"])
 def test_fast_forms_fall_back_to_fp64_outside_their_limits(ctx, prec):
     for kw in ({"T": 90, "p": 25}, {"W": 7, "H": 6, "D": 5, "T": 600, "p": 4}):
         st = fl_study(seed=16, **kw)
@@ -267,3 +271,37 @@ def test_fast_forms_fall_back_to_fp64_outside_their_limits(ctx, prec):
         with capi.FlPlan(ctx, st) as plan:
             plan.permutation_phase(0, perm)
             assert plan.timing().precision_used == capi.PREC_FP64
+
+
+def test_split_f16_hands_long_series_to_the_split_bf16_form(ctx):
+    # 480 < T <= 560: the FP16 basis no longer fits in LDS beside the tile
+    st = fl_study(seed=17, W=7, H=6, D=5, T=500, p=6)
+    K = 16
+    perm = capi.gen_perm_first_level(500, K, seed=3)
+    print("first-level split_f16 at T=500", flush=True)
+    _, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+        assert plan.timing().precision_used == capi.PREC_SPLIT_BF16
+    check_stats(stats, stats_ref, in_mask(st))
+    check_max(md, stats)
+
+
+@pytest.mark.parametrize("scale", [1e-7, 3e7])
+@pytest.mark.parametrize("prec", ["split_f16", "split_bf16"])
+def test_statistics_do_not_depend_on_the_data_scale(ctx, scale, prec):
+    # the split-FP16 form rescales each series by a power of two; the
+    # statistic is scale-free, so tiny and huge intensities give the same
+    # maps as the oracle (strong AR(1): a wide dynamic range in x)
+    st = fl_study(seed=18, T=96, p=6, ar=(0.85, 0.05, 0.0, 0.0))
+    st["Y"] = (st["Y"].astype(np.float64) * scale).astype(np.float32)
+    st["precision"] = prec
+    K = 24
+    perm = capi.gen_perm_first_level(96, K, seed=11)
+    print(f"first-level data scale {scale} {prec}", flush=True)
+    _, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+        assert plan.timing().precision_used == PREC[prec]
+    check_stats(stats, stats_ref, in_mask(st))
+    check_max(md, stats)
