@@ -37,7 +37,7 @@ struct bpg_fl_plan {
     DBuf<double> Q, A, u;      // model
     DBuf<float> Q32, u32;      // rotated model of the FP32 form ([Tq][16], [16])
     DBuf<unsigned short> Qb;   // rotated model of the split-BF16 form ([Tq32/32][2][64][8])
-    DBuf<unsigned short> Qh;   // rotated model of the split-FP16 form ([2][ng][T4][4][4])
+    DBuf<unsigned short> Qh;   // rotated model of the split-FP16 form ([T8][2][16][8])
     DBuf<float> wscale;        // [Vpad] power-of-two scale of each whitened series
     DBuf<unsigned char> flags; // [K] permutations to repeat in FP64
     HBuf<unsigned char> h_flags;
@@ -154,25 +154,24 @@ void split_basis(const std::vector<double>& rot, int Tq32, std::vector<unsigned 
             }
 }
 
-// The A operands of the split-FP16 form: for time block c and 4-column
-// group g, qh[((c * 2 + hl) * ng + g) * 16 + 4 i + k] is the FP16 hi part
+// The A operands of the split-FP16 form: for time block c (8
+// points), qh[((c * 2 + hl) * 16 + col) * 8 + k] is the FP16 hi part
 // (hl = 0) or remainder (hl = 1), rounded to nearest even, of
-// HX_QSCALE * rot[(4c + k) * 16 + 4g + i] (zero beyond T).
-void split_basis_f16(const std::vector<double>& rot, int T, int ng, std::vector<unsigned short>& qh) {
-    const int T4 = (T + 3) / 4;
-    qh.assign((size_t)2 * ng * T4 * 16, 0);
-    for (int g = 0; g < ng; g++)
-        for (int c = 0; c < T4; c++)
-            for (int i = 0; i < 4; i++)
-                for (int k = 0; k < 4; k++) {
-                    const int t = 4 * c + k;
-                    const double q = t < T ? (double)bpg::fl::HX_QSCALE * rot[(size_t)t * 16 + 4 * g + i] : 0.0;
-                    const _Float16 h = (_Float16)q;
-                    const _Float16 l = (_Float16)(q - (double)h);
-                    const size_t o = ((size_t)c * 2 * ng + g) * 16 + 4 * i + k;
-                    std::memcpy(&qh[o], &h, 2);
-                    std::memcpy(&qh[o + (size_t)ng * 16], &l, 2);
-                }
+// HX_QSCALE * rot[(8c + k) * 16 + col] (zero beyond T).
+void split_basis_f16(const std::vector<double>& rot, int T, std::vector<unsigned short>& qh) {
+    const int T8 = (T + 7) / 8;
+    qh.assign((size_t)T8 * 2 * 16 * 8, 0);
+    for (int c = 0; c < T8; c++)
+        for (int col = 0; col < 16; col++)
+            for (int k = 0; k < 8; k++) {
+                const int t = 8 * c + k;
+                const double q = t < T ? (double)bpg::fl::HX_QSCALE * rot[(size_t)t * 16 + col] : 0.0;
+                const _Float16 h = (_Float16)q;
+                const _Float16 l = (_Float16)(q - (double)h);
+                const size_t o = ((size_t)c * 2 * 16 + col) * 8 + k;
+                std::memcpy(&qh[o], &h, 2);
+                std::memcpy(&qh[o + 16 * 8], &l, 2);
+            }
 }
 
 // The device set-up: regression, then three rounds of AR(4) estimation,
@@ -346,7 +345,7 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
     if (m.pq <= 16) {
         const int Tq = (d.T + 3) & ~3, Tq32 = (d.T + 31) & ~31;
         if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16) || pl->Qb.ensure((size_t)Tq32 / 32 * 2 * 64 * 8) ||
-            pl->Qh.ensure((size_t)2 * 4 * (Tq / 4) * 16)) {
+            pl->Qh.ensure((size_t)(Tq32 / 8) * 2 * 16 * 8)) {
             ctx->err = "device allocation failed";
             return fail(BPG_ERR_NOMEM);
         }
@@ -476,7 +475,6 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     const bool hx = fast && d.precision == BPG_PREC_SPLIT_F16 && T <= bpg::fl::HX_T_MAX;
     const bool bx = fast && !hx && (d.precision == BPG_PREC_SPLIT_BF16 || d.precision == BPG_PREC_SPLIT_F16);
     const bool mx = fast;
-    const int ng = (m.pq + 3) / 4;
     *fast_used = mx;
     const std::vector<double>& Arows = m.A[map];
     BPG_HIP(ctx, hipMemcpyAsync(pl->A.p, Arows.data(), Arows.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -487,7 +485,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
         const int Tq = (T + 3) & ~3, Tq32 = (T + 31) & ~31;
         rotated_basis(m, map, Tq32, rot, u32);
         if (hx) {
-            split_basis_f16(rot, T, ng, qb);
+            split_basis_f16(rot, T, qb);
             BPG_HIP(ctx, hipMemcpyAsync(pl->Qh.p, qb.data(), qb.size() * sizeof(unsigned short), hipMemcpyHostToDevice,
                                         st));
         } else if (bx) {
@@ -527,7 +525,6 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
         a.Qh = reinterpret_cast<const unsigned*>(pl->Qh.p);
         a.wscale = pl->wscale.p;
         a.neg_one = -1.0f;
-        a.ng = ng;
         a.u32 = pl->u32.p;
         a.flags = pl->flags.p;
     }

@@ -864,40 +864,52 @@ __global__ __launch_bounds__(512) void k_fl_perm_bx(PermArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// the split-FP16 form: v_mfma_f32_4x4x4_16b_f16 is 16 independent 4x4x4
-// products; in block b = lane / 4, lane l supplies row l % 4 of A and
-// column l % 4 of B (4 values along k each) and receives column l % 4 of D
-// (rows in its 4 result registers).  With A = 4 basis columns over 4 time
-// points (the same in every block) and B = 4 time points of each lane's own
-// x, lane l receives z_{4g + i}, i = 0..3, of its own voxel: no lane
-// exchanges before or after.  Per 4 time points a lane splits its x into
-// FP16 hi + lo (two instructions per value), and 3 products (hi.hi, hi.lo,
-// lo.hi) per group of 4 columns accumulate z.  Each voxel's series is scaled
-// by a power of two (exact, and the statistic is scale-free) so that its
-// values sit well inside the FP16 range; a sample with |x|^2 >= 4e9 may
-// have left it and is flagged, as is one with SSE < |x|^2 / 32.
+// the split-FP16 form: the FP32 recursion of the other forms, the
+// projections on v_mfma_f32_32x32x16_f16 from FP16 hi/lo splits (two
+// instructions per value), 8 time points per product.  Each voxel's series
+// is scaled by a power of two (exact, and the statistic is scale-free) so
+// that its values sit well inside the FP16 range; a sample with
+// |x|^2 >= 4e9 may have left it and is flagged, as is one with
+// SSE < |x|^2 / 32.
 // ---------------------------------------------------------------------------
 
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+// Lane l supplies B column l % 32 for k-half l / 32 (its own x at 8 time
+// points), so a column mixes the series of lanes j and j + 32.  A is
+// block-diagonal: rows 0..15 (the basis columns for lane j's series) read
+// k 0..7 only, rows 16..31 (for lane j + 32's) k 8..15 only, so D rows
+// 0..15 are z of lane j's voxel and rows 16..31 z of lane j + 32's.  Row i
+// of D lands in lane half (i / 4) % 2, register i % 4 + 4 (i / 8) (lane
+// layouts probed on the device).  The A operand of a lane is 8 basis values
+// (its column i % 16) or zeros: lanes with i < 16 in the upper half, or
+// i >= 16 in the lower, read a zero block and do not advance.  Half of
+// each product is zeros; 3 products per 8 time points and permutation
+// keep the matrix pipe at a fraction of the vector work.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // x -> FP16 hi (rounded toward zero) and the remainder, rounded toward zero:
 // |x - hi - lo| < 2^-20 |x| in the FP16 normal range.  m1 is -1 held in a
-// register, so that x - hi is one v_fma_mix_f32.
-__device__ __forceinline__ void split4_f16(const float (&x)[4], float m1, f16x4& hi, f16x4& lo) {
-    const auto h01 = __builtin_amdgcn_cvt_pkrtz(x[0], x[1]);
-    const auto h23 = __builtin_amdgcn_cvt_pkrtz(x[2], x[3]);
-    const auto l01 = __builtin_amdgcn_cvt_pkrtz(__builtin_fmaf((float)h01[0], m1, x[0]),
-                                                __builtin_fmaf((float)h01[1], m1, x[1]));
-    const auto l23 = __builtin_amdgcn_cvt_pkrtz(__builtin_fmaf((float)h23[0], m1, x[2]),
-                                                __builtin_fmaf((float)h23[1], m1, x[3]));
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-    hi = __builtin_bit_cast(f16x4, u32x2{__builtin_bit_cast(unsigned, h01), __builtin_bit_cast(unsigned, h23)});
-    lo = __builtin_bit_cast(f16x4, u32x2{__builtin_bit_cast(unsigned, l01), __builtin_bit_cast(unsigned, l23)});
+// register, so that x - hi is one v_fma_mix_f32 reading the FP16 half.
+__device__ __forceinline__ void split8_f16(const float (&x)[8], float m1, f16x8& hi, f16x8& lo) {
+    unsigned h[4], l[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const auto hp = __builtin_amdgcn_cvt_pkrtz(x[2 * q], x[2 * q + 1]);
+        const auto lp = __builtin_amdgcn_cvt_pkrtz(__builtin_fmaf((float)hp[0], m1, x[2 * q]),
+                                                   __builtin_fmaf((float)hp[1], m1, x[2 * q + 1]));
+        h[q] = __builtin_bit_cast(unsigned, hp);
+        l[q] = __builtin_bit_cast(unsigned, lp);
+    }
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    hi = __builtin_bit_cast(f16x8, u32x4{h[0], h[1], h[2], h[3]});
+    lo = __builtin_bit_cast(f16x8, u32x4{l[0], l[1], l[2], l[3]});
 }
 
-template <bool MEAN, int NG>
+template <bool MEAN>
 __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
-    extern __shared__ float tile[];  // [T][64] scaled whitened tile, then the [T4][2][NG][4] f16x4 basis
+    // [T][64] scaled whitened tile, then 512 zero bytes, then the basis:
+    // [T8][hi, lo][16 columns][8 time points] FP16
+    extern __shared__ float tile[];
     __shared__ float smax[WAVES * ROUNDS * NP];
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const int lid = xcd_logical_block((int)(gridDim.x >> 3));
@@ -908,17 +920,23 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int tile0 = vchunk * a.tiles_per_chunk;
     const int tile1 = min(tile0 + a.tiles_per_chunk, a.n_vtiles);
-    const int T4 = (a.T + 3) >> 2;
-    f16x4* qs = reinterpret_cast<f16x4*>(tile + (size_t)a.T * TILE);
+    const int T8 = (a.T + 7) >> 3;
+    char* qbase = reinterpret_cast<char*>(tile + (size_t)a.T * TILE);
     {
-        // the basis, once per block (the first barrier below orders it)
-        const int n = 2 * NG * T4 * 4 * 2;
-        unsigned* dst = reinterpret_cast<unsigned*>(qs);
-        for (int i = threadIdx.x; i < n; i += WAVES * 64) dst[i] = a.Qh[i];
+        // zero block + basis, once per block (the first barrier below orders it)
+        unsigned* dst = reinterpret_cast<unsigned*>(qbase);
+        for (int i = threadIdx.x; i < 128; i += WAVES * 64) dst[i] = 0u;
+        const int n = T8 * 128;
+        for (int i = threadIdx.x; i < n; i += WAVES * 64) dst[128 + i] = a.Qh[i];
     }
+    // this lane's A operand: column (lane & 15) of the basis, or zeros
+    const bool live_a = ((lane & 31) < 16) == (lane < 32);
+    const f16x8* qa = reinterpret_cast<const f16x8*>(qbase + (live_a ? 512 + (lane & 15) * 16 : 0));
+    const int qstep = live_a ? 512 / 16 : 0;  // f16x8 per block of 8 time points
+    const int h = lane >> 5;
     const float m1 = a.neg_one;
     const float invT = 1.0f / (float)a.T, invTm1 = 1.0f / ((float)a.T - 1.0f);
-    constexpr float iq = 1.0f / HX_QSCALE, iq2 = 1.0f / (HX_QSCALE * HX_QSCALE);
+    constexpr float iq = 1.0f / HX_QSCALE;
     float* my = smax + wave * (ROUNDS * NP);
     if (lane == 0)
         for (int i = 0; i < ROUNDS * NP; i++) my[i] = -INFINITY;
@@ -927,7 +945,6 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
         const int v = tl * TILE + lane;
         __syncthreads();
         {
-            // every thread stages values of voxel (threadIdx.x & 63) only
             const float sc = a.wscale[(size_t)tl * TILE + (threadIdx.x & 63)];
             const int n = a.T * TILE;
             for (int i = threadIdx.x; i < n; i += WAVES * 64)
@@ -946,42 +963,38 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
                 const int k = min(kbase + j, a.K - 1);
                 prow[j] = (cuint_p)(a.perm + (size_t)k * a.Tp);
             }
-            f32x4 acc[NP][NG];
-            float h[NP][4], s[NP], m[NP];
+            f32x16 acc[NP];
+            float hs[NP][4], s[NP], m[NP];
 #pragma unroll
             for (int j = 0; j < NP; j++) {
 #pragma unroll
-                for (int g = 0; g < NG; g++) acc[j][g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int e = 0; e < 16; e++) acc[j][e] = 0.0f;
 #pragma unroll
-                for (int q = 0; q < 4; q++) h[j][q] = 0.0f;
+                for (int q = 0; q < 4; q++) hs[j][q] = 0.0f;
                 s[j] = 0.0f;
                 m[j] = 0.0f;
             }
-            // time points 4c .. 4c + n - 1
-            auto chunk = [&](int c, int n) {
-                f16x4 qh[NG], ql[NG];
-#pragma unroll
-                for (int g = 0; g < NG; g++) {
-                    qh[g] = qs[((size_t)c * 2 * NG + g) * 4 + (lane & 3)];
-                    ql[g] = qs[((size_t)c * 2 * NG + NG + g) * 4 + (lane & 3)];
-                }
+            // time points 8c .. 8c + n - 1
+            auto block8 = [&](int c, int n) {
+                const f16x8 qh = qa[(size_t)c * qstep], ql = qa[(size_t)c * qstep + 16];
 #pragma unroll
                 for (int j = 0; j < NP; j++) {
-                    const unsigned pk[2] = {prow[j][2 * c], prow[j][2 * c + 1]};
-                    float xs[4];
+                    const unsigned pk[4] = {prow[j][4 * c], prow[j][4 * c + 1], prow[j][4 * c + 2],
+                                            prow[j][4 * c + 3]};
+                    float xs[8];
 #pragma unroll
-                    for (int e = 0; e < 4; e++) {
+                    for (int e = 0; e < 8; e++) {
                         if (e < n) {
                             const unsigned idx = (pk[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu;
                             float x = tile[idx * TILE + lane];
-                            x = fmaf(a4, h[j][3], x);
-                            x = fmaf(a3, h[j][2], x);
-                            x = fmaf(a2, h[j][1], x);
-                            x = fmaf(a1, h[j][0], x);
-                            h[j][3] = h[j][2];
-                            h[j][2] = h[j][1];
-                            h[j][1] = h[j][0];
-                            h[j][0] = x;
+                            x = fmaf(a4, hs[j][3], x);
+                            x = fmaf(a3, hs[j][2], x);
+                            x = fmaf(a2, hs[j][1], x);
+                            x = fmaf(a1, hs[j][0], x);
+                            hs[j][3] = hs[j][2];
+                            hs[j][2] = hs[j][1];
+                            hs[j][1] = hs[j][0];
+                            hs[j][0] = x;
                             s[j] = fmaf(x, x, s[j]);
                             if (MEAN) m[j] += x;
                             xs[e] = x;
@@ -989,44 +1002,46 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
                             xs[e] = 0.0f;
                         }
                     }
-                    f16x4 xh, xl;
-                    split4_f16(xs, m1, xh, xl);
-#pragma unroll
-                    for (int g = 0; g < NG; g++) {
-                        acc[j][g] = __builtin_amdgcn_mfma_f32_4x4x4f16(qh[g], xh, acc[j][g], 0, 0, 0);
-                        acc[j][g] = __builtin_amdgcn_mfma_f32_4x4x4f16(qh[g], xl, acc[j][g], 0, 0, 0);
-                        acc[j][g] = __builtin_amdgcn_mfma_f32_4x4x4f16(ql[g], xh, acc[j][g], 0, 0, 0);
-                    }
+                    f16x8 xh, xl;
+                    split8_f16(xs, m1, xh, xl);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(qh, xh, acc[j], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(qh, xl, acc[j], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ql, xh, acc[j], 0, 0, 0);
                 }
             };
-            // two chunks per trip: the AR history stays in place between them
-            const int T4f = a.T >> 2;
-            int c = 0;
-            for (; c + 1 < T4f; c += 2) {
-                chunk(c, 4);
-                chunk(c + 1, 4);
-            }
-            if (c < T4f) chunk(c, 4);
-            if (T4f < T4) chunk(T4f, a.T - 4 * T4f);
+            const int T8f = a.T >> 3;
+            for (int c = 0; c < T8f; c++) block8(c, 8);
+            if (T8f < T8) block8(T8f, a.T - 8 * T8f);
 #pragma unroll
             for (int j = 0; j < NP; j++) {
                 const int k = kbase + j;
                 const bool valid = k < a.K;
-                float ss = 0.0f, fn = 0.0f, uz = 0.0f;
+                // Register r of lane half h holds column (r % 4) + 8 ((r % 8) / 4)
+                // + 4 h of voxel (lane % 32) for r < 8 and of voxel
+                // (lane % 32) + 32 for r >= 8.  Each lane sums what it holds
+                // for both voxels; one exchange with lane ^ 32 per sum
+                // completes its own voxel's.
+                float sv[2] = {0.0f, 0.0f}, fv[2] = {0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
 #pragma unroll
-                for (int g = 0; g < NG; g++)
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int col = 4 * g + i;
-                        const float z = acc[j][g][i];
-                        ss = fmaf(z, z, ss);
-                        if (col < a.nrows) fn = fmaf(z, z, fn);
-                        if (MEAN) uz = fmaf(a.u32[col], z, uz);
-                    }
-                const float sse = s[j] - ss * iq2;
-                const float me = MEAN ? m[j] - uz * iq : 0.0f;
+                for (int rr = 0; rr < 16; rr++) {
+                    const float z = acc[j][rr] * iq;
+                    const int vv = rr >> 3;
+                    const int c0 = (rr & 3) + 8 * ((rr & 7) >> 2);  // the column in lane half 0
+                    const int col = c0 + 4 * h;
+                    sv[vv] = fmaf(z, z, sv[vv]);
+                    if (col < a.nrows) fv[vv] = fmaf(z, z, fv[vv]);
+                    if (MEAN) uv[vv] = fmaf(h ? a.u32[c0 + 4] : a.u32[c0], z, uv[vv]);
+                }
+                const float ss = (h ? sv[1] : sv[0]) + __shfl_xor(h ? sv[0] : sv[1], 32);
+                const float fn = (h ? fv[1] : fv[0]) + __shfl_xor(h ? fv[0] : fv[1], 32);
+                const float uz = MEAN ? (h ? uv[1] : uv[0]) + __shfl_xor(h ? uv[0] : uv[1], 32) : 0.0f;
+                // column 0 of voxel (lane % 32) + 32 sits in register 8 of lane half 0
+                const float z8 = __shfl_xor(acc[j][8] * iq, 32);
+                const float z0 = h ? z8 : acc[j][0] * iq;
+                const float sse = s[j] - ss;
+                const float me = MEAN ? m[j] - uz : 0.0f;
                 const float var = (sse - me * me * invT) * invTm1;
-                const float st = a.ftest ? fn * iq2 / (var * (float)a.nrows) : acc[j][0][0] * iq / sqrtf(var);
+                const float st = a.ftest ? fn / (var * (float)a.nrows) : z0 / sqrtf(var);
                 const bool act = in_mask && valid;
                 const bool bad = act && !(sse >= s[j] * 0.03125f && s[j] < 4.0e9f);
                 if (__ballot(bad) && lane == 0) a.flags[k] = 1;
@@ -1063,25 +1078,13 @@ hipError_t launch_lds(F* kern, unsigned grid, size_t lds, const PermArgs& a, hip
     return hipGetLastError();
 }
 
-template <bool MEAN>
-hipError_t launch_hx_m(const PermArgs& a, unsigned grid, size_t lds, hipStream_t s) {
-    switch (a.ng) {
-        case 1: return launch_lds(&k_fl_perm_hx<MEAN, 1>, grid, lds, a, s);
-        case 2: return launch_lds(&k_fl_perm_hx<MEAN, 2>, grid, lds, a, s);
-        case 3: return launch_lds(&k_fl_perm_hx<MEAN, 3>, grid, lds, a, s);
-        case 4: return launch_lds(&k_fl_perm_hx<MEAN, 4>, grid, lds, a, s);
-    }
-    return hipErrorInvalidValue;
-}
-
 hipError_t launch_perm_hx(PermArgs a, hipStream_t s) {
     if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
-    if (a.T > HX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS || a.ng != (a.pq + 3) / 4 || a.Tp < a.T)
-        return hipErrorInvalidValue;
+    if (a.T > HX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS || a.Tp < a.T) return hipErrorInvalidValue;
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
-    const size_t lds = (size_t)a.T * TILE * sizeof(float) + (size_t)2 * a.ng * ((a.T + 3) / 4) * 4 * sizeof(f16x4);
-    return a.mean ? launch_hx_m<true>(a, grid, lds, s) : launch_hx_m<false>(a, grid, lds, s);
+    const size_t lds = (size_t)a.T * TILE * sizeof(float) + 512 + (size_t)((a.T + 7) / 8) * 512;
+    return a.mean ? launch_lds(&k_fl_perm_hx<true>, grid, lds, a, s) : launch_lds(&k_fl_perm_hx<false>, grid, lds, a, s);
 }
 
 hipError_t launch_perm_bx(PermArgs a, hipStream_t s) {
