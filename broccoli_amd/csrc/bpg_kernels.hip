@@ -292,17 +292,40 @@ hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int 
     return hipErrorInvalidValue;
 }
 
-// out[k] = max_c partial[c*K + k]
-__global__ __launch_bounds__(256) void k_reduce_max(const float* partial, int n_part, int K, float* out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+// out[k] = max_c partial[c*K + k].  A block takes 32 permutations and splits
+// the chunks into 32 slices (thread = permutation + 32 * slice, every 32nd
+// chunk, 8 loads in flight); the slices meet in LDS.  With few permutations
+// there are up to 1024 chunks, which one thread per permutation would walk
+// one dependent load at a time.
+__global__ __launch_bounds__(1024) void k_reduce_max(const float* __restrict__ partial, int n_part, int K,
+                                                     float* __restrict__ out) {
+    __shared__ float red[32][33];
+    const int kl = threadIdx.x & 31, sl = threadIdx.x >> 5;
+    const int k = blockIdx.x * 32 + kl;
     float m = -INFINITY;
-    for (int c = 0; c < n_part; c++) m = fmaxf(m, partial[(size_t)c * K + k]);
-    out[k] = m;
+    if (k < K) {
+        int c = sl;
+        for (; c + 7 * 32 < n_part; c += 8 * 32) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = partial[(size_t)(c + 32 * u) * K + k];
+#pragma unroll
+            for (int u = 0; u < 8; u++) m = fmaxf(m, v[u]);
+        }
+        for (; c < n_part; c += 32) m = fmaxf(m, partial[(size_t)c * K + k]);
+    }
+    red[sl][kl] = m;
+    __syncthreads();
+    if (sl == 0 && k < K) {
+        float r = red[0][kl];
+        for (int i = 1; i < 32; i++) r = fmaxf(r, red[i][kl]);
+        out[k] = r;
+    }
 }
 
 hipError_t launch_reduce_max(const float* partial, int n_part, int K, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_max, dim3((K + 255) / 256), dim3(256), 0, s, partial, n_part, K, out);
+    if (K <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_max, dim3((K + 31) / 32), dim3(1024), 0, s, partial, n_part, K, out);
     return hipGetLastError();
 }
 
