@@ -416,6 +416,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.f_scale = (float)((double)m.dof / (double)m.r);
         ca.cdt = d.cdt;
         ca.tau = bf ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+        ca.pinf = INFINITY;
         ca.unsafe = pl->unsafe.p;
         ca.flags = pl->flags.p;
         BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));

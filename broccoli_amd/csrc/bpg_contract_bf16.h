@@ -83,7 +83,10 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
         const float sse = (yhi - ss) + ylo;
         bad |= sse < tyy;
         u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
-        rm[P] = fmaxf(rm[P], u[P]);
+        // max(u, rm) as med3(u, rm, +inf): one v_med3_f32, where fmaxf costs
+        // a second instruction to quiet the loop-carried rm; a NaN u (padding
+        // voxel) yields rm, as fmaxf does (checked on the device)
+        rm[P] = __builtin_amdgcn_fmed3f(u[P], rm[P], a.pinf);
     }
     if (OUT) {
         const int hl = lane >> 5;

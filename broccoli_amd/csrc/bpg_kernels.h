@@ -88,6 +88,7 @@ struct ContractArgs {
     // SSE / |y*|^2 shrinks; a wave that meets SSE < tau * |y*|^2 adds one
     // to *unsafe
     float tau;
+    float pinf;            // +infinity (a run-time value: the running max is one v_med3_f32)
     unsigned* unsafe;
     unsigned char* flags;  // [K]: 1 for a permutation with an ill-conditioned sample
 };
