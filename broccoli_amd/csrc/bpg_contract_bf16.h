@@ -73,14 +73,29 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
 #pragma unroll
     for (int P = 0; P < PER; P++) {
         const float z0 = acc[P * PP];
-        float ss = z0 * z0, num = ss;
+        float sse;
+        float num = 0.0f;
+        if (FT) {
+            float ss = z0 * z0;
+            num = ss;
 #pragma unroll
-        for (int g = 1; g < PP; g++) {
-            const float z = acc[P * PP + g];
-            ss = fmaf(z, z, ss);
-            if (FT) num = (g < a.r) ? ss : num;
+            for (int g = 1; g < PP; g++) {
+                const float z = acc[P * PP + g];
+                ss = fmaf(z, z, ss);
+                num = (g < a.r) ? ss : num;
+            }
+            sse = (yhi - ss) + ylo;
+        } else {
+            // |y*|^2 - |z|^2 as one FMA chain from |y*|^2 (one rounding per
+            // step, one instruction fewer than squaring and subtracting)
+            float d = fmaf(-z0, z0, yhi);
+#pragma unroll
+            for (int g = 1; g < PP; g++) {
+                const float z = acc[P * PP + g];
+                d = fmaf(-z, z, d);
+            }
+            sse = d + ylo;
         }
-        const float sse = (yhi - ss) + ylo;
         bad |= sse < tyy;
         u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
         // max(u, rm) as med3(u, rm, +inf): one v_med3_f32, where fmaxf costs
