@@ -305,3 +305,25 @@ def test_statistics_do_not_depend_on_the_data_scale(ctx, scale, prec):
         assert plan.timing().precision_used == PREC[prec]
     check_stats(stats, stats_ref, in_mask(st))
     check_max(md, stats)
+
+
+def test_near_unit_root_series_match_fp64(ctx):
+    # AR(1) near 1: the smoothed estimates come out above 1 (1.18..1.40 here),
+    # so inverse whitening grows without bound; the samples whose |x|^2
+    # leaves the FP16 range are flagged and every permutation is repeated in
+    # FP64, which still matches the oracle
+    st = fl_study(seed=19, T=128, p=4, ar=(0.97, 0.02, 0.0, 0.0))
+    st["precision"] = "split_f16"
+    K = 32
+    perm = capi.gen_perm_first_level(128, K, seed=13)
+    print("first-level near unit root, split_f16", flush=True)
+    _, stats_ref = oracle_null(st, perm)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+        tm = plan.timing()
+        ar = plan.ar_estimates()
+    m = in_mask(st)
+    print(f"AR1 estimates {ar[0][m].min():.3f}..{ar[0][m].max():.3f}, repeated in FP64: {tm.unsafe_waves}", flush=True)
+    assert tm.unsafe_waves > 0 and tm.unsafe_waves + tm.fp32_permutations == K
+    check_stats(stats, stats_ref, m)
+    check_max(md, stats)
