@@ -341,7 +341,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the permutation phase is measured on a single "
-              "MI355X (sharding over permutations is described in DESIGN.md)", file=sys.stderr)
+              "MI355X (across GPUs the permutations would be split, each GPU holding a copy of the "
+              "study, with one gather of the K maxima at the end)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == 4:

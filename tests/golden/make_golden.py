@@ -14,7 +14,7 @@ unmodified, see oracle/Makefile) can be built.  The fixture holds:
   srand(1)), the host Clusterize (26-connected
   BFS, labels 1..n), and pinv.
 
-Nothing here runs on the GPU box; the tests only read the .npz.
+Nothing here runs on a GPU; the tests only read the .npz.
 """
 from __future__ import annotations
 
