@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY (see bpg_oracle.h).  The product library never
  * links or calls this file; tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py use it as the checker / the timed CPU
- * restatement of the reference algorithm.
+ * cpu_baseline leg of bench.py use it to check the GPU results or as the
+ * timed CPU restatement of the reference algorithm.
  *
  * Every function names the reference lines it restates.  Reference paths are
  * relative to the BROCCOLI repository:

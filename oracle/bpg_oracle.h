@@ -2,8 +2,8 @@
  * bpg_oracle.h -- CPU oracle for the second-level permutation-test GLM.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this code.  It is used by tests/ (as the checker), by
- * __graft_entry__.smoke() (as the checker) and by bench.py's cpu_baseline leg
+ * calls this code.  It is used by tests/ (to check the GPU results), by
+ * __graft_entry__.smoke() (to check the GPU results) and by bench.py's cpu_baseline leg
  * (as the timed CPU restatement of the reference algorithm).
  *
  * Two precisions restate the reference path:

@@ -4,7 +4,7 @@
  * whitening of permuted whitened volumes, and the per-permutation t / F map.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this code.  It is used by tests/ (as the checker) and by bench.py's
+ * calls this code.  It is used by tests/ (to check the GPU results) and by bench.py's
  * cpu_baseline leg.
  *
  * Everything is FP64 with the reference's formulas and quirks kept; the

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
-checker) and bench.py's cpu_baseline leg.  The product package never imports
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (to
+check the GPU results) and bench.py's cpu_baseline leg.  The product package never imports
 it.  The functions restate the reference path (see bpg_oracle.c for the
 reference lines of each).
 """

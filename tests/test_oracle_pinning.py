@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+"""Pins the CPU oracle before it is trusted to check the HIP path.
 
 Each check is chosen so a plausible mistake (a dropped term, a wrong sign or
 index, a transposed operand) fails it:
