@@ -83,9 +83,10 @@ void smoothing_filters(float fwhm, const float* vox, float (*f)[9]) {
 // The basis of the FP32 and split-BF16 forms for statistical map `map`:
 // Q M with M orthogonal (p x p) whose first r columns are the statistic rows
 // A_i (orthonormal), so z' = M'z has z'_i = A_i z for i < r and |z'| = |z|;
-// completed by Gram-Schmidt on the unit vectors.  rot[t*16 + j] (FP64, rows
+// completed by Gram-Schmidt on the unit vectors.  rot[t*W + j] (FP64, W = 16
+// or 32 columns, rows
 // to Tq, zeros beyond T and p), u32 = M'u.
-void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<double>& rot, std::vector<float>& u32) {
+void rotated_basis(const bpg::FlModel& m, int map, int Tq, int W, std::vector<double>& rot, std::vector<float>& u32) {
     const int p = m.p, r = m.nrows[map];
     const std::vector<double>& A = m.A[map];
     std::vector<std::vector<double>> cols;
@@ -109,8 +110,8 @@ void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<double>& 
         for (double& x : v) x /= std::sqrt(n2);
         cols.push_back(v);
     }
-    rot.assign((size_t)Tq * 16, 0.0);
-    u32.assign(16, 0.0f);
+    rot.assign((size_t)Tq * W, 0.0);
+    u32.assign(W, 0.0f);
     for (int i = 0; i < (int)cols.size(); i++) {
         double uu = 0.0;
         for (int j = 0; j < p; j++) uu += m.u[j] * cols[i][j];
@@ -118,7 +119,7 @@ void rotated_basis(const bpg::FlModel& m, int map, int Tq, std::vector<double>& 
         for (int t = 0; t < m.T; t++) {
             double acc = 0.0;
             for (int j = 0; j < p; j++) acc += m.Q[(size_t)t * m.pq + j] * cols[i][j];
-            rot[(size_t)t * 16 + i] = acc;
+            rot[(size_t)t * W + i] = acc;
         }
     }
 }
@@ -154,24 +155,27 @@ void split_basis(const std::vector<double>& rot, int Tq32, std::vector<unsigned 
             }
 }
 
-// The A operands of the split-FP16 form: for time block c (8
-// points), qh[((c * 2 + hl) * 16 + col) * 8 + k] is the FP16 hi part
+// The A operands of the split-FP16 form (NB blocks of 16 basis columns,
+// rot of width 16 NB): for time block c (8 points),
+// qh[(((c * 2 + hl) * NB + nb) * 16 + col) * 8 + k] is the FP16 hi part
 // (hl = 0) or remainder (hl = 1), rounded to nearest even, of
-// HX_QSCALE * rot[(8c + k) * 16 + col] (zero beyond T).
-void split_basis_f16(const std::vector<double>& rot, int T, std::vector<unsigned short>& qh) {
-    const int T8 = (T + 7) / 8;
-    qh.assign((size_t)T8 * 2 * 16 * 8, 0);
+// HX_QSCALE * rot[(8c + k) * 16 NB + 16 nb + col] (zero beyond T).
+void split_basis_f16(const std::vector<double>& rot, int T, int NB, std::vector<unsigned short>& qh) {
+    const int T8 = (T + 7) / 8, W = 16 * NB;
+    qh.assign((size_t)T8 * 2 * NB * 16 * 8, 0);
     for (int c = 0; c < T8; c++)
-        for (int col = 0; col < 16; col++)
-            for (int k = 0; k < 8; k++) {
-                const int t = 8 * c + k;
-                const double q = t < T ? (double)bpg::fl::HX_QSCALE * rot[(size_t)t * 16 + col] : 0.0;
-                const _Float16 h = (_Float16)q;
-                const _Float16 l = (_Float16)(q - (double)h);
-                const size_t o = ((size_t)c * 2 * 16 + col) * 8 + k;
-                std::memcpy(&qh[o], &h, 2);
-                std::memcpy(&qh[o + 16 * 8], &l, 2);
-            }
+        for (int nb = 0; nb < NB; nb++)
+            for (int col = 0; col < 16; col++)
+                for (int k = 0; k < 8; k++) {
+                    const int t = 8 * c + k;
+                    const double q =
+                        t < T ? (double)bpg::fl::HX_QSCALE * rot[(size_t)t * W + 16 * nb + col] : 0.0;
+                    const _Float16 h = (_Float16)q;
+                    const _Float16 l = (_Float16)(q - (double)h);
+                    const size_t o = (((size_t)c * 2 * NB + nb) * 16 + col) * 8 + k;
+                    std::memcpy(&qh[o], &h, 2);
+                    std::memcpy(&qh[o + (size_t)NB * 16 * 8], &l, 2);
+                }
 }
 
 // The device set-up: regression, then three rounds of AR(4) estimation,
@@ -342,10 +346,11 @@ extern "C" int bpg_fl_plan_create(bpg_ctx* ctx, const bpg_fl_desc* desc, bpg_fl_
         ctx->err = "upload of the model failed";
         return fail(BPG_ERR_HIP);
     }
-    if (m.pq <= 16) {
+    {
+        // the fast forms' rotated bases (the split-FP16 one up to 32 columns)
         const int Tq = (d.T + 3) & ~3, Tq32 = (d.T + 31) & ~31;
-        if (pl->Q32.ensure((size_t)Tq * 16) || pl->u32.ensure(16) || pl->Qb.ensure((size_t)Tq32 / 32 * 2 * 64 * 8) ||
-            pl->Qh.ensure((size_t)(Tq32 / 8) * 2 * 16 * 8)) {
+        if (pl->u32.ensure(32) || pl->Qh.ensure((size_t)(Tq32 / 8) * 2 * 2 * 16 * 8) ||
+            (m.pq <= 16 && (pl->Q32.ensure((size_t)Tq * 16) || pl->Qb.ensure((size_t)Tq32 / 32 * 2 * 64 * 8)))) {
             ctx->err = "device allocation failed";
             return fail(BPG_ERR_NOMEM);
         }
@@ -468,24 +473,29 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
         BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
         BPG_HIP(ctx, pl->maxsize.ensure(K));
     }
-    // the fast forms (FP32 MFMA or split-BF16 MFMA projections) and their
-    // limits; the FP64 form otherwise and for the repeats
-    const bool fast = allow_fast && d.precision != BPG_PREC_FP64 && m.pq <= 16 && T <= bpg::fl::MX_T_MAX &&
-                      m.nrows[map] <= BPG_FL_MAX_FROWS;
-    const bool hx = fast && d.precision == BPG_PREC_SPLIT_F16 && T <= bpg::fl::HX_T_MAX;
-    const bool bx = fast && !hx && (d.precision == BPG_PREC_SPLIT_BF16 || d.precision == BPG_PREC_SPLIT_F16);
-    const bool mx = fast;
-    *fast_used = mx;
+    // the fast forms and their limits (the split-FP16 form up to 32 basis
+    // columns, the others up to 16); the FP64 form otherwise and for the
+    // repeats
+    const bool frows = allow_fast && d.precision != BPG_PREC_FP64 && m.nrows[map] <= BPG_FL_MAX_FROWS;
+    const int nb = m.pq <= 16 ? 1 : 2;
+    const bool hx = frows && d.precision == BPG_PREC_SPLIT_F16 && m.pq <= 32 &&
+                    T <= (nb == 1 ? bpg::fl::HX_T_MAX : bpg::fl::HX_T_MAX2);
+    const bool fast16 = frows && !hx && m.pq <= 16 && T <= bpg::fl::MX_T_MAX;
+    const bool bx = fast16 && (d.precision == BPG_PREC_SPLIT_BF16 || d.precision == BPG_PREC_SPLIT_F16);
+    const bool mx = fast16 && !bx;
+    const bool fast = hx || fast16;
+    const int W = hx ? 16 * nb : 16;
+    *fast_used = fast;
     const std::vector<double>& Arows = m.A[map];
     BPG_HIP(ctx, hipMemcpyAsync(pl->A.p, Arows.data(), Arows.size() * sizeof(double), hipMemcpyHostToDevice, st));
     std::vector<double> rot;
     std::vector<float> q32, u32;
     std::vector<unsigned short> qb;
-    if (mx) {
+    if (fast) {
         const int Tq = (T + 3) & ~3, Tq32 = (T + 31) & ~31;
-        rotated_basis(m, map, Tq32, rot, u32);
+        rotated_basis(m, map, Tq32, W, rot, u32);
         if (hx) {
-            split_basis_f16(rot, T, qb);
+            split_basis_f16(rot, T, nb, qb);
             BPG_HIP(ctx, hipMemcpyAsync(pl->Qh.p, qb.data(), qb.size() * sizeof(unsigned short), hipMemcpyHostToDevice,
                                         st));
         } else if (bx) {
@@ -497,7 +507,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
             for (size_t i = 0; i < q32.size(); i++) q32[i] = (float)rot[i];
             BPG_HIP(ctx, hipMemcpyAsync(pl->Q32.p, q32.data(), q32.size() * sizeof(float), hipMemcpyHostToDevice, st));
         }
-        BPG_HIP(ctx, hipMemcpyAsync(pl->u32.p, u32.data(), 16 * sizeof(float), hipMemcpyHostToDevice, st));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->u32.p, u32.data(), W * sizeof(float), hipMemcpyHostToDevice, st));
         BPG_HIP(ctx, pl->flags.ensure(K));
         BPG_HIP(ctx, pl->h_flags.ensure(K));
         BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
@@ -519,7 +529,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     a.mean = m.const_in_span ? 0 : 1;
     a.K = (int)K; a.Vc = Vc; a.Vpad = pl->Vpad; a.nwords = pl->nwords;
     a.n_vtiles = n_vtiles; a.n_vchunks = n_vchunks; a.tiles_per_chunk = tpc; a.n_pgroups = n_pgroups;
-    if (mx) {
+    if (fast) {
         a.Q32 = pl->Q32.p;
         a.Qb = reinterpret_cast<const unsigned*>(pl->Qb.p);
         a.Qh = reinterpret_cast<const unsigned*>(pl->Qh.p);
@@ -534,7 +544,7 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
                  : mx ? bpg::fl::launch_perm_mx(a, st)
                       : bpg::fl::launch_perm(a, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
-    if (mx) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
+    if (fast) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
     if (cluster) {
         bpg::CclArgs cc{};
         cc.bits = pl->bits.p;
@@ -575,9 +585,9 @@ int fl_phase(bpg_fl_plan* pl, int map, const unsigned short* perm, long K, float
     tm.contract_launches = 1;
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = d.p;
-    tm.slots_per_perm = mx ? 16 : m.pq;
+    tm.slots_per_perm = fast ? W : m.pq;
     tm.precision_used = hx ? BPG_PREC_SPLIT_F16 : bx ? BPG_PREC_SPLIT_BF16 : mx ? BPG_PREC_FP32 : BPG_PREC_FP64;
-    tm.fp32_permutations = mx ? K : 0;
+    tm.fp32_permutations = fast ? K : 0;
     return BPG_OK;
 }
 

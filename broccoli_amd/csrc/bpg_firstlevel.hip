@@ -905,10 +905,10 @@ __device__ __forceinline__ void split8_f16(const float (&x)[8], float m1, f16x8&
     lo = __builtin_bit_cast(f16x8, u32x4{l[0], l[1], l[2], l[3]});
 }
 
-template <bool MEAN>
+template <bool MEAN, int NB>
 __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
-    // [T][64] scaled whitened tile, then 512 zero bytes, then the basis:
-    // [T8][hi, lo][16 columns][8 time points] FP16
+    // [T][64] scaled whitened tile, then NB * 512 zero bytes, then the basis:
+    // [T8][hi, lo][NB blocks of 16 columns][8 time points] FP16
     extern __shared__ float tile[];
     __shared__ float smax[WAVES * ROUNDS * NP];
     const int n_blocks = a.n_vchunks * a.n_pgroups;
@@ -925,14 +925,15 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
     {
         // zero block + basis, once per block (the first barrier below orders it)
         unsigned* dst = reinterpret_cast<unsigned*>(qbase);
-        for (int i = threadIdx.x; i < 128; i += WAVES * 64) dst[i] = 0u;
-        const int n = T8 * 128;
-        for (int i = threadIdx.x; i < n; i += WAVES * 64) dst[128 + i] = a.Qh[i];
+        for (int i = threadIdx.x; i < NB * 128; i += WAVES * 64) dst[i] = 0u;
+        const int n = T8 * NB * 128;
+        for (int i = threadIdx.x; i < n; i += WAVES * 64) dst[NB * 128 + i] = a.Qh[i];
     }
-    // this lane's A operand: column (lane & 15) of the basis, or zeros
+    // this lane's A operand: column (lane & 15) of each column block, or
+    // zeros (the zero block covers every offset a zero lane reads)
     const bool live_a = ((lane & 31) < 16) == (lane < 32);
-    const f16x8* qa = reinterpret_cast<const f16x8*>(qbase + (live_a ? 512 + (lane & 15) * 16 : 0));
-    const int qstep = live_a ? 512 / 16 : 0;  // f16x8 per block of 8 time points
+    const f16x8* qa = reinterpret_cast<const f16x8*>(qbase + (live_a ? NB * 512 + (lane & 15) * 16 : 0));
+    const int qstep = live_a ? NB * 32 : 0;  // f16x8 per block of 8 time points
     const int h = lane >> 5;
     const float m1 = a.neg_one;
     const float invT = 1.0f / (float)a.T, invTm1 = 1.0f / ((float)a.T - 1.0f);
@@ -963,12 +964,14 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
                 const int k = min(kbase + j, a.K - 1);
                 prow[j] = (cuint_p)(a.perm + (size_t)k * a.Tp);
             }
-            f32x16 acc[NP];
+            f32x16 acc[NP][NB];
             float hs[NP][4], s[NP], m[NP];
 #pragma unroll
             for (int j = 0; j < NP; j++) {
 #pragma unroll
-                for (int e = 0; e < 16; e++) acc[j][e] = 0.0f;
+                for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                    for (int e = 0; e < 16; e++) acc[j][nb][e] = 0.0f;
 #pragma unroll
                 for (int q = 0; q < 4; q++) hs[j][q] = 0.0f;
                 s[j] = 0.0f;
@@ -976,7 +979,12 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
             }
             // time points 8c .. 8c + n - 1
             auto block8 = [&](int c, int n) {
-                const f16x8 qh = qa[(size_t)c * qstep], ql = qa[(size_t)c * qstep + 16];
+                f16x8 qh[NB], ql[NB];
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++) {
+                    qh[nb] = qa[(size_t)c * qstep + nb * 16];
+                    ql[nb] = qa[(size_t)c * qstep + (NB + nb) * 16];
+                }
 #pragma unroll
                 for (int j = 0; j < NP; j++) {
                     const unsigned pk[4] = {prow[j][4 * c], prow[j][4 * c + 1], prow[j][4 * c + 2],
@@ -1004,9 +1012,12 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
                     }
                     f16x8 xh, xl;
                     split8_f16(xs, m1, xh, xl);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(qh, xh, acc[j], 0, 0, 0);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(qh, xl, acc[j], 0, 0, 0);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ql, xh, acc[j], 0, 0, 0);
+#pragma unroll
+                    for (int nb = 0; nb < NB; nb++) {
+                        acc[j][nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(qh[nb], xh, acc[j][nb], 0, 0, 0);
+                        acc[j][nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(qh[nb], xl, acc[j][nb], 0, 0, 0);
+                        acc[j][nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ql[nb], xh, acc[j][nb], 0, 0, 0);
+                    }
                 }
             };
             const int T8f = a.T >> 3;
@@ -1023,21 +1034,24 @@ __global__ __launch_bounds__(512) void k_fl_perm_hx(PermArgs a) {
                 // completes its own voxel's.
                 float sv[2] = {0.0f, 0.0f}, fv[2] = {0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
 #pragma unroll
-                for (int rr = 0; rr < 16; rr++) {
-                    const float z = acc[j][rr] * iq;
-                    const int vv = rr >> 3;
-                    const int c0 = (rr & 3) + 8 * ((rr & 7) >> 2);  // the column in lane half 0
-                    const int col = c0 + 4 * h;
-                    sv[vv] = fmaf(z, z, sv[vv]);
-                    if (col < a.nrows) fv[vv] = fmaf(z, z, fv[vv]);
-                    if (MEAN) uv[vv] = fmaf(h ? a.u32[c0 + 4] : a.u32[c0], z, uv[vv]);
-                }
+                for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                    for (int rr = 0; rr < 16; rr++) {
+                        const float z = acc[j][nb][rr] * iq;
+                        const int vv = rr >> 3;
+                        // the column in lane half 0 (block nb)
+                        const int c0 = 16 * nb + (rr & 3) + 8 * ((rr & 7) >> 2);
+                        const int col = c0 + 4 * h;
+                        sv[vv] = fmaf(z, z, sv[vv]);
+                        if (col < a.nrows) fv[vv] = fmaf(z, z, fv[vv]);
+                        if (MEAN) uv[vv] = fmaf(h ? a.u32[c0 + 4] : a.u32[c0], z, uv[vv]);
+                    }
                 const float ss = (h ? sv[1] : sv[0]) + __shfl_xor(h ? sv[0] : sv[1], 32);
                 const float fn = (h ? fv[1] : fv[0]) + __shfl_xor(h ? fv[0] : fv[1], 32);
                 const float uz = MEAN ? (h ? uv[1] : uv[0]) + __shfl_xor(h ? uv[0] : uv[1], 32) : 0.0f;
                 // column 0 of voxel (lane % 32) + 32 sits in register 8 of lane half 0
-                const float z8 = __shfl_xor(acc[j][8] * iq, 32);
-                const float z0 = h ? z8 : acc[j][0] * iq;
+                const float z8 = __shfl_xor(acc[j][0][8] * iq, 32);
+                const float z0 = h ? z8 : acc[j][0][0] * iq;
                 const float sse = s[j] - ss;
                 const float me = MEAN ? m[j] - uz : 0.0f;
                 const float var = (sse - me * me * invT) * invTm1;
@@ -1080,11 +1094,17 @@ hipError_t launch_lds(F* kern, unsigned grid, size_t lds, const PermArgs& a, hip
 
 hipError_t launch_perm_hx(PermArgs a, hipStream_t s) {
     if (a.K <= 0 || a.n_vchunks <= 0) return hipSuccess;
-    if (a.T > HX_T_MAX || a.pq > 16 || a.nrows > BPG_FL_MAX_FROWS || a.Tp < a.T) return hipErrorInvalidValue;
+    const int nb = a.pq <= 16 ? 1 : 2;
+    if (a.T > (nb == 1 ? HX_T_MAX : HX_T_MAX2) || a.pq > 32 || a.nrows > BPG_FL_MAX_FROWS || a.Tp < a.T)
+        return hipErrorInvalidValue;
     const int n_blocks = a.n_vchunks * a.n_pgroups;
     const unsigned grid = (unsigned)((n_blocks + 7) / 8 * 8);
-    const size_t lds = (size_t)a.T * TILE * sizeof(float) + 512 + (size_t)((a.T + 7) / 8) * 512;
-    return a.mean ? launch_lds(&k_fl_perm_hx<true>, grid, lds, a, s) : launch_lds(&k_fl_perm_hx<false>, grid, lds, a, s);
+    const size_t lds = (size_t)a.T * TILE * sizeof(float) + (size_t)nb * 512 + (size_t)((a.T + 7) / 8) * nb * 512;
+    if (nb == 1)
+        return a.mean ? launch_lds(&k_fl_perm_hx<true, 1>, grid, lds, a, s)
+                      : launch_lds(&k_fl_perm_hx<false, 1>, grid, lds, a, s);
+    return a.mean ? launch_lds(&k_fl_perm_hx<true, 2>, grid, lds, a, s)
+                  : launch_lds(&k_fl_perm_hx<false, 2>, grid, lds, a, s);
 }
 
 hipError_t launch_perm_bx(PermArgs a, hipStream_t s) {

@@ -23,9 +23,10 @@ constexpr int LDS_T_MAX = 600;
 // largest T of the FP32 / MFMA form ([T][64] tile in LDS next to the
 // 6.25 KB of per-wave reduction space)
 constexpr int MX_T_MAX = 560;
-// largest T of the split-FP16 form (the tile plus the [T/8][2][16][8]
-// FP16 basis in LDS)
+// largest T of the split-FP16 form (the tile plus the [T/8][2][NB][16][8]
+// FP16 basis in LDS): one block of 16 basis columns (pq <= 16), two (pq <= 32)
 constexpr int HX_T_MAX = 480;
+constexpr int HX_T_MAX2 = 384;
 // the split-FP16 form scales each voxel's whitened series by a power of two
 // that brings its largest magnitude into [2^8, 2^9); the basis is scaled by
 // HX_QSCALE = 2^4
@@ -76,9 +77,9 @@ struct PermArgs {
     const double* A;               // [nrows][pq]
     const double* u;               // [pq]
     const float* Q32;              // [Tq][16] FP32 rotated basis, rows to T rounded up to 4 zero (FP32 form)
-    const float* u32;              // [16] FP32 rotated Q'1 (FP32 and split-BF16 forms)
+    const float* u32;              // [16] FP32 rotated Q'1 (FP32 and split forms; [32] for the split-FP16 form)
     const unsigned* Qb;            // [Tq32/32][2][64] lanes x 8 bf16: the rotated basis, hi then lo (split-BF16 form)
-    const unsigned* Qh;            // [T8][2][16][8] fp16: HX_QSCALE x the rotated basis, hi then lo (split-FP16 form)
+    const unsigned* Qh;            // [T8][2][NB][16][8] fp16: HX_QSCALE x the rotated basis, hi then lo (split-FP16 form)
     const float* wscale;           // [Vpad] power-of-two scale of each whitened series (split-FP16 form)
     float neg_one;                 // -1 (split-FP16 form)
     unsigned char* flags;          // [K] ill-conditioned permutations (FP32 form)
@@ -100,7 +101,8 @@ hipError_t launch_perm_mx(PermArgs a, hipStream_t s);
 // The split-BF16 form (same limits): projections on the BF16 matrix cores
 // from hi/lo splits of x and of the basis; flags samples with SSE < |x|^2/8.
 hipError_t launch_perm_bx(PermArgs a, hipStream_t s);
-// The split-FP16 form (pq <= 16, T <= HX_T_MAX): each lane runs its own
+// The split-FP16 form (pq <= 16 and T <= HX_T_MAX, or pq <= 32 and
+// T <= HX_T_MAX2, with two 16-column blocks of the basis): each lane runs its own
 // voxel's recursion and feeds 8 time points at a time, split into FP16 hi +
 // lo, to v_mfma_f32_32x32x16_f16 against the scaled FP16 basis (block-
 // diagonal over the two lane halves); flags samples with SSE < |x|^2/32 or

@@ -227,8 +227,9 @@ typedef struct {
                                FP64 (p <= 16, T <= 560; other studies run
                                FP64): BPG_PREC_SPLIT_F16 (FP16 hi/lo splits of
                                power-of-two scaled series, samples with
-                               SSE < |x|^2/32 repeated; T <= 480, the
-                               split-BF16 form above that),
+                               SSE < |x|^2/32 repeated; p <= 16 with
+                               T <= 480, p <= 25 with T <= 384, the
+                               split-BF16 form for p <= 16 above that),
                                BPG_PREC_SPLIT_BF16 (BF16 hi/lo splits,
                                SSE < |x|^2/8) or BPG_PREC_FP32 (FP32 MFMA,
                                SSE < |x|^2/64); BPG_PREC_FP64: everything in

@@ -263,14 +263,19 @@ def test_ill_conditioned_permutations_repeat_in_fp64(ctx, prec):
 
 @pytest.mark.parametrize("prec", ["split_f16", "split_bf16", "fp32"])
 def test_fast_forms_fall_back_to_fp64_outside_their_limits(ctx, prec):
-    for kw in ({"T": 90, "p": 25}, {"W": 7, "H": 6, "D": 5, "T": 600, "p": 4}):
+    # the split-FP16 form covers 25 regressors (two blocks of 16 basis
+    # columns, T <= 384); the other fast forms 16
+    cases = [({"T": 90, "p": 25}, capi.PREC_SPLIT_F16 if prec == "split_f16" else capi.PREC_FP64),
+             ({"W": 7, "H": 6, "D": 5, "T": 400, "p": 20}, capi.PREC_FP64),
+             ({"W": 7, "H": 6, "D": 5, "T": 600, "p": 4}, capi.PREC_FP64)]
+    for kw, want in cases:
         st = fl_study(seed=16, **kw)
         st["precision"] = prec
         perm = capi.gen_perm_first_level(kw["T"], 8, seed=1)
         print(f"first-level fallback {kw}", flush=True)
         with capi.FlPlan(ctx, st) as plan:
             plan.permutation_phase(0, perm)
-            assert plan.timing().precision_used == capi.PREC_FP64
+            assert plan.timing().precision_used == want
 
 
 def test_split_f16_hands_long_series_to_the_split_bf16_form(ctx):
@@ -326,4 +331,30 @@ def test_near_unit_root_series_match_fp64(ctx):
     print(f"AR1 estimates {ar[0][m].min():.3f}..{ar[0][m].max():.3f}, repeated in FP64: {tm.unsafe_waves}", flush=True)
     assert tm.unsafe_waves > 0 and tm.unsafe_waves + tm.fp32_permutations == K
     check_stats(stats, stats_ref, m)
+    check_max(md, stats)
+
+
+@pytest.mark.parametrize("test", [0, 1])
+def test_split_f16_two_column_blocks(ctx, test):
+    # 20 regressors without an intercept (the mean term), t and a 3-row F:
+    # the split-FP16 form with two blocks of 16 basis columns
+    st = fl_study(seed=20, T=120, p=21)
+    st["X"] = np.ascontiguousarray(st["X"][:, 1:])
+    p = st["X"].shape[1]
+    C = np.zeros((3, p), np.float32)
+    C[0, 1] = 1.0
+    C[1, 2] = 1.0
+    C[2, 17] = 1.0
+    st["C"] = C[:1] if test == 0 else C
+    st["test"] = capi.TTEST if test == 0 else capi.FTEST
+    st["precision"] = "split_f16"
+    K = 24
+    perm = capi.gen_perm_first_level(120, K, seed=17)
+    print(f"first-level split_f16 p={p} test {test}", flush=True)
+    _, stats_ref = oracle_null(st, perm, test=test)
+    with capi.FlPlan(ctx, st) as plan:
+        md, stats = plan.permutation_phase(0, perm, want_stats=True)
+        tm = plan.timing()
+    assert tm.precision_used == capi.PREC_SPLIT_F16
+    check_stats(stats, stats_ref, in_mask(st))
     check_max(md, stats)
