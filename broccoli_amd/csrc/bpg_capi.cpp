@@ -301,6 +301,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const bool want_stats = stats != nullptr || mass || tfce;
     const int pfit = m.pfit;
     const int KK = (N + 15) / 16;
+    const int npy = bpg::bf16_y_pieces(KK, m.sign_operand);
     const int NS = f32_rows(N);
     const int PP = bf16_slots(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
@@ -321,7 +322,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     BPG_HIP(ctx, pl->Qn.ensure((size_t)NR * std::max(QB, 1)));
     BPG_HIP(ctx, pl->dscale.ensure(NR));
     if (bf) {
-        BPG_HIP(ctx, pl->Yb.ensure((size_t)(Vpad / 32) * KK * 2 * 64 * 4));
+        BPG_HIP(ctx, pl->Yb.ensure((size_t)(Vpad / 32) * KK * npy * 64 * 4));
         BPG_HIP(ctx, pl->Gb.ensure((size_t)n_sub * KK * 2 * 64 * 4));
     } else {
         BPG_HIP(ctx, pl->Y32.ensure((size_t)NS * Vpad));
@@ -359,7 +360,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         bpg::TransformArgs ta{};
         ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;
         ta.N = N; ta.q = m.q; ta.Vc = Vc; ta.Vpad = Vpad;
-        ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : NS;
+        ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : NS; ta.npy = npy;
         ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf ? nullptr : pl->Y32.p;
         BPG_HIP(ctx, bpg::launch_transform(ta, st));
     }
@@ -415,7 +416,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.sqrt_dof = (float)std::sqrt((double)m.dof);
         ca.f_scale = (float)((double)m.dof / (double)m.r);
         ca.cdt = d.cdt;
-        ca.tau = bf ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+        // with three data fragments every product is exact: the FP32 bound
+        ca.tau = bf && npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
         ca.pinf = INFINITY;
         ca.unsafe = pl->unsafe.p;
         ca.flags = pl->flags.p;

@@ -30,23 +30,28 @@
 
 namespace bpg {
 
-template <int KK>
+// The data tile of voxel tile vt: NPY fragments per k-step (hi, lo; or hi,
+// mid, lo for exact +-1 operands at small N), bm unused with NPY = 2.
+template <int KK, int NPY>
 __device__ __forceinline__ void load_tile(const bf16x8* Y, const double* yy, int vt, int lane,
-                                          bf16x8 (&bh)[KK], bf16x8 (&bl)[KK], double& yv) {
+                                          bf16x8 (&bh)[KK], bf16x8 (&bm)[NPY == 3 ? KK : 1], bf16x8 (&bl)[KK],
+                                          double& yv) {
 #pragma unroll
     for (int kk = 0; kk < KK; kk++) {
-        const size_t base = (((size_t)vt * KK + kk) * 2) * 64 + lane;
+        const size_t base = (((size_t)vt * KK + kk) * NPY) * 64 + lane;
         bh[kk] = Y[base];
-        bl[kk] = Y[base + 64];
+        if constexpr (NPY == 3) bm[kk] = Y[base + 64];
+        bl[kk] = Y[base + (NPY - 1) * 64];
     }
     yv = yy[vt * 32 + (lane & 31)];
 }
 
-// z for one subtile and one voxel tile: lo*hi + hi*lo + hi*hi (the lo*hi
-// product vanishes for exact +-1 operands)
-template <int KK, bool SIGN>
+// z for one subtile and one voxel tile: lo*hi + hi*lo + hi*hi; for exact
+// +-1 operands the lo*hi product vanishes and, with NPY = 3, hi*mid joins
+// (every product then exact: the error is the FP32 accumulation's)
+template <int KK, bool SIGN, int NPY>
 __device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&al)[KK], const bf16x8 (&bh)[KK],
-                                        const bf16x8 (&bl)[KK]) {
+                                        const bf16x8 (&bm)[NPY == 3 ? KK : 1], const bf16x8 (&bl)[KK]) {
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[e] = 0.0f;
@@ -54,6 +59,7 @@ __device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&a
     for (int kk = 0; kk < KK; kk++) {
         if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[kk], bh[kk], acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kk], bl[kk], acc, 0, 0, 0);
+        if constexpr (NPY == 3) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kk], bm[kk], acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kk], bh[kk], acc, 0, 0, 0);
     }
     return acc;
@@ -130,10 +136,11 @@ struct Pending {
     int vt;
 };
 
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE, int NPY>
 __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
                                               const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
-                                              const bf16x8 (&bl)[KK], double yv, int vt, int lane, int sub0,
+                                              const bf16x8 (&bm)[NPY == 3 ? KK : 1], const bf16x8 (&bl)[KK],
+                                              double yv, int vt, int lane, int sub0,
                                               float (&runmax)[R][16 / PP], Pending& pd,
                                               bool (&bad)[PIPE ? 1 : R]) {
     const float yhi = (float)yv;  // NaN for padding voxels
@@ -141,16 +148,16 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
     if constexpr (!PIPE) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
+            const f32x16 acc = chain<KK, SIGN, NPY>(ahi[r], alo[r], bh, bm, bl);
             epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[PIPE ? 0 : r]);
         }
     } else {
-        constexpr int NM = KK * (SIGN ? 2 : 3);
+        constexpr int NM = KK * (SIGN ? NPY : 3);
         constexpr int PER = 16 / PP;
         constexpr int NV = (PER * (PP + 5) + NM - 1) / NM;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const f32x16 acc = chain<KK, SIGN>(ahi[r], alo[r], bh, bl);
+            const f32x16 acc = chain<KK, SIGN, NPY>(ahi[r], alo[r], bh, bm, bl);
             if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0]);
             else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad[0]);
 #pragma unroll
@@ -216,7 +223,9 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
 
     const int vt0 = vc * L.vt_per_chunk;
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
-    bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK];
+    constexpr int NPY = bf16_y_pieces(KK, SIGN);
+    constexpr int NBM = NPY == 3 ? KK : 1;
+    bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK], b0m[NBM], b1m[NBM];
     double y0 = 0.0, y1 = 0.0;
     // this lane met an ill-conditioned sample, per resident subtile (one
     // flag for all of them in the pipelined schedule, which has no registers
@@ -225,14 +234,15 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
     bool bad[NB];
 #pragma unroll
     for (int r = 0; r < NB; r++) bad[r] = false;
-    if (vt0 < vt1) load_tile<KK>(Y, a.yy, vt0, lane, b0h, b0l, y0);
+    if (vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
     for (int vt = vt0; vt < vt1; vt += 2) {
-        if (vt + 1 < vt1) load_tile<KK>(Y, a.yy, vt + 1, lane, b1h, b1l, y1);
-        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b0h, b0l, y0, vt, lane, sub0, runmax, pd, bad);
+        if (vt + 1 < vt1) load_tile<KK, NPY>(Y, a.yy, vt + 1, lane, b1h, b1m, b1l, y1);
+        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b0h, b0m, b0l, y0, vt, lane, sub0, runmax,
+                                                           pd, bad);
         if (vt + 1 >= vt1) break;
-        if (vt + 2 < vt1) load_tile<KK>(Y, a.yy, vt + 2, lane, b0h, b0l, y0);
-        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE>(a, ahi, alo, b1h, b1l, y1, vt + 1, lane, sub0, runmax, pd,
-                                                      bad);
+        if (vt + 2 < vt1) load_tile<KK, NPY>(Y, a.yy, vt + 2, lane, b0h, b0m, b0l, y0);
+        contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b1h, b1m, b1l, y1, vt + 1, lane, sub0,
+                                                           runmax, pd, bad);
     }
     if constexpr (PIPE)
         epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0]);

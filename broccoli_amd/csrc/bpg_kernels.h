@@ -44,9 +44,10 @@ struct TransformArgs {
     const double* dscale;  // [NR], zero beyond N (NR = N rounded up to 16)
     int N, q, Vc, Vpad;
     int KK;                // BF16 k-steps of 16 subjects (with Yb)
+    int npy;               // BF16 fragments per subject in Yb: 2 (hi, lo) or 3 (hi, mid, lo)
     int NS;                // FP32 operand rows, >= N (with Y32)
     double* yy;            // [Vpad] |y*|^2 (padding voxels: NaN)
-    unsigned* Yb;          // BF16 fragments, (Vpad/32)*KK*2*64 uint4
+    unsigned* Yb;          // BF16 fragments, (Vpad/32)*KK*npy*64 uint4
     float* Y32;            // [NS][Vpad]
 };
 // padded width of the nuisance basis for q columns (0 for none, -1 if q > 24)
@@ -107,6 +108,10 @@ struct Bf16Launch {
 constexpr int bf16_rows_per_wave(int KK, bool sign) {
     return KK <= 4 ? (sign ? BPG_R_SIGN : BPG_R_SMALL) : (KK <= BPG_R2_KK ? 2 : 1);
 }
+// data fragments per subject: hi + mid + lo for exact +-1 operands at
+// KK <= 4 (every product exact, so the FP32 form's conditioning bound
+// applies), hi + lo otherwise
+constexpr int bf16_y_pieces(int KK, bool sign) { return sign && KK <= 4 ? 3 : 2; }
 constexpr bool bf16_pipelined(int KK, bool sign) {
     return !sign && (KK <= 4 ? BPG_PIPE_SMALL : (KK <= 8 ? BPG_PIPE_MID : BPG_PIPE_LARGE)) != 0;
 }

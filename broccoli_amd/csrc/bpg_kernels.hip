@@ -92,18 +92,33 @@ __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
             for (int h = 0; h < 2; h++) {
                 double o[8];
                 residual8(kk * 16 + h * 8, o, yy);
-                unsigned hi[4], lo[4];
+                // hi, (mid,) lo: each the bf16 rounding of what the earlier
+                // parts leave of o
+                unsigned hi[4], mi[4], lo[4];
 #pragma unroll
                 for (int e = 0; e < 8; e += 2) {
-                    const unsigned short h0 = f2bf((float)o[e]), h1 = f2bf((float)o[e + 1]);
-                    const unsigned short l0 = f2bf((float)(o[e] - (double)bf2f(h0)));
-                    const unsigned short l1 = f2bf((float)(o[e + 1] - (double)bf2f(h1)));
-                    hi[e / 2] = h0 | ((unsigned)h1 << 16);
-                    lo[e / 2] = l0 | ((unsigned)l1 << 16);
+                    unsigned short pc[2][3];
+#pragma unroll
+                    for (int d = 0; d < 2; d++) {
+                        double r = o[e + d];
+                        pc[d][0] = f2bf((float)r);
+                        r -= (double)bf2f(pc[d][0]);
+                        pc[d][1] = f2bf((float)r);
+                        r -= (double)bf2f(pc[d][1]);
+                        pc[d][2] = f2bf((float)r);
+                    }
+                    hi[e / 2] = pc[0][0] | ((unsigned)pc[1][0] << 16);
+                    mi[e / 2] = pc[0][1] | ((unsigned)pc[1][1] << 16);
+                    lo[e / 2] = pc[0][2] | ((unsigned)pc[1][2] << 16);
                 }
-                const size_t base = (((size_t)vt * a.KK + kk) * 2) * 64 + h * 32 + col;
+                const size_t base = (((size_t)vt * a.KK + kk) * a.npy) * 64 + h * 32 + col;
                 Yb4[base] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-                Yb4[base + 64] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+                if (a.npy == 3) {
+                    Yb4[base + 64] = make_uint4(mi[0], mi[1], mi[2], mi[3]);
+                    Yb4[base + 128] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+                } else {
+                    Yb4[base + 64] = make_uint4(mi[0], mi[1], mi[2], mi[3]);  // the two-part lo
+                }
             }
     } else {
         for (int s0 = 0; s0 < a.NS; s0 += 8) {
