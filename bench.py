@@ -365,7 +365,7 @@ def main():
 
     plan, md, dt, kern_ms, tm = run_form(ctx, st, perms, "split_bf16", args.steps, args.warmup)
     plan.close()
-    sign_form = "sign" in perms
+    n_products = 2 if ("sign" in perms and st["N"] > 64) else 3
     print(f"# split_bf16: {dt*1e3:.3f} ms/step, contraction {kern_ms:.3f} ms", file=sys.stderr, flush=True)
     value = K * V / dt
     flops = 2.0 * N * p * K * V
@@ -392,9 +392,10 @@ def main():
                      "avg_launch_ms": kern_ms,
                      "mfma_pipe_busy_est": mfma_busy(st, K, kern_ms, tm.slots_per_perm),
                      # the split form issues 3 BF16 products per FP32-accurate
-                     # product (2 for +-1 operands): its own ceiling
-                     "split_form_peak": PEAK_BF16_TFLOPS / (2 if sign_form else 3),
-                     "frac_of_split_form_peak": achieved / (PEAK_BF16_TFLOPS / (2 if sign_form else 3))},
+                     # product (2 for +-1 operands above 64 subjects): its own
+                     # ceiling
+                     "split_form_peak": PEAK_BF16_TFLOPS / n_products,
+                     "frac_of_split_form_peak": achieved / (PEAK_BF16_TFLOPS / n_products)},
         "fit_columns": tm.fit_columns,
         "precision_used": "split_bf16" if tm.precision_used == capi.PREC_SPLIT_BF16 else "fp32",
         "ill_conditioned_waves": tm.unsafe_waves,
