@@ -149,11 +149,11 @@ def run_first_level(args):
         hip_sync()
         return plan, md, (time.perf_counter() - t0) / steps, float(np.mean(kern)), setup_wall
 
-    plan64, md64, dt64, kern64, _ = timed("fp64", max(2, args.steps // 2), 1)
+    plan64, md64, dt64, kern64, _ = timed("fp64", max(2, min(10, args.steps // 2)), 1)
     plan64.close()
-    plan32, md32, dt32, kern32, _ = timed("fp32", max(2, args.steps // 2), 1)
+    plan32, md32, dt32, kern32, _ = timed("fp32", max(2, min(10, args.steps // 2)), 1)
     plan32.close()
-    planb, mdb, dtb, kernb, _ = timed("split_bf16", max(2, args.steps // 2), 1)
+    planb, mdb, dtb, kernb, _ = timed("split_bf16", max(2, min(10, args.steps // 2)), 1)
     planb.close()
     plan, md, dt, kern_ms, setup_wall = timed("split_f16", args.steps, args.warmup)
     V = plan.num_voxels
@@ -323,7 +323,7 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--K", type=int, default=10000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -402,7 +402,7 @@ def main():
         "permutations_repeated_in_fp32": tm.fp32_permutations,
     }
     if not args.no_fp32:
-        plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, args.steps // 2), 1)
+        plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, min(10, args.steps // 2)), 1)
         plan.close()
         print(f"# fp32: {dt32*1e3:.3f} ms/step, contraction {kern32:.3f} ms", file=sys.stderr, flush=True)
         a32 = flops / (kern32 * 1e-3) / 1e12
