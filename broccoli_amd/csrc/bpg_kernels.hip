@@ -649,6 +649,15 @@ __device__ __forceinline__ void tfce_add(float* acc, float sqrt_size, float h) {
 // voxel behind (y-1) touches its two x-neighbours, so when one of them is
 // active the union with it stands for the others; their own links to it
 // are made by the in-plane rules of that plane.
+// The 13 neighbour entries of compact voxel c, loaded before any of them is
+// tested: the loads issue together (one memory latency per voxel, where
+// loads behind the tests would follow each other)
+__device__ __forceinline__ void load_nbr(const int* nbr, int c, int (&nc)[BPG_NBR]) {
+    const int* row = nbr + (size_t)c * BPG_NBR;
+#pragma unroll
+    for (int j = 0; j < BPG_NBR; j++) nc[j] = row[j];
+}
+
 __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)a.nmaps * a.nwords) return;
@@ -671,7 +680,8 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
         } else if (stage == 1) {
             // neighbour order: x-1, y-1, z-1, then (x-1,y-1), (x+1,y-1), then
             // the 8 others of the plane below
-            const int* nc = a.nbr + (size_t)c * BPG_NBR;
+            int nc[BPG_NBR];
+            load_nbr(a.nbr, c, nc);
             const int nx = nc[0], ny = nc[1], nz = nc[2];
             if (active(nx)) uf_union(parent, (unsigned)c, (unsigned)nx);
             const bool ay = active(ny), az = active(nz);
@@ -679,9 +689,11 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
             if (az) uf_union(parent, (unsigned)c, (unsigned)nz);
             if (a.connectivity == 6) continue;
             if (!ay)
+#pragma unroll
                 for (int j = 3; j < 5; j++)
                     if (active(nc[j])) uf_union(parent, (unsigned)c, (unsigned)nc[j]);
             if (!az)
+#pragma unroll
                 for (int j = 5; j < BPG_NBR; j++)
                     if (active(nc[j])) uf_union(parent, (unsigned)c, (unsigned)nc[j]);
         } else if (stage == 2) {
@@ -798,7 +810,8 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
             if (c >= a.Vc) break;
             const unsigned lc = srank[w] + __popc(sbits[w] & ((1u << b) - 1u));
             // the same implied-link rules as the global path
-            const int* nc = a.nbr + (size_t)c * BPG_NBR;
+            int nc[BPG_NBR];
+            load_nbr(a.nbr, c, nc);
             const int nx = nc[0], ny = nc[1], nz = nc[2];
             if (active(nx)) lds_union(spar, lc, rank(nx));
             const bool ay = active(ny), az = active(nz);
@@ -806,9 +819,11 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
             if (az) lds_union(spar, lc, rank(nz));
             if (nb6) continue;
             if (!ay)
+#pragma unroll
                 for (int j = 3; j < 5; j++)
                     if (active(nc[j])) lds_union(spar, lc, rank(nc[j]));
             if (!az)
+#pragma unroll
                 for (int j = 5; j < BPG_NBR; j++)
                     if (active(nc[j])) lds_union(spar, lc, rank(nc[j]));
         }
