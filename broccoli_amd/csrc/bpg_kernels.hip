@@ -767,7 +767,15 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     unsigned* spar = srank + nw;      // [cap]
     unsigned* ssz = spar + cap;       // [cap]
     const unsigned* bits = a.bits + (size_t)map * nw;
-    for (int w = tid; w < nw; w += CCL_LDS_THREADS) sbits[w] = bits[w];
+    // 8 loads in flight per thread (indices clamped, stores guarded)
+    for (int w0 = tid; w0 < nw; w0 += 8 * CCL_LDS_THREADS) {
+        unsigned v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = bits[min(w0 + u * CCL_LDS_THREADS, nw - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (w0 + u * CCL_LDS_THREADS < nw) sbits[w0 + u * CCL_LDS_THREADS] = v[u];
+    }
     __syncthreads();
     // ranks: contiguous chunks per thread, block scan of the chunk counts
     const int per = (nw + CCL_LDS_THREADS - 1) / CCL_LDS_THREADS;
