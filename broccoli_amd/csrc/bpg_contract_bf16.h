@@ -234,8 +234,43 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
     bool bad[NB];
 #pragma unroll
     for (int r = 0; r < NB; r++) bad[r] = false;
-    if (vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
-    for (int vt = vt0; vt < vt1; vt += 2) {
+    // Rolling schedule (general operands, KK >= 5, no software pipelining):
+    // one data buffer; the k-step loop is outermost, and once k-step kk has
+    // fed every resident subtile its registers take the next tile's k-step
+    // kk (prefetched a whole tile ahead, as with two buffers, for half the
+    // registers); the R accumulation chains interleave.
+    constexpr bool ROLL = !PIPE && !SIGN && KK >= 5;
+    if constexpr (ROLL) {
+        if (vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
+        for (int vt = vt0; vt < vt1; vt++) {
+            const int vn = min(vt + 1, vt1 - 1);  // the last tile reloads itself
+            f32x16 acc[R];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[r][e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < KK; kk++) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], b0h[kk], acc[r], 0, 0, 0);
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], b0l[kk], acc[r], 0, 0, 0);
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], b0h[kk], acc[r], 0, 0, 0);
+                }
+                const size_t base = (((size_t)vn * KK + kk) * 2) * 64 + lane;
+                b0h[kk] = Y[base];
+                b0l[kk] = Y[base + 64];
+            }
+            const double yn = a.yy[vn * 32 + (lane & 31)];
+            const float yhi = (float)y0;  // NaN for padding voxels
+            const float ylo = (float)(y0 - (double)yhi);
+#pragma unroll
+            for (int r = 0; r < R; r++) epi<PP, FT, OUT>(a, acc[r], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r]);
+            y0 = yn;
+        }
+    }
+    if (!ROLL && vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
+    for (int vt = vt0; vt < vt1 && !ROLL; vt += 2) {
         if (vt + 1 < vt1) load_tile<KK, NPY>(Y, a.yy, vt + 1, lane, b1h, b1m, b1l, y1);
         contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b0h, b0m, b0l, y0, vt, lane, sub0, runmax,
                                                            pd, bad);
