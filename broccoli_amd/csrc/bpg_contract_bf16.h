@@ -179,7 +179,7 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 // multiplied.  At the end, a 32-lane butterfly per permutation and one
 // partial maximum per (voxel chunk, permutation).
 template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
-__global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
+__device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16Launch& L) {
     constexpr int PER = 16 / PP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -320,13 +320,37 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
         }
 }
 
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+__global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
+    contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
+}
+
+// The rolling schedule (general operands, KK 5..8), held to 256 registers so
+// that two waves share each SIMD
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+__global__ __launch_bounds__(256, 2) void k_contract_bf16_roll(ContractArgs a, Bf16Launch L) {
+    contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
+}
+
 template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
     const int blocks = L.per_xcd * 8;
-    if (a.stats || a.bits)
-        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s, a, L);
-    else
-        hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, false, PIPE>), dim3(blocks), dim3(256), 0, s, a, L);
+    constexpr bool ROLL = !PIPE && !SIGN && KK >= 5;
+    if constexpr (ROLL) {
+        if (a.stats || a.bits)
+            hipLaunchKernelGGL((k_contract_bf16_roll<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s,
+                               a, L);
+        else
+            hipLaunchKernelGGL((k_contract_bf16_roll<KK, R, PP, SIGN, FT, false, PIPE>), dim3(blocks), dim3(256), 0,
+                               s, a, L);
+    } else {
+        if (a.stats || a.bits)
+            hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s, a,
+                               L);
+        else
+            hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, false, PIPE>), dim3(blocks), dim3(256), 0, s,
+                               a, L);
+    }
     return hipGetLastError();
 }
 
