@@ -239,7 +239,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
     // fed every resident subtile its registers take the next tile's k-step
     // kk (prefetched a whole tile ahead, as with two buffers, for half the
     // registers); the R accumulation chains interleave.
-    constexpr bool ROLL = !PIPE && !SIGN && KK >= 5;
+    constexpr bool ROLL = !PIPE && !SIGN && KK >= BPG_ROLL_MIN_KK;
     if constexpr (ROLL) {
         if (vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
         for (int vt = vt0; vt < vt1; vt++) {
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_bf16_roll(ContractArgs a, B
 template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
     const int blocks = L.per_xcd * 8;
-    constexpr bool ROLL = !PIPE && !SIGN && KK >= 5;
+    constexpr bool ROLL = !PIPE && !SIGN && KK >= BPG_ROLL_MIN_KK;
     if constexpr (ROLL) {
         if (a.stats || a.bits)
             hipLaunchKernelGGL((k_contract_bf16_roll<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s,

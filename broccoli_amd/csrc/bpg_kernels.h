@@ -29,6 +29,9 @@
 #ifndef BPG_PIPE_LARGE
 #define BPG_PIPE_LARGE 1
 #endif
+#ifndef BPG_ROLL_MIN_KK
+#define BPG_ROLL_MIN_KK 5
+#endif
 #ifndef BPG_TARGET_BLOCKS
 #define BPG_TARGET_BLOCKS 4096
 #endif
