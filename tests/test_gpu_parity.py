@@ -306,6 +306,36 @@ def test_design_size_limits(ctx, N, p, nc, ftest):
     check_maxT(md, ref, stats)
 
 
+@pytest.mark.parametrize("want_stats", [False, True])
+def test_rolling_schedule_t_test(ctx, want_stats):
+    """90 subjects (6 k-steps of 16): the rolling single-buffer contraction,
+    max-T only and with every statistic written, against the FP64 oracle."""
+    N, p = 90, 3
+    rng = np.random.default_rng(90)
+    st = synth.study(2, scale=0.1)
+    st["N"], st["p"], st["nc"] = N, p, 1
+    st["X"] = np.c_[np.ones(N), rng.standard_normal((N, p - 1))].astype(np.float32)
+    st["C"] = np.array([[0.0, 1.0, 0.0]], np.float32)
+    st["Y"] = rng.standard_normal((N,) + st["mask"].shape).astype(np.float32)
+    st["inference"] = capi.VOXEL
+    K = 40
+    perms = dict(perm=capi.gen_perm_correlation(N, K, compat=False, seed=11))
+    print(f"rolling schedule N={N} want_stats={want_stats}", flush=True)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=want_stats, **perms)
+        tm = plan.timing()
+    assert tm.precision_used == capi.PREC_SPLIT_BF16
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    if want_stats:
+        err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+        assert err.max() <= TOL, err.max()
+        check_maxT(md, ref, stats)
+    else:
+        q_ref = np.trunc(ref.max(axis=1) * 1e4)
+        q_gpu = np.round(md.astype(np.float64) * 1e4)
+        assert np.all(np.abs(q_ref - q_gpu) <= 1), np.abs(q_ref - q_gpu).max()
+
+
 @pytest.mark.parametrize("conn", [26, 6])
 def test_tfce_null_and_voxel_pvalues(ctx, conn):
     """TFCE (LIB:16756-16762, LIB:18464-18508, KC:482-510): the null of each
