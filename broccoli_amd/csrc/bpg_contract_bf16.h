@@ -234,12 +234,12 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
     bool bad[NB];
 #pragma unroll
     for (int r = 0; r < NB; r++) bad[r] = false;
-    // Rolling schedule (general operands, KK >= 5, no software pipelining):
+    // Rolling schedule (two data fragments, KK >= 5, no software pipelining):
     // one data buffer; the k-step loop is outermost, and once k-step kk has
     // fed every resident subtile its registers take the next tile's k-step
     // kk (prefetched a whole tile ahead, as with two buffers, for half the
     // registers); the R accumulation chains interleave.
-    constexpr bool ROLL = !PIPE && !SIGN && KK >= BPG_ROLL_MIN_KK;
+    constexpr bool ROLL = !PIPE && NPY == 2 && KK >= BPG_ROLL_MIN_KK;
     if constexpr (ROLL) {
         if (vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
         for (int vt = vt0; vt < vt1; vt++) {
@@ -253,7 +253,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
             for (int kk = 0; kk < KK; kk++) {
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], b0h[kk], acc[r], 0, 0, 0);
+                    if (!SIGN) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], b0h[kk], acc[r], 0, 0, 0);
                     acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], b0l[kk], acc[r], 0, 0, 0);
                     acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], b0h[kk], acc[r], 0, 0, 0);
                 }
@@ -325,8 +325,8 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
 }
 
-// The rolling schedule (general operands, KK 5..8), held to 256 registers so
-// that two waves share each SIMD
+// The rolling schedule (two data fragments, KK 5..8), held to 256 registers
+// so that two waves share each SIMD
 template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
 __global__ __launch_bounds__(256, 2) void k_contract_bf16_roll(ContractArgs a, Bf16Launch L) {
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_bf16_roll(ContractArgs a, B
 template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
     const int blocks = L.per_xcd * 8;
-    constexpr bool ROLL = !PIPE && !SIGN && KK >= BPG_ROLL_MIN_KK;
+    constexpr bool ROLL = !PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK;
     if constexpr (ROLL) {
         if (a.stats || a.bits)
             hipLaunchKernelGGL((k_contract_bf16_roll<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s,

@@ -336,6 +336,29 @@ def test_rolling_schedule_t_test(ctx, want_stats):
         assert np.all(np.abs(q_ref - q_gpu) <= 1), np.abs(q_ref - q_gpu).max()
 
 
+def test_rolling_schedule_sign_flips(ctx):
+    """Sign flips with 100 subjects (7 k-steps): the rolling contraction with
+    two products per k-step, against the FP64 oracle."""
+    N = 100
+    rng = np.random.default_rng(100)
+    st = synth.study(1, scale=0.1)
+    st["N"] = N
+    st["X"] = np.ones((N, 1), np.float32)
+    st["Y"] = (rng.standard_normal((N,) + st["mask"].shape) + 0.2).astype(np.float32)
+    K = 48
+    perms = synth.permutations(st, K, seed=13)
+    assert "sign" in perms
+    print("rolling schedule, sign flips N=100", flush=True)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        tm = plan.timing()
+    assert tm.precision_used == capi.PREC_SPLIT_BF16
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+    assert err.max() <= TOL, err.max()
+    check_maxT(md, ref, stats)
+
+
 @pytest.mark.parametrize("conn", [26, 6])
 def test_tfce_null_and_voxel_pvalues(ctx, conn):
     """TFCE (LIB:16756-16762, LIB:18464-18508, KC:482-510): the null of each
