@@ -130,6 +130,7 @@ extern "C" int bpg_create(int device, bpg_ctx** out) {
     c->device = device;
     c->arch = prop.gcnArchName;
     c->lds_per_block = prop.sharedMemPerBlockOptin ? prop.sharedMemPerBlockOptin : prop.sharedMemPerBlock;
+    if (prop.multiProcessorCount > 0) c->cus = prop.multiProcessorCount;
     c->name = prop.name[0] ? std::string(prop.name) : "AMD GPU " + c->arch;
     if (c->arch.compare(0, 6, "gfx950") != 0) {
         delete c;
@@ -305,7 +306,12 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const int NS = f32_rows(N);
     const int PP = bf16_slots(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
-    const bpg::Bf16Launch Lb = bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad, m.sign_operand) : bpg::Bf16Launch{};
+    const bpg::Bf16Launch Lb =
+        bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad, m.sign_operand,
+                                     ctx->cus / 8 *
+                                         bpg::contract_bf16_blocks_per_cu(KK, PP, m.sign_operand,
+                                                                          d.test == BPG_FTEST, want_stats || cluster))
+           : bpg::Bf16Launch{};
     const bpg::F32Launch Lf = bf ? bpg::F32Launch{} : bpg::plan_contract_f32((int)K, pfit, Vpad);
     const int n_sub = Lb.n_sub;
     const int n_part = bf ? Lb.n_vchunks : Lf.n_vchunks;

@@ -332,50 +332,47 @@ __global__ __launch_bounds__(256, 2) void k_contract_bf16_roll(ContractArgs a, B
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
 }
 
+// Launches the instantiation for these operands, or, with occ set, stores
+// in *occ the blocks of that instantiation one CU holds at once (on entry
+// *occ != 0 asks for the one that writes statistics or bitmasks).
 template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
-hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s) {
-    const int blocks = L.per_xcd * 8;
+hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s, int* occ) {
     constexpr bool ROLL = !PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK;
-    if constexpr (ROLL) {
-        if (a.stats || a.bits)
-            hipLaunchKernelGGL((k_contract_bf16_roll<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s,
-                               a, L);
-        else
-            hipLaunchKernelGGL((k_contract_bf16_roll<KK, R, PP, SIGN, FT, false, PIPE>), dim3(blocks), dim3(256), 0,
-                               s, a, L);
-    } else {
-        if (a.stats || a.bits)
-            hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, true, PIPE>), dim3(blocks), dim3(256), 0, s, a,
-                               L);
-        else
-            hipLaunchKernelGGL((k_contract_bf16<KK, R, PP, SIGN, FT, false, PIPE>), dim3(blocks), dim3(256), 0, s,
-                               a, L);
-    }
+    const bool out = occ ? *occ != 0 : (a.stats || a.bits);
+    void (*kern)(ContractArgs, Bf16Launch);
+    if constexpr (ROLL)
+        kern = out ? k_contract_bf16_roll<KK, R, PP, SIGN, FT, true, PIPE> : k_contract_bf16_roll<KK, R, PP, SIGN, FT, false, PIPE>;
+    else
+        kern = out ? k_contract_bf16<KK, R, PP, SIGN, FT, true, PIPE> : k_contract_bf16<KK, R, PP, SIGN, FT, false, PIPE>;
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, reinterpret_cast<const void*>(kern), 256, 0);
+    hipLaunchKernelGGL(kern, dim3(L.per_xcd * 8), dim3(256), 0, s, a, L);
     return hipGetLastError();
 }
 
 template <int KK, int PP>
-hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool sign, hipStream_t s) {
+hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool sign, hipStream_t s, int* occ) {
     constexpr int RS = bf16_rows_per_wave(KK, true);
     constexpr int R = bf16_rows_per_wave(KK, false);
     constexpr bool PIPE = bf16_pipelined(KK, false);
     if (sign) {
         if (a.ftest || PP != 1) return hipErrorInvalidValue;
-        if constexpr (PP == 1) return run_bf16<KK, RS, 1, true, false, bf16_pipelined(KK, true)>(a, L, s);
+        if constexpr (PP == 1) return run_bf16<KK, RS, 1, true, false, bf16_pipelined(KK, true)>(a, L, s, occ);
     }
-    return a.ftest ? run_bf16<KK, R, PP, false, true, PIPE>(a, L, s) : run_bf16<KK, R, PP, false, false, PIPE>(a, L, s);
+    return a.ftest ? run_bf16<KK, R, PP, false, true, PIPE>(a, L, s, occ)
+                   : run_bf16<KK, R, PP, false, false, PIPE>(a, L, s, occ);
 }
 
 template <int KK>
-hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s) {
+hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s,
+                            int* occ) {
     switch (PP) {
-        case 1: return dispatch_bf16_kind<KK, 1>(a, L, sign, s);
-        case 2: return dispatch_bf16_kind<KK, 2>(a, L, sign, s);
-        case 3: return dispatch_bf16_kind<KK, 3>(a, L, sign, s);
-        case 4: return dispatch_bf16_kind<KK, 4>(a, L, sign, s);
-        case 5: return dispatch_bf16_kind<KK, 5>(a, L, sign, s);
-        case 8: return dispatch_bf16_kind<KK, 8>(a, L, sign, s);
-        case 16: return dispatch_bf16_kind<KK, 16>(a, L, sign, s);
+        case 1: return dispatch_bf16_kind<KK, 1>(a, L, sign, s, occ);
+        case 2: return dispatch_bf16_kind<KK, 2>(a, L, sign, s, occ);
+        case 3: return dispatch_bf16_kind<KK, 3>(a, L, sign, s, occ);
+        case 4: return dispatch_bf16_kind<KK, 4>(a, L, sign, s, occ);
+        case 5: return dispatch_bf16_kind<KK, 5>(a, L, sign, s, occ);
+        case 8: return dispatch_bf16_kind<KK, 8>(a, L, sign, s, occ);
+        case 16: return dispatch_bf16_kind<KK, 16>(a, L, sign, s, occ);
     }
     return hipErrorInvalidValue;
 }

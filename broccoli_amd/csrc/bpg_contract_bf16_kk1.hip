@@ -3,8 +3,8 @@
 #include "bpg_contract_bf16.h"
 
 namespace bpg {
-template hipError_t dispatch_bf16_pp<1>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t);
-template hipError_t dispatch_bf16_pp<2>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t);
-template hipError_t dispatch_bf16_pp<3>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t);
-template hipError_t dispatch_bf16_pp<4>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t);
+template hipError_t dispatch_bf16_pp<1>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
+template hipError_t dispatch_bf16_pp<2>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
+template hipError_t dispatch_bf16_pp<3>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
+template hipError_t dispatch_bf16_pp<4>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
 }  // namespace bpg

@@ -118,9 +118,14 @@ constexpr int bf16_y_pieces(int KK, bool sign) { return sign && KK <= 4 ? 3 : 2;
 constexpr bool bf16_pipelined(int KK, bool sign) {
     return !sign && (KK <= 4 ? BPG_PIPE_SMALL : (KK <= 8 ? BPG_PIPE_MID : BPG_PIPE_LARGE)) != 0;
 }
-Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign);
+// slots_per_xcd: blocks of the contraction that one XCD holds at once
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd);
+// with occ set, stores the blocks per CU of the instantiation instead of
+// launching (see run_bf16)
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
-                                bool sign_operand, hipStream_t s);
+                                bool sign_operand, hipStream_t s, int* occ = nullptr);
+// blocks of the contraction instantiation for these operands per CU
+int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out);
 
 // Grid of the FP32 contraction: blocks of PTT permutations over chunks of
 // 256-voxel steps.

@@ -263,46 +263,71 @@ __device__ __forceinline__ float stat_f(float num, float sse, float f_scale) {
 // split-BF16 MFMA contraction: bpg_contract_bf16.h, instantiated per KK in
 // bpg_contract_bf16_*.hip
 template <int KK>
-hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s);
+hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s, int* occ);
 
 
-Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign) {
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd) {
     Bf16Launch L{};
     const int R = bf16_rows_per_wave(KK, sign);
     L.n_sub = (K + 2 * (16 / PP) - 1) / (2 * (16 / PP));
     L.n_vt = Vpad / 32;
     L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);
-    // voxel chunks: enough blocks to cover the 256 CUs several times, at
-    // most BPG_MAX_VCHUNKS partial maxima per permutation
-    int nvc = (BPG_TARGET_BLOCKS + L.n_mgroups - 1) / L.n_mgroups;
-    if (nvc > BPG_MAX_VCHUNKS) nvc = BPG_MAX_VCHUNKS;
-    if (nvc > L.n_vt) nvc = L.n_vt;
-    if (nvc < 1) nvc = 1;
-    L.vt_per_chunk = (L.n_vt + nvc - 1) / nvc;
-    L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
+    // Voxel chunks.  Blocks of one chunk length run in rounds of
+    // slots_per_xcd on each XCD, and the last round of a grid that does not
+    // fill it leaves most of the XCD idle for a whole block (1/17 of the
+    // kernel with 4116 blocks of one block per CU).  Among the chunk counts
+    // that give at least BPG_TARGET_BLOCKS/2 blocks (enough for the dispatcher
+    // to even out blocks that run slower), and at most BPG_MAX_VCHUNKS partial
+    // maxima per permutation, take the one with the fewest rounds x chunk
+    // length, counting a block's operand loads as 2 voxel tiles.
+    const int slots = slots_per_xcd > 0 ? slots_per_xcd : 1;
+    const int maxc = std::max(1, std::min(BPG_MAX_VCHUNKS, L.n_vt));
+    long best = -1;
+    int len = std::max(1, L.n_vt);
+    for (int nvc = 1; nvc <= maxc; nvc++) {
+        if ((long)L.n_mgroups * nvc < BPG_TARGET_BLOCKS / 2 && nvc < maxc) continue;
+        const int s = std::max(1, (L.n_vt + nvc - 1) / nvc);
+        const long blocks = (long)L.n_mgroups * ((L.n_vt + s - 1) / s);
+        const long rounds = ((blocks + 7) / 8 + slots - 1) / slots;
+        const long cost = rounds * (s + 2);
+        if (best < 0 || cost < best) {
+            best = cost;
+            len = s;
+        }
+    }
+    L.vt_per_chunk = len;
+    L.n_vchunks = (L.n_vt + len - 1) / len;
     L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
     return L;
 }
 
+int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out) {
+    ContractArgs a{};
+    a.ftest = ftest;
+    int occ = out ? 1 : 0;
+    if (launch_contract_bf16(a, Bf16Launch{}, KK, PP, sign, nullptr, &occ) != hipSuccess || occ < 1) return 1;
+    return occ;
+}
+
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
-                                bool sign_operand, hipStream_t s) {
+                                bool sign_operand, hipStream_t s, int* occ) {
     switch (KK) {
-        case 1: return dispatch_bf16_pp<1>(a, L, PP, sign_operand, s);
-        case 2: return dispatch_bf16_pp<2>(a, L, PP, sign_operand, s);
-        case 3: return dispatch_bf16_pp<3>(a, L, PP, sign_operand, s);
-        case 4: return dispatch_bf16_pp<4>(a, L, PP, sign_operand, s);
-        case 5: return dispatch_bf16_pp<5>(a, L, PP, sign_operand, s);
-        case 6: return dispatch_bf16_pp<6>(a, L, PP, sign_operand, s);
-        case 7: return dispatch_bf16_pp<7>(a, L, PP, sign_operand, s);
-        case 8: return dispatch_bf16_pp<8>(a, L, PP, sign_operand, s);
-        case 9: return dispatch_bf16_pp<9>(a, L, PP, sign_operand, s);
-        case 10: return dispatch_bf16_pp<10>(a, L, PP, sign_operand, s);
-        case 11: return dispatch_bf16_pp<11>(a, L, PP, sign_operand, s);
-        case 12: return dispatch_bf16_pp<12>(a, L, PP, sign_operand, s);
-        case 13: return dispatch_bf16_pp<13>(a, L, PP, sign_operand, s);
-        case 14: return dispatch_bf16_pp<14>(a, L, PP, sign_operand, s);
-        case 15: return dispatch_bf16_pp<15>(a, L, PP, sign_operand, s);
-        case 16: return dispatch_bf16_pp<16>(a, L, PP, sign_operand, s);
+        case 1: return dispatch_bf16_pp<1>(a, L, PP, sign_operand, s, occ);
+        case 2: return dispatch_bf16_pp<2>(a, L, PP, sign_operand, s, occ);
+        case 3: return dispatch_bf16_pp<3>(a, L, PP, sign_operand, s, occ);
+        case 4: return dispatch_bf16_pp<4>(a, L, PP, sign_operand, s, occ);
+        case 5: return dispatch_bf16_pp<5>(a, L, PP, sign_operand, s, occ);
+        case 6: return dispatch_bf16_pp<6>(a, L, PP, sign_operand, s, occ);
+        case 7: return dispatch_bf16_pp<7>(a, L, PP, sign_operand, s, occ);
+        case 8: return dispatch_bf16_pp<8>(a, L, PP, sign_operand, s, occ);
+        case 9: return dispatch_bf16_pp<9>(a, L, PP, sign_operand, s, occ);
+        case 10: return dispatch_bf16_pp<10>(a, L, PP, sign_operand, s, occ);
+        case 11: return dispatch_bf16_pp<11>(a, L, PP, sign_operand, s, occ);
+        case 12: return dispatch_bf16_pp<12>(a, L, PP, sign_operand, s, occ);
+        case 13: return dispatch_bf16_pp<13>(a, L, PP, sign_operand, s, occ);
+        case 14: return dispatch_bf16_pp<14>(a, L, PP, sign_operand, s, occ);
+        case 15: return dispatch_bf16_pp<15>(a, L, PP, sign_operand, s, occ);
+        case 16: return dispatch_bf16_pp<16>(a, L, PP, sign_operand, s, occ);
     }
     return hipErrorInvalidValue;
 }

@@ -17,6 +17,7 @@ struct bpg_ctx {
     std::string name;
     std::string arch;
     size_t lds_per_block = 0;  // shared memory one block may use
+    int cus = 256;             // compute units (8 XCDs on gfx950)
 };
 
 namespace bpg {
