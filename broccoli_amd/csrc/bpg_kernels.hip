@@ -744,6 +744,7 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
 
 // ---- whole-map CCL in LDS ----
 constexpr int CCL_LDS_THREADS = 1024;
+constexpr int CCL_BATCH = 4;  // active voxels whose neighbour rows load together
 constexpr size_t CCL_LDS_BYTES = 160 * 1024;  // LDS of one CU (gfx950)
 
 __device__ __forceinline__ unsigned lds_find(unsigned* p, unsigned x) {
@@ -834,17 +835,45 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     auto active = [&](int n) { return n >= 0 && ((sbits[n >> 5] >> (n & 31)) & 1u); };
     auto rank = [&](int n) { return srank[n >> 5] + __popc(sbits[n >> 5] & ((1u << (n & 31)) - 1u)); };
     const int nb6 = a.connectivity == 6;
-    for (int w = tid; w < nw; w += CCL_LDS_THREADS) {
-        unsigned word = sbits[w];
-        while (word) {
+    // A thread walks the active voxels of its words (tid, tid + 1024, ...)
+    // CCL_BATCH at a time: the neighbour rows of a batch are loaded together,
+    // so one memory latency serves CCL_BATCH voxels.  Linking by the smaller
+    // root makes the result independent of the order of the unions.
+    int ww = tid;
+    unsigned word = ww < nw ? sbits[ww] : 0u;
+    bool more = true;
+    while (more) {
+        int cs[CCL_BATCH];
+        unsigned ls[CCL_BATCH];
+#pragma unroll
+        for (int u = 0; u < CCL_BATCH; u++) {
+            cs[u] = -1;
+            ls[u] = 0u;
+            while (more && !word) {
+                ww += CCL_LDS_THREADS;
+                if (ww < nw) word = sbits[ww];
+                else more = false;
+            }
+            if (!more) continue;
             const int b = __ffs(word) - 1;
             word &= word - 1;
-            const int c = w * 32 + b;
-            if (c >= a.Vc) break;
-            const unsigned lc = srank[w] + __popc(sbits[w] & ((1u << b) - 1u));
+            const int c = ww * 32 + b;
+            if (c >= a.Vc) {  // bits past the last voxel (none are set): stop
+                more = false;
+                continue;
+            }
+            cs[u] = c;
+            ls[u] = srank[ww] + __popc(sbits[ww] & ((1u << b) - 1u));
+        }
+        int ncs[CCL_BATCH][BPG_NBR];
+#pragma unroll
+        for (int u = 0; u < CCL_BATCH; u++) load_nbr(a.nbr, max(cs[u], 0), ncs[u]);  // row 0 for empty slots
+#pragma unroll
+        for (int u = 0; u < CCL_BATCH; u++) {
+            if (cs[u] < 0) continue;
+            const unsigned lc = ls[u];
+            const int* nc = ncs[u];
             // the same implied-link rules as the global path
-            int nc[BPG_NBR];
-            load_nbr(a.nbr, c, nc);
             const int nx = nc[0], ny = nc[1], nz = nc[2];
             if (active(nx)) lds_union(spar, lc, rank(nx));
             const bool ay = active(ny), az = active(nz);
