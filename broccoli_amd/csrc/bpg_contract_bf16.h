@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launc
 // The rolling schedule (two data fragments, KK 5..8), held to 256 registers
 // so that two waves share each SIMD
 template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
-__global__ __launch_bounds__(256, 2) void k_contract_bf16_roll(ContractArgs a, Bf16Launch L) {
+__global__ __launch_bounds__(256, R >= 3 ? 1 : 2) void k_contract_bf16_roll(ContractArgs a, Bf16Launch L) {
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
 }
 

@@ -20,6 +20,10 @@
 #ifndef BPG_R2_KK
 #define BPG_R2_KK 16
 #endif
+// operand subtiles per wave at KK 5..8 (the rolling schedule)
+#ifndef BPG_R_MID
+#define BPG_R_MID 2
+#endif
 #ifndef BPG_PIPE_SMALL
 #define BPG_PIPE_SMALL 1
 #endif
@@ -109,7 +113,7 @@ struct Bf16Launch {
 };
 // operand subtiles per wave (the planner and the kernels share this rule)
 constexpr int bf16_rows_per_wave(int KK, bool sign) {
-    return KK <= 4 ? (sign ? BPG_R_SIGN : BPG_R_SMALL) : (KK <= BPG_R2_KK ? 2 : 1);
+    return KK <= 4 ? (sign ? BPG_R_SIGN : BPG_R_SMALL) : (KK <= 8 ? BPG_R_MID : (KK <= BPG_R2_KK ? 2 : 1));
 }
 // data fragments per subject: hi + mid + lo for exact +-1 operands at
 // KK <= 4 (every product exact, so the FP32 form's conditioning bound
