@@ -13,9 +13,11 @@ uploaded once before timing.
 
 Printed JSON line: metric/value for the split-BF16 MFMA form (the headline),
 the plain FP32 form measured in the same run, accuracy against the FP64
-oracle on sampled permutations, the roofline of the dominant kernel (its
-launch time from HIP events on the plan's stream), and the reference
-algorithm restated on the host cores (cpu_baseline).
+oracle (every statistic of sampled permutations; the quantised max-T of all
+K permutations and the p-value counts of the observed map, with their ties),
+the roofline of the dominant kernel (its launch time from HIP events on the
+plan's stream), and the reference algorithm restated on the host cores
+(cpu_baseline).
 """
 from __future__ import annotations
 
@@ -206,13 +208,22 @@ def run_first_level(args):
     print(json.dumps(res))
 
 
+def host_cores():
+    """Host cores this process may use: its affinity set, capped by
+    OMP_NUM_THREADS when that is set (the GPU box sets it to the CPU share
+    of one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
 def cpu_baseline_first_level(st, perm, target_s):
     """The oracle's FP64 restatement of the first-level loop (inverse
     whitening of the whole grid, then the statistic map, OpenMP over the
     host cores) on a bounded sample of the permutations."""
     from oracle import oracle
 
-    ncores = min(len(os.sched_getaffinity(0)), 16)
+    ncores = host_cores()
     os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
     wh, ar = oracle.fl_prepare(st["Y"], st["mask"], st["X"], st.get("fwhm", 8.0), st["voxel"])
     V = int((st["mask"] == 1).sum())
@@ -276,10 +287,50 @@ def accuracy(ctx, st, perms, nsample):
             md, stats = plan.permutation_phase(0, want_stats=True, **sub)
         ref = oracle.perm_stats(st, sub, nsample)[:, st["mask"].ravel() == 1.0]
         err = np.abs(stats - ref) / (1.0 + np.abs(ref))
-        q64 = np.trunc(ref.max(axis=1) * 1e4)
-        qg = np.round(md.astype(np.float64) * 1e4)
+        q64 = oracle.quantised_max(ref.max(axis=1))
+        qg = md
         out[prec] = dict(max_err=float(err.max()), sampled_perms=nsample,
                          maxT_quanta_differing=int((q64 != qg).sum()))
+    return out
+
+
+def full_k_check(st, perms, K, md, md32, obs, P, ties, ncores):
+    """Every permutation's quantised max-T against the FP64 oracle's, and the
+    p-value counts of the observed map against those of the FP64 null.
+
+    A quantum may differ only inside the accuracy gate's band: the GPU
+    maximum is within 1e-4 (1 + |m|) of the FP64 one, so its quantum lies in
+    [trunc((m64 - b) 1e4), trunc((m64 + b) 1e4)].  A count may differ only
+    by comparisons the library reports as ties (null values within
+    1e-4 (2 + |T|) of T)."""
+    from oracle import oracle
+
+    t0 = time.perf_counter()
+    m64 = oracle.perm_max64(st, perms, K, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    # the reference's stored null value of each FP64 maximum, and of the
+    # edges of the accuracy gate's band around it (a monotone map)
+    q64 = oracle.quantised_max(m64)
+    band = 1e-4 * (1.0 + np.abs(m64))
+    lo = oracle.quantised_max(m64 - band)
+    hi = oracle.quantised_max(m64 + band)
+    out = {"permutations": int(K), "oracle_seconds": dt, "oracle_cores": ncores}
+    for name, m in (("split_bf16", md), ("fp32", md32)):
+        if m is None:
+            continue
+        d = m != q64
+        out[name] = {"maxT_quanta_differing": int(d.sum()),
+                     "maxT_outside_band": int((d & ((m < lo) | (m > hi))).sum()),
+                     "max_abs_diff": float(np.abs(m.astype(np.float64) - q64).max())}
+    mask = st["mask"].ravel() == 1.0
+    T = obs[0][mask]
+    cg = np.round(P[0][mask].astype(np.float64) * K).astype(np.int64)
+    c64 = np.searchsorted(np.sort(q64), T, side="left")
+    tv = ties[0][mask].astype(np.int64)
+    dc = np.abs(cg - c64)
+    out["pvalues"] = {"voxels": int(mask.sum()), "counts_differing": int((dc > 0).sum()),
+                      "counts_differing_outside_ties": int((dc > tv).sum()),
+                      "ties": int(tv.sum()), "voxels_with_ties": int((tv > 0).sum())}
     return out
 
 
@@ -289,7 +340,7 @@ def cpu_baseline(st, perms, target_s):
     workload's permutations."""
     from oracle import oracle
 
-    ncores = min(len(os.sched_getaffinity(0)), 16)
+    ncores = host_cores()
     Ystar = oracle.transformed32(st, 0)
     sub = {k: v for k, v in perms.items() if k in ("perm", "sign")}
     one = {k: v[:2] for k, v in sub.items()}
@@ -364,6 +415,11 @@ def main():
     print(f"# study ready: V={V} N={N} p={p} K={K} device={ctx.device_name}", file=sys.stderr, flush=True)
 
     plan, md, dt, kern_ms, tm = run_form(ctx, st, perms, "split_bf16", args.steps, args.warmup)
+    # the observed map and its p-values (with ties) against this null
+    t0 = time.perf_counter()
+    obs = plan.observed_maps()
+    Pv, pties = plan.pvalues([md], want_ties=True)
+    pval_ms = (time.perf_counter() - t0) * 1e3
     plan.close()
     n_products = 2 if ("sign" in perms and st["N"] > 64) else 3
     print(f"# split_bf16: {dt*1e3:.3f} ms/step, contraction {kern_ms:.3f} ms", file=sys.stderr, flush=True)
@@ -397,10 +453,14 @@ def main():
                      "split_form_peak": PEAK_BF16_TFLOPS / n_products,
                      "frac_of_split_form_peak": achieved / (PEAK_BF16_TFLOPS / n_products)},
         "fit_columns": tm.fit_columns,
-        "precision_used": "split_bf16" if tm.precision_used == capi.PREC_SPLIT_BF16 else "fp32",
+        "precision_used": {capi.PREC_SPLIT_BF16: "split_bf16", capi.PREC_FP32: "fp32",
+                           capi.PREC_FP64: "fp64"}[tm.precision_used],
         "ill_conditioned_waves": tm.unsafe_waves,
         "permutations_repeated_in_fp32": tm.fp32_permutations,
+        "permutations_repeated_in_fp64": tm.fp64_permutations,
+        "observed_and_pvalues_ms": pval_ms,
     }
+    md32 = None
     if not args.no_fp32:
         plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, min(10, args.steps // 2)), 1)
         plan.close()
@@ -412,6 +472,14 @@ def main():
         res["maxT_equal_fp32_vs_bf16"] = int(np.sum(md32 == md))
     if not args.no_accuracy:
         res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8)
+        if args.workload in (1, 2):
+            full = full_k_check(st, perms, K, md, md32, obs, Pv, pties, host_cores())
+            res["accuracy_vs_fp64"]["full_K"] = full
+            res["maxT_quanta_differing"] = full["split_bf16"]["maxT_quanta_differing"]
+            res["maxT_outside_band"] = full["split_bf16"]["maxT_outside_band"]
+            res["pvalue_ties"] = full["pvalues"]["ties"]
+            res["pvalue_counts_differing_outside_ties"] = full["pvalues"]["counts_differing_outside_ties"]
+            print(f"# full K vs FP64: {full}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(st, perms, args.cpu_seconds)
     ctx.close()

@@ -22,7 +22,7 @@ VOXEL, CLUSTER_EXTENT, CLUSTER_MASS, TFCE = 0, 1, 2, 3
 TWOSAMPLE, CORRELATION, MEAN_DESIGN = 0, 1, 2
 PREC_FP32, PREC_SPLIT_BF16, PREC_FP64, PREC_SPLIT_F16 = 0, 1, 2, 3
 FL_PRECISIONS = {"split_f16": PREC_SPLIT_F16, "split_bf16": PREC_SPLIT_BF16, "fp32": PREC_FP32, "fp64": PREC_FP64}
-PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16}
+PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp64": PREC_FP64}
 
 OK = 0
 ERRORS = {-1: "BPG_ERR_ARG", -2: "BPG_ERR_HIP", -3: "BPG_ERR_NOMEM", -4: "BPG_ERR_DESIGN",
@@ -31,10 +31,10 @@ ERRORS = {-1: "BPG_ERR_ARG", -2: "BPG_ERR_HIP", -3: "BPG_ERR_NOMEM", -4: "BPG_ER
 # every symbol include/bpg.h declares
 EXPORTS = [
     "bpg_create", "bpg_destroy", "bpg_last_error", "bpg_device_name",
-    "bpg_plan_create", "bpg_plan_destroy", "bpg_plan_num_voxels",
+    "bpg_plan_create", "bpg_validate_study", "bpg_plan_destroy", "bpg_plan_num_voxels",
     "bpg_plan_permutation_phase", "bpg_plans_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
     "bpg_plan_glm",
-    "bpg_plan_pvalues", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
+    "bpg_plan_pvalues", "bpg_plan_pvalues_ties", "bpg_plan_last_tie_flags", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
     "bpg_gen_perm_correlation", "bpg_detect_designs", "bpg_max_permutations",
     "bpg_model_precompute",
     "bpg_gen_perm_first_level", "bpg_fl_plan_create", "bpg_fl_plan_destroy", "bpg_fl_plan_num_voxels",
@@ -63,7 +63,8 @@ class Timing(ctypes.Structure):
                 ("contract_ms", c_double), ("cluster_ms", c_double),
                 ("contract_launches", c_int), ("perm_voxels", c_long),
                 ("fit_columns", c_int), ("slots_per_perm", c_int), ("precision_used", c_int),
-                ("unsafe_waves", c_int), ("fp32_permutations", c_long)]
+                ("unsafe_waves", c_int), ("fp32_permutations", c_long), ("fp64_permutations", c_long),
+                ("cdt_tie_maps", c_long), ("pvalue_ties", c_long)]
 
 
 class BpgError(RuntimeError):
@@ -92,6 +93,7 @@ def load() -> ctypes.CDLL:
         "bpg_device_name": (ctypes.c_char_p, [vp]),
         "bpg_plan_create": (c_int, [vp, POINTER(StudyDesc), POINTER(c_void_p)]),
         "bpg_plan_destroy": (None, [vp]),
+        "bpg_validate_study": (c_int, [POINTER(StudyDesc), ctypes.c_char_p, c_int]),
         "bpg_plan_num_voxels": (c_long, [vp]),
         "bpg_plan_permutation_phase": (c_int, [vp, c_int, vp, vp, c_long, vp, vp]),
         "bpg_plans_permutation_phase": (c_int, [vp, c_int, vp, vp, vp, vp, vp]),
@@ -99,6 +101,8 @@ def load() -> ctypes.CDLL:
         "bpg_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
         "bpg_plan_observed_maps": (c_int, [vp, vp]),
         "bpg_plan_pvalues": (c_int, [vp, vp, vp, vp, vp]),
+        "bpg_plan_pvalues_ties": (c_int, [vp, vp, vp, vp, vp, vp]),
+        "bpg_plan_last_tie_flags": (c_int, [vp, vp]),
         "bpg_gen_sign_matrix": (c_int, [c_int, c_long, c_int, c_ulonglong, vp]),
         "bpg_gen_perm_two_sample": (c_int, [c_int, c_int, c_int, c_long, c_int, c_ulonglong, vp]),
         "bpg_gen_perm_correlation": (c_int, [c_int, c_long, c_int, c_ulonglong, vp]),
@@ -212,6 +216,31 @@ class Context:
         self.close()
 
 
+def _study_desc(s: dict, precision):
+    """bpg_study_desc of a study dict, and the arrays it points into."""
+    keep = (np.ascontiguousarray(s["Y"], np.float32), np.ascontiguousarray(s["mask"], np.float32),
+            np.asfortranarray(s["X"], np.float32).ravel(order="F").copy(),
+            np.ascontiguousarray(s["C"], np.float32).ravel())
+    d = StudyDesc()
+    d.test = s["test"]
+    d.inference = s.get("inference", VOXEL)
+    d.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+    d.connectivity = s.get("connectivity", 26)
+    d.W, d.H, d.D = s["W"], s["H"], s["D"]
+    d.N, d.p, d.nc = s["N"], s["p"], s["nc"]
+    d.Y, d.mask, d.X, d.C = (_ptr(a) for a in keep)
+    d.cdt = s.get("cdt", 0.0)
+    return d, keep
+
+
+def validate_study(study: dict, precision: str = "split_bf16"):
+    """bpg_validate_study: (status, message); no device needed."""
+    d, keep = _study_desc(study, precision)
+    buf = ctypes.create_string_buffer(256)
+    rc = load().bpg_validate_study(ctypes.byref(d), buf, 256)
+    return rc, buf.value.decode()
+
+
 class Plan:
     """A device-resident study (bpg_plan).
 
@@ -225,19 +254,7 @@ class Plan:
         self._lib = ctx._lib
         self.study = study
         s = study
-        self._Y = np.ascontiguousarray(s["Y"], np.float32)
-        self._mask = np.ascontiguousarray(s["mask"], np.float32)
-        self._X = np.asfortranarray(s["X"], np.float32).ravel(order="F").copy()
-        self._C = np.ascontiguousarray(s["C"], np.float32).ravel()
-        d = StudyDesc()
-        d.test = s["test"]
-        d.inference = s.get("inference", VOXEL)
-        d.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
-        d.connectivity = s.get("connectivity", 26)
-        d.W, d.H, d.D = s["W"], s["H"], s["D"]
-        d.N, d.p, d.nc = s["N"], s["p"], s["nc"]
-        d.Y, d.mask, d.X, d.C = _ptr(self._Y), _ptr(self._mask), _ptr(self._X), _ptr(self._C)
-        d.cdt = s.get("cdt", 0.0)
+        d, self._keep = _study_desc(study, precision)
         self.desc = d
         h = c_void_p()
         _check(self._lib.bpg_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), ctx.handle)
@@ -286,16 +303,35 @@ class Plan:
         _check(self._lib.bpg_plan_glm(self.handle, _ptr(b), _ptr(m), _ptr(r), _ptr(v)), self.ctx.handle)
         return b.reshape(p, self.G), m.reshape(self.nmaps, self.G), r.reshape(N, self.G), v
 
-    def pvalues(self, maxdists, want_labels: bool = False):
+    def pvalues(self, maxdists, want_labels: bool = False, want_ties: bool = False):
+        """P (nmaps, G); with want_labels also the cluster labels and with
+        want_ties the per-voxel tie counts (bpg_plan_pvalues_ties), in that
+        order after P."""
         Ks = np.array([len(m) for m in maxdists], np.int64)
         arrs = [np.ascontiguousarray(m, np.float32) for m in maxdists]
         ptrs = (c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
         P = np.zeros(self.nmaps * self.G, np.float32)
         labels = np.zeros(self.nmaps * self.G, np.uint32) if want_labels else None
-        _check(self._lib.bpg_plan_pvalues(self.handle, _ptr(Ks), ctypes.cast(ptrs, c_void_p), _ptr(P),
-                                          _ptr(labels)), self.ctx.handle)
-        P = P.reshape(self.nmaps, self.G)
-        return (P, labels.reshape(self.nmaps, self.G)) if want_labels else P
+        ties = np.zeros(self.nmaps * self.G, np.uint32) if want_ties else None
+        if want_ties:
+            _check(self._lib.bpg_plan_pvalues_ties(self.handle, _ptr(Ks), ctypes.cast(ptrs, c_void_p), _ptr(P),
+                                                   _ptr(labels), _ptr(ties)), self.ctx.handle)
+        else:
+            _check(self._lib.bpg_plan_pvalues(self.handle, _ptr(Ks), ctypes.cast(ptrs, c_void_p), _ptr(P),
+                                              _ptr(labels)), self.ctx.handle)
+        out = [P.reshape(self.nmaps, self.G)]
+        if want_labels:
+            out.append(labels.reshape(self.nmaps, self.G))
+        if want_ties:
+            out.append(ties.reshape(self.nmaps, self.G))
+        return tuple(out) if len(out) > 1 else out[0]
+
+    def last_tie_flags(self, K: int) -> np.ndarray:
+        """Per permutation of the last phase: 1 when its map had a statistic
+        within the error band of the cluster-defining threshold."""
+        out = np.zeros(K, np.uint8)
+        _check(self._lib.bpg_plan_last_tie_flags(self.handle, _ptr(out)))
+        return out
 
     def close(self):
         if self.handle:

@@ -651,7 +651,17 @@ int main(int argc, char** argv) {
         B.SetStatisticalTest(1);
         B.PerformGLMFTestSecondLevelPermutationWrapper();
     }
-    if (verbose) printf("\nIt took %f seconds to run the permutation test\n", (float)(wall() - t0));
+    if (verbose) {
+        printf("\nIt took %f seconds to run the permutation test\n", (float)(wall() - t0));
+        long pties = 0, cties = 0, r32 = 0, r64 = 0;
+        B.GetPermutationTies(&pties, &cties);
+        B.GetPermutationRepeats(&r32, &r64);
+        printf("Permutation p-value comparisons inside the 1e-4 error band (ties): %ld\n", pties);
+        if (inference != 0 && inference != 3)
+            printf("Permutations with a statistic within the error band of the cluster defining threshold: %ld\n",
+                   cties);
+        printf("Permutations repeated for ill-conditioned samples: %ld in FP32, %ld in FP64\n", r32, r64);
+    }
     bool failed = false;
     {
         const int* e = B.GetOpenCLCreateBufferErrors();

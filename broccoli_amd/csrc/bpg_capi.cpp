@@ -55,14 +55,21 @@ struct bpg_plan {
     hipEvent_t ev[6] = {};
     bpg_timing timing{};
     bool have_obs = false;
+    // cluster inference: per map, the tied maps of its last permutation phase
+    // (cdt_tie_maps), and the per-permutation tie flags of the last phase
+    std::vector<long> map_ties, last_K;
+    std::vector<unsigned char> last_ties;
     // the phase enqueued on `stream` and not yet finished
     struct Pending {
         bool active = false;
         long K = 0;
-        bool cluster = false, mass = false, bf = false, tfce = false;
+        bool cluster = false, mass = false, tfce = false;
+        int form = BPG_PREC_FP32;  // arithmetic of this run (BPG_PREC_*)
+        bool exact = false;        // split-BF16 with every product exact (three data parts)
         int map = 0;
-        long fp32_perms = 0;  // permutations repeated in FP32
-        bool have_times = false;  // kernel times kept from before a partial repeat
+        long fp32_perms = 0;  // permutations whose results come from the FP32 form
+        long fp64_perms = 0;  // ... from the FP64 form
+        bool have_times = false;  // kernel times kept from before a repeat
         double t_transform = 0, t_operand = 0, t_contract = 0, t_cluster = 0;
         const unsigned short* perm = nullptr;
         const float* sign = nullptr;
@@ -76,6 +83,9 @@ struct bpg_plan {
     DBuf<double> yy, Qt, Qn, dscale;
     DBuf<unsigned> Yb, Gb;
     DBuf<float> Y32, G32;
+    DBuf<double> Y64;
+    DBuf<unsigned char> ties;
+    HBuf<unsigned char> h_ties;
     DBuf<unsigned short> perm;
     DBuf<float> sign;
     DBuf<float> partial, maxout;
@@ -89,6 +99,7 @@ struct bpg_plan {
     DBuf<unsigned char> flags;
     DBuf<double> oX, oXp, oC, osc;
     DBuf<float> d_obs, sorted, P, glm_out;
+    DBuf<unsigned> d_pties;
     HBuf<double> h_model;
     HBuf<unsigned short> h_perm;
     HBuf<float> h_sign, h_max;
@@ -98,13 +109,14 @@ struct bpg_plan {
 
     void release_all() {
         h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release(); h_unsafe.release(); h_flags.release();
+        h_ties.release(); Y64.release(); ties.release();
         Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
         cl.release(); maxsize.release(); labels.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
-        d_obs.release(); sorted.release(); P.release(); glm_out.release();
+        d_obs.release(); sorted.release(); P.release(); glm_out.release(); d_pties.release();
     }
 };
 
@@ -146,28 +158,85 @@ extern "C" const char* bpg_last_error(const bpg_ctx* ctx) { return ctx ? ctx->er
 
 extern "C" const char* bpg_device_name(const bpg_ctx* ctx) { return ctx ? ctx->name.c_str() : ""; }
 
+namespace {
+
+// The argument checks of bpg_plan_create (everything but the data).
+int check_study_args(const bpg_study_desc& d, std::string& err) {
+    auto fail = [&](const char* msg) { err = msg; return BPG_ERR_ARG; };
+    if (d.test != BPG_TTEST && d.test != BPG_FTEST && d.test != BPG_GROUP_MEAN) return fail("unknown statistical test");
+    if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS &&
+        d.inference != BPG_TFCE)
+        return fail("unknown inference mode");
+    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16 && d.precision != BPG_PREC_FP64)
+        return fail("unknown precision");
+    if (d.connectivity != 26 && d.connectivity != 6) return fail("connectivity must be 26 or 6");
+    if (d.W <= 0 || d.H <= 0 || d.D <= 0) return fail("grid dimensions must be positive");
+    if ((double)d.W * d.H * d.D * 3.0 >= 4294967295.0) return fail("grid too large for 32-bit labels");
+    if (!d.Y || !d.mask || !d.X || !d.C) return fail("null input pointer");
+    const int p = d.test == BPG_GROUP_MEAN ? 1 : d.p;
+    if (p < 1 || p > BPG_MAX_REGRESSORS) return fail("number of regressors must be 1..25");
+    if (d.nc < 1) return fail("number of contrasts must be positive");
+    if (d.test == BPG_FTEST && (d.nc > BPG_MAX_FROWS || d.nc > p)) return fail("F contrast must have 1..min(10, p) rows");
+    if (d.N <= p || d.N > BPG_MAX_SUBJECTS) return fail("need p < N <= 256 subjects");
+    if (d.test == BPG_FTEST && d.N <= p + 1) return fail("F test needs N > p + 1");
+    return BPG_OK;
+}
+
+// A NaN or infinite value inside the mask makes that voxel's statistic NaN
+// for every permutation.  The reference then converts NaN with
+// (int)(T * 1e4) (KM:299), whose result depends on the device (x86:
+// INT_MIN, which never wins the max; AMD GPUs: 0), so there is no result to
+// reproduce: such data are refused.  (Finite data whose residualised values
+// are all zero, e.g. a voxel constant over subjects with the intercept in
+// the nuisance space, give 0/0 = NaN as well; the max leaves those samples
+// out and they never exceed a threshold, which is what comparisons with
+// NaN give in the reference.)
+const char* kNonFinite = "the data hold a NaN or infinite value inside the mask";
+
+}  // namespace
+
+extern "C" int bpg_validate_study(const bpg_study_desc* desc, char* err, int err_len) {
+    std::string e;
+    int rc = BPG_ERR_ARG;
+    if (!desc) {
+        e = "null study";
+    } else if ((rc = check_study_args(*desc, e)) == BPG_OK) {
+        const bpg_study_desc& d = *desc;
+        const size_t G = (size_t)d.W * d.H * d.D;
+        std::vector<unsigned> c2g;
+        std::vector<int> g2c;
+        bpg::compact_mask(d.mask, G, c2g, g2c);
+        for (int s = 0; s < d.N && rc == BPG_OK; s++)
+            for (unsigned g : c2g)
+                if (!std::isfinite(d.Y[(size_t)s * G + g])) {
+                    e = kNonFinite;
+                    rc = BPG_ERR_ARG;
+                    break;
+                }
+        bpg_study_desc dm = d;
+        if (d.test == BPG_GROUP_MEAN) dm.p = dm.nc = 1;
+        const int nmaps = d.test == BPG_TTEST ? d.nc : 1;
+        for (int m = 0; m < nmaps && rc == BPG_OK; m++) {
+            bpg::MapModel mm;
+            rc = bpg::build_map_model(dm, m, mm, e);
+        }
+    }
+    if (err && err_len > 0) {
+        std::strncpy(err, e.c_str(), (size_t)err_len - 1);
+        err[err_len - 1] = '\0';
+    }
+    return rc;
+}
+
 extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_plan** out) {
     if (!ctx || !desc || !out) return BPG_ERR_ARG;
     *out = nullptr;
     const bpg_study_desc& d = *desc;
-    if (d.test != BPG_TTEST && d.test != BPG_FTEST && d.test != BPG_GROUP_MEAN)
-        BPG_FAIL(ctx, BPG_ERR_ARG, "unknown statistical test");
-    if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS &&
-        d.inference != BPG_TFCE)
-        BPG_FAIL(ctx, BPG_ERR_ARG, "unknown inference mode");
-    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16)
-        BPG_FAIL(ctx, BPG_ERR_ARG, "unknown precision");
-    if (d.connectivity != 26 && d.connectivity != 6) BPG_FAIL(ctx, BPG_ERR_ARG, "connectivity must be 26 or 6");
-    if (d.W <= 0 || d.H <= 0 || d.D <= 0) BPG_FAIL(ctx, BPG_ERR_ARG, "grid dimensions must be positive");
-    if ((double)d.W * d.H * d.D * 3.0 >= 4294967295.0) BPG_FAIL(ctx, BPG_ERR_ARG, "grid too large for 32-bit labels");
-    if (!d.Y || !d.mask || !d.X || !d.C) BPG_FAIL(ctx, BPG_ERR_ARG, "null input pointer");
+    {
+        const int rc = check_study_args(d, ctx->err);
+        if (rc != BPG_OK) return rc;
+    }
     const int p = d.test == BPG_GROUP_MEAN ? 1 : d.p;
-    if (p < 1 || p > BPG_MAX_REGRESSORS) BPG_FAIL(ctx, BPG_ERR_ARG, "number of regressors must be 1..25");
-    if (d.nc < 1) BPG_FAIL(ctx, BPG_ERR_ARG, "number of contrasts must be positive");
-    if (d.test == BPG_FTEST && (d.nc > BPG_MAX_FROWS || d.nc > p))
-        BPG_FAIL(ctx, BPG_ERR_ARG, "F contrast must have 1..min(10, p) rows");
-    if (d.N <= p || d.N > BPG_MAX_SUBJECTS) BPG_FAIL(ctx, BPG_ERR_ARG, "need p < N <= 256 subjects");
-    if (d.test == BPG_FTEST && d.N <= p + 1) BPG_FAIL(ctx, BPG_ERR_ARG, "F test needs N > p + 1");
 
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     bpg_plan* pl = new (std::nothrow) bpg_plan();
@@ -204,10 +273,19 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
     pl->Vpad = std::max(32, (pl->Vc + 255) / 256 * 256);
     pl->nwords = pl->Vpad / 32;
     std::vector<float> yc((size_t)d.N * std::max(pl->Vc, 1));
+    bool finite = true;
     for (int s = 0; s < d.N; s++) {
         const float* src = d.Y + (size_t)s * pl->G;
         float* dst = yc.data() + (size_t)s * pl->Vc;
-        for (int v = 0; v < pl->Vc; v++) dst[v] = src[pl->comp2grid[v]];
+        for (int v = 0; v < pl->Vc; v++) {
+            dst[v] = src[pl->comp2grid[v]];
+            finite &= std::isfinite(dst[v]);
+        }
+    }
+    if (!finite) {
+        ctx->err = kNonFinite;
+        delete pl;
+        return BPG_ERR_ARG;
     }
     auto fail = [&](int rc) { pl->release_all(); if (pl->stream) (void)hipStreamDestroy(pl->stream); delete pl; return rc; };
     if (hipStreamCreate(&pl->stream) != hipSuccess) { ctx->err = "hipStreamCreate failed"; return fail(BPG_ERR_HIP); }
@@ -252,8 +330,10 @@ namespace {
 // Everything of one permutation phase up to the copies of its results into
 // pinned host memory, enqueued on the plan's stream; phase_finish waits and
 // quantises.  Between the two the host is free (to enqueue other plans).
+// form: the arithmetic (BPG_PREC_SPLIT_BF16, _FP32, _FP64), or -1 for the
+// plan's own.
 int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float* sign, long K, float* stats,
-                  bool force_fp32 = false) {
+                  int form = -1) {
     bpg_ctx* ctx = pl->ctx;
     const bpg_study_desc& d = pl->d;
     if (map < 0 || map >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
@@ -283,6 +363,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ps.cluster = clu;
         ps.mass = d.inference == BPG_CLUSTER_MASS;
         ps.tfce = d.inference == BPG_TFCE;
+        ps.form = d.precision;
         ps.have_times = true;
         ps.pfit = m.pfit;
         ps.perm = perm;
@@ -293,7 +374,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     BPG_HIP(ctx, hipSetDevice(ctx->device));
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = pl->stream;
-    const bool bf = d.precision == BPG_PREC_SPLIT_BF16 && !force_fp32;
+    if (form < 0) form = d.precision;
+    const bool bf = form == BPG_PREC_SPLIT_BF16;
+    const bool f64 = form == BPG_PREC_FP64;
     const bool cluster = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
     // cluster mass and TFCE need the statistic of every voxel: the
     // contraction then also writes the statistic values
@@ -312,11 +395,14 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
                                          bpg::contract_bf16_blocks_per_cu(KK, PP, m.sign_operand,
                                                                           d.test == BPG_FTEST, want_stats || cluster))
            : bpg::Bf16Launch{};
-    const bpg::F32Launch Lf = bf ? bpg::F32Launch{} : bpg::plan_contract_f32((int)K, pfit, Vpad);
+    const bpg::F32Launch Lf = bf ? bpg::F32Launch{}
+                              : f64 ? bpg::plan_contract_f64((int)K, pfit, Vpad)
+                                    : bpg::plan_contract_f32((int)K, pfit, Vpad);
     const int n_sub = Lb.n_sub;
     const int n_part = bf ? Lb.n_vchunks : Lf.n_vchunks;
-    const int PT = bf ? 1 : bpg::f32_perm_tile(pfit);
-    const long Kpad = bf ? K : (K + PT - 1) / PT * PT;
+    const int PT = bf || f64 ? 1 : bpg::f32_perm_tile(pfit);
+    const long Kpad = (K + PT - 1) / PT * PT;
+    const int NS64 = (N + 7) / 8 * 8;  // FP64 data rows (zero beyond N)
 
     // buffers
     BPG_HIP(ctx, pl->yy.ensure(Vpad));
@@ -330,6 +416,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (bf) {
         BPG_HIP(ctx, pl->Yb.ensure((size_t)(Vpad / 32) * KK * npy * 64 * 4));
         BPG_HIP(ctx, pl->Gb.ensure((size_t)n_sub * KK * 2 * 64 * 4));
+    } else if (f64) {
+        BPG_HIP(ctx, pl->Y64.ensure((size_t)NS64 * Vpad));
     } else {
         BPG_HIP(ctx, pl->Y32.ensure((size_t)NS * Vpad));
         BPG_HIP(ctx, pl->G32.ensure((size_t)Kpad * pfit * NS));
@@ -342,6 +430,10 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     BPG_HIP(ctx, pl->h_unsafe.ensure(1));
     BPG_HIP(ctx, pl->flags.ensure(K));
     BPG_HIP(ctx, pl->h_flags.ensure(K));
+    if (cluster) {
+        BPG_HIP(ctx, pl->ties.ensure(K));
+        BPG_HIP(ctx, pl->h_ties.ensure(K));
+    }
     if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
@@ -356,7 +448,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     std::fill(hm + nQt, hm + nQt + nQn + NR, 0.0);
     for (int j = 0; j < m.q; j++)
         for (int s = 0; s < N; s++) hm[nQt + (size_t)s * QB + j] = m.Qn[(size_t)j * N + s];
-    std::memcpy(hm + nQt + nQn, m.dscale.data(), N * sizeof(double));
+    // the FP64 form reads y* itself (its operand carries the scale)
+    if (f64) std::fill(hm + nQt + nQn, hm + nQt + nQn + N, 1.0);
+    else std::memcpy(hm + nQt + nQn, m.dscale.data(), N * sizeof(double));
 
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
     BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, hm, nQt * sizeof(double), hipMemcpyHostToDevice, st));
@@ -366,8 +460,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         bpg::TransformArgs ta{};
         ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;
         ta.N = N; ta.q = m.q; ta.Vc = Vc; ta.Vpad = Vpad;
-        ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : NS; ta.npy = npy;
-        ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf ? nullptr : pl->Y32.p;
+        ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : f64 ? NS64 : NS; ta.npy = npy;
+        ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf || f64 ? nullptr : pl->Y32.p;
+        ta.Y64 = f64 ? pl->Y64.p : nullptr;
         BPG_HIP(ctx, bpg::launch_transform(ta, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
@@ -402,6 +497,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             oa.n_sub = n_sub;
             oa.Gb = pl->Gb.p;
             BPG_HIP(ctx, bpg::launch_operand_bf16(oa, st));
+        } else if (f64) {
+            // the FP64 kernel builds its operand tiles itself
         } else {
             oa.n_sub = (int)Kpad;
             oa.G32 = pl->G32.p;
@@ -412,7 +509,11 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     int launches = 0;
     {
         bpg::ContractArgs ca{};
-        ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p;
+        ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p; ca.Y64 = pl->Y64.p;
+        ca.perm = m.sign_operand ? nullptr : pl->perm.p;
+        ca.sign = m.sign_operand ? pl->sign.p : nullptr;
+        ca.Qt = pl->Qt.p;
+        ca.dof = (double)m.dof;
         ca.yy = pl->yy.p; ca.partial = pl->partial.p;
         ca.stats = want_stats ? pl->stats.p : nullptr;
         ca.bits = cluster ? pl->bits.p : nullptr;
@@ -427,9 +528,17 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.pinf = INFINITY;
         ca.unsafe = pl->unsafe.p;
         ca.flags = pl->flags.p;
+        // the band of the accuracy gate, 1e-4 (1 + |cdt|), around the
+        // threshold; none for the FP64 form
+        const float band = f64 ? 0.0f : (float)(1e-4 * (1.0 + std::fabs((double)d.cdt)));
+        ca.cdt_lo = d.cdt - band;
+        ca.cdt_hi = d.cdt + band;
+        ca.ties = cluster && !f64 ? pl->ties.p : nullptr;
         BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));
         BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
+        if (cluster) BPG_HIP(ctx, hipMemsetAsync(pl->ties.p, 0, K, st));
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
+        else if (f64) BPG_HIP(ctx, bpg::launch_contract_f64(ca, Lf, st));
         else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
         launches = 1;
         if (!cluster) BPG_HIP(ctx, bpg::launch_reduce_max(pl->partial.p, n_part, (int)K, pl->maxout.p, st));
@@ -516,12 +625,14 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipMemcpyAsync(pl->h_unsafe.p, pl->unsafe.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    if (bf) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
+    if (!f64) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
+    if (cluster) BPG_HIP(ctx, hipMemcpyAsync(pl->h_ties.p, pl->ties.p, K, hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[5], st));
     auto& ps = pl->pending;
     ps.active = true;
     ps.map = map;
-    ps.fp32_perms = bf ? 0 : K;
+    ps.fp32_perms = form == BPG_PREC_FP32 ? K : 0;
+    ps.fp64_perms = f64 ? K : 0;
     ps.have_times = false;
     ps.perm = perm;
     ps.sign = sign;
@@ -530,7 +641,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     ps.cluster = cluster;
     ps.mass = mass;
     ps.tfce = tfce;
-    ps.bf = bf;
+    ps.form = form;
+    ps.exact = bf && npy == 3;
     ps.pfit = pfit;
     ps.PP = PP;
     ps.launches = launches;
@@ -538,63 +650,56 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     return BPG_OK;
 }
 
-// Repeats the flagged permutations of the finished split-BF16 phase in the
-// FP32 form (synchronously) and merges their results into the pinned result
-// array and the caller's statistics.  With more than half flagged the whole
-// phase is repeated.
-int repeat_in_fp32(bpg_plan* pl) {
+// Repeats permutations S of the finished phase (indices into it) in `form`,
+// synchronously, and merges their results into the pinned result array, the
+// tie flags and the caller's statistics.  On return S holds those of them
+// whose repeat met ill-conditioned samples again (never for FP64), and
+// final_form records the form each permutation's results now come from.
+int repeat_subset(bpg_plan* pl, std::vector<long>& S, int form, std::vector<signed char>& final_form) {
     bpg_ctx* ctx = pl->ctx;
     auto& ps = pl->pending;
-    auto saved = ps;
-    const long K = ps.K;
+    const auto saved = ps;
+    const long K = ps.K, Ks = (long)S.size();
     const int N = pl->d.N, Vc = pl->Vc;
-    std::vector<long> S;
-    for (long k = 0; k < K; k++)
-        if (pl->h_flags.p[k]) S.push_back(k);
-    if (S.empty()) return BPG_OK;
-    if ((long)S.size() * 2 > K) {
-        const int rc = phase_enqueue(pl, ps.map, ps.perm, ps.sign, K, ps.stats, true);
-        if (rc != BPG_OK) return rc;
-        BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
-        ps.active = false;
-        ps.t0 = saved.t0;
-        return BPG_OK;
-    }
-    const long Ks = (long)S.size();
-    // the kernel times reported are those of the split-BF16 run
-    auto& sv = saved;
-    sv.have_times = true;
-    sv.t_transform = elapsed(pl->ev[0], pl->ev[1]);
-    sv.t_operand = elapsed(pl->ev[1], pl->ev[2]);
-    sv.t_contract = elapsed(pl->ev[2], pl->ev[3]);
-    sv.t_cluster = elapsed(pl->ev[3], pl->ev[4]);
+    const bool whole = Ks == K;
     std::vector<unsigned> keep(reinterpret_cast<const unsigned*>(pl->h_max.p),
                                reinterpret_cast<const unsigned*>(pl->h_max.p) + K);
+    std::vector<unsigned char> keep_ties;
+    if (ps.cluster) keep_ties.assign(pl->h_ties.p, pl->h_ties.p + K);
     std::vector<unsigned short> sp;
     std::vector<float> ss, sst;
-    if (ps.sign) {
-        ss.resize((size_t)Ks * N);
-        for (long i = 0; i < Ks; i++) std::copy(ps.sign + S[i] * N, ps.sign + (S[i] + 1) * N, ss.begin() + i * N);
-    } else {
-        sp.resize((size_t)Ks * N);
-        for (long i = 0; i < Ks; i++) std::copy(ps.perm + S[i] * N, ps.perm + (S[i] + 1) * N, sp.begin() + i * N);
+    if (!whole) {
+        if (ps.sign) {
+            ss.resize((size_t)Ks * N);
+            for (long i = 0; i < Ks; i++) std::copy(ps.sign + S[i] * N, ps.sign + (S[i] + 1) * N, ss.begin() + i * N);
+        } else {
+            sp.resize((size_t)Ks * N);
+            for (long i = 0; i < Ks; i++) std::copy(ps.perm + S[i] * N, ps.perm + (S[i] + 1) * N, sp.begin() + i * N);
+        }
+        if (ps.stats) sst.resize((size_t)Ks * Vc);
     }
-    if (ps.stats) sst.resize((size_t)Ks * Vc);
-    const int rc = phase_enqueue(pl, ps.map, ps.sign ? nullptr : sp.data(), ps.sign ? ss.data() : nullptr, Ks,
-                                 ps.stats ? sst.data() : nullptr, true);
+    const unsigned short* rp = whole ? ps.perm : (ps.sign ? nullptr : sp.data());
+    const float* rs = whole ? ps.sign : (ps.sign ? ss.data() : nullptr);
+    float* rst = whole ? ps.stats : (ps.stats ? sst.data() : nullptr);
+    const int rc = phase_enqueue(pl, ps.map, rp, rs, Ks, rst, form);
     if (rc != BPG_OK) return rc;
     BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
-    const unsigned fp32_unsafe = *pl->h_unsafe.p;
+    const bool again = form != BPG_PREC_FP64 && *pl->h_unsafe.p > 0;
     const unsigned* sub = reinterpret_cast<const unsigned*>(pl->h_max.p);
-    for (long i = 0; i < Ks; i++) keep[S[i]] = sub[i];
-    std::copy(keep.begin(), keep.end(), reinterpret_cast<unsigned*>(pl->h_max.p));
-    if (ps.stats)
+    std::vector<long> S2;
+    for (long i = 0; i < Ks; i++) {
+        keep[S[i]] = sub[i];
+        if (saved.cluster) keep_ties[S[i]] = pl->h_ties.p[i];
+        final_form[S[i]] = (signed char)form;
+        if (again && pl->h_flags.p[i]) S2.push_back(S[i]);
+    }
+    if (!whole && saved.stats)
         for (long i = 0; i < Ks; i++)
             std::copy(sst.begin() + i * Vc, sst.begin() + (i + 1) * Vc, saved.stats + S[i] * Vc);
+    std::copy(keep.begin(), keep.end(), reinterpret_cast<unsigned*>(pl->h_max.p));
+    if (saved.cluster) std::copy(keep_ties.begin(), keep_ties.end(), pl->h_ties.p);
     ps = saved;
-    ps.active = false;
-    ps.fp32_perms = Ks;
-    *pl->h_unsafe.p = fp32_unsafe;
+    S.swap(S2);
     return BPG_OK;
 }
 
@@ -604,23 +709,48 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     if (!ps.active) BPG_FAIL(ctx, BPG_ERR_ARG, "no permutation phase in flight");
     ps.active = false;
     BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
-    if (ps.bf && *pl->h_unsafe.p > 0) {
-        // ill-conditioned samples: the split-BF16 z is not accurate enough
-        // for them.  The flagged permutations are repeated in the FP32 form
-        // (all of them when most are flagged) and their results replace the
-        // split-BF16 ones.
-        const int rc = repeat_in_fp32(pl);
-        if (rc != BPG_OK) return rc;
+    const int unsafe_first = ps.form != BPG_PREC_FP64 ? (int)*pl->h_unsafe.p : 0;
+    if (unsafe_first > 0 && pl->Vc > 0) {
+        // Ill-conditioned samples: SSE = |y*|^2 - |z|^2 is too small a
+        // fraction of |y*|^2 for this form's z.  The flagged permutations are
+        // repeated in the next form (split-BF16 -> FP32 -> FP64; an exact
+        // split-BF16 run goes to FP64 directly), whose results replace
+        // theirs, until no sample is flagged; FP64 computes the residual
+        // itself and is never flagged.
+        if (!ps.have_times) {
+            ps.have_times = true;
+            ps.t_transform = elapsed(pl->ev[0], pl->ev[1]);
+            ps.t_operand = elapsed(pl->ev[1], pl->ev[2]);
+            ps.t_contract = elapsed(pl->ev[2], pl->ev[3]);
+            ps.t_cluster = elapsed(pl->ev[3], pl->ev[4]);
+        }
+        std::vector<signed char> final_form(ps.K, (signed char)ps.form);
+        std::vector<long> S;
+        for (long k = 0; k < ps.K; k++)
+            if (pl->h_flags.p[k]) S.push_back(k);
+        int form = ps.form;
+        while (!S.empty() && form != BPG_PREC_FP64) {
+            form = (form == BPG_PREC_SPLIT_BF16 && !ps.exact) ? BPG_PREC_FP32 : BPG_PREC_FP64;
+            const int rc = repeat_subset(pl, S, form, final_form);
+            if (rc != BPG_OK) return rc;
+        }
+        ps.fp32_perms = ps.fp64_perms = 0;
+        for (long k = 0; k < ps.K; k++) {
+            ps.fp32_perms += final_form[k] == BPG_PREC_FP32;
+            ps.fp64_perms += final_form[k] == BPG_PREC_FP64;
+        }
     }
     const long K = ps.K;
     const bool cluster = ps.cluster, mass = ps.mass;
     const int Vc = pl->Vc;
     const unsigned* host = reinterpret_cast<const unsigned*>(pl->h_max.p);
     const float* hmax = pl->h_max.p;
+    long ties = 0;
     for (long k = 0; k < K; k++) {
         if (cluster) {
             // CalculateLargestCluster, LIB:18453-18460: extent, or mass / 1e4
             maxdist[k] = mass ? (float)host[k] / 10000.0f : (float)host[k];
+            if (Vc > 0) ties += pl->h_ties.p[k] != 0;
         } else if (Vc == 0 || !(hmax[k] > -INFINITY)) {
             maxdist[k] = empty_max();
         } else {
@@ -628,6 +758,19 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
             // so it commutes with the max.
             maxdist[k] = (float)((float)quantize(hmax[k]) / 10000.0f);
         }
+    }
+    pl->last_K.resize(pl->nmaps, 0);
+    pl->last_K[ps.map] = K;
+    if (cluster) {
+        pl->map_ties.resize(pl->nmaps, 0);
+        pl->map_ties[ps.map] = ties;
+        if (Vc > 0) {
+            pl->last_ties.assign(pl->h_ties.p, pl->h_ties.p + K);
+        } else {
+            pl->last_ties.assign(K, 0);
+        }
+    } else {
+        pl->last_ties.assign(K, 0);
     }
     const auto t1 = std::chrono::steady_clock::now();
     bpg_timing& tm = pl->timing;
@@ -639,10 +782,12 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     tm.contract_launches = ps.launches;
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = ps.pfit;
-    tm.slots_per_perm = ps.bf ? ps.PP : ps.pfit;
-    tm.precision_used = ps.bf ? BPG_PREC_SPLIT_BF16 : BPG_PREC_FP32;
-    tm.unsafe_waves = (int)*pl->h_unsafe.p;
+    tm.slots_per_perm = ps.form == BPG_PREC_SPLIT_BF16 ? ps.PP : ps.pfit;
+    tm.precision_used = ps.form;
+    tm.unsafe_waves = unsafe_first;
     tm.fp32_permutations = ps.fp32_perms;
+    tm.fp64_permutations = ps.fp64_perms;
+    tm.cdt_tie_maps = cluster ? ties : 0;
     return BPG_OK;
 }
 
@@ -685,12 +830,14 @@ extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const 
         // repeated when the study is collected
         bpg_plan* pl = plans[i];
         auto& ps = pl->pending;
-        if (hipStreamSynchronize(pl->stream) != hipSuccess || !ps.bf || *pl->h_unsafe.p == 0) continue;
+        if (hipStreamSynchronize(pl->stream) != hipSuccess || ps.form != BPG_PREC_SPLIT_BF16 || ps.exact ||
+            *pl->h_unsafe.p == 0)
+            continue;
         long nf = 0;
         for (long k = 0; k < ps.K; k++) nf += pl->h_flags.p[k] != 0;
         if (nf * 2 <= ps.K) continue;
         const auto t0 = ps.t0;
-        const int r = phase_enqueue(pl, ps.map, ps.perm, ps.sign, ps.K, ps.stats, true);
+        const int r = phase_enqueue(pl, ps.map, ps.perm, ps.sign, ps.K, ps.stats, BPG_PREC_FP32);
         ps.t0 = t0;
         if (r != BPG_OK) rc = r;
     }
@@ -704,6 +851,12 @@ extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const 
 extern "C" int bpg_plan_last_timing(const bpg_plan* pl, bpg_timing* out) {
     if (!pl || !out) return BPG_ERR_ARG;
     *out = pl->timing;
+    return BPG_OK;
+}
+
+extern "C" int bpg_plan_last_tie_flags(const bpg_plan* pl, unsigned char* flags) {
+    if (!pl || !flags) return BPG_ERR_ARG;
+    std::copy(pl->last_ties.begin(), pl->last_ties.end(), flags);
     return BPG_OK;
 }
 
@@ -776,6 +929,11 @@ extern "C" int bpg_plan_glm(bpg_plan* pl, float* betas, float* maps, float* resi
 
 extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const* maxdist,
                                 float* pvalues, unsigned* labels) {
+    return bpg_plan_pvalues_ties(pl, K, maxdist, pvalues, labels, nullptr);
+}
+
+extern "C" int bpg_plan_pvalues_ties(bpg_plan* pl, const long* K, const float* const* maxdist,
+                                     float* pvalues, unsigned* labels, unsigned* ties) {
     if (!pl || !K || !maxdist || !pvalues) return BPG_ERR_ARG;
     bpg_ctx* ctx = pl->ctx;
     if (!pl->have_obs) BPG_FAIL(ctx, BPG_ERR_ARG, "observed maps have not been computed");
@@ -786,8 +944,11 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
     const bool cluster = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
     std::fill(pvalues, pvalues + (size_t)pl->nmaps * pl->G, 0.0f);
     if (labels) std::fill(labels, labels + (size_t)pl->nmaps * pl->G, (unsigned)(pl->G * 3));
+    if (ties) std::fill(ties, ties + (size_t)pl->nmaps * pl->G, 0u);
+    pl->timing.pvalue_ties = 0;
     if (Vc == 0) return BPG_OK;
     BPG_HIP(ctx, pl->P.ensure(Vc));
+    BPG_HIP(ctx, pl->d_pties.ensure(Vc));
     if (cluster) {
         BPG_HIP(ctx, pl->bits.ensure(pl->nwords));
         BPG_HIP(ctx, pl->cl.parent.ensure(Vc));
@@ -796,7 +957,8 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
         BPG_HIP(ctx, pl->maxsize.ensure(1));
     }
     std::vector<float> h(Vc);
-    std::vector<unsigned> hl(cluster && labels ? Vc : 0);
+    std::vector<unsigned> hl(cluster && labels ? Vc : 0), ht(Vc);
+    long total_ties = 0;
     for (int c = 0; c < pl->nmaps; c++) {
         if (K[c] <= 0 || !maxdist[c]) BPG_FAIL(ctx, BPG_ERR_ARG, "empty null distribution");
         std::vector<float> srt(maxdist[c], maxdist[c] + K[c]);
@@ -824,17 +986,29 @@ extern "C" int bpg_plan_pvalues(bpg_plan* pl, const long* K, const float* const*
         pa.labels = cluster ? pl->labels.p : nullptr;
         pa.sizes = cluster ? pl->cl.sizes.p : nullptr;
         pa.K = (int)K[c]; pa.Vc = Vc; pa.inference = d.inference; pa.cdt = d.cdt; pa.P = pl->P.p;
+        pa.ties = pl->d_pties.p;
+        // the tied maps of this map's last permutation phase, when that phase
+        // had as many permutations as this null
+        pa.cluster_ties = cluster && c < (int)pl->map_ties.size() &&
+                                  pl->last_K.size() > (size_t)c && pl->last_K[c] == K[c]
+                              ? (unsigned)pl->map_ties[c]
+                              : 0u;
         BPG_HIP(ctx, bpg::launch_pvalues(pa, st));
         BPG_HIP(ctx, hipMemcpyAsync(h.data(), pl->P.p, Vc * sizeof(float), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipMemcpyAsync(ht.data(), pl->d_pties.p, Vc * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         if (cluster && labels)
             BPG_HIP(ctx, hipMemcpyAsync(hl.data(), pl->labels.p, Vc * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         BPG_HIP(ctx, hipStreamSynchronize(st));
         for (int v = 0; v < Vc; v++) pvalues[(size_t)c * pl->G + pl->comp2grid[v]] = h[v];
+        for (int v = 0; v < Vc; v++) total_ties += ht[v];
+        if (ties)
+            for (int v = 0; v < Vc; v++) ties[(size_t)c * pl->G + pl->comp2grid[v]] = ht[v];
         // grid labels: the component's minimum linear index (its root's grid
         // index), 3*W*H*D elsewhere
         if (cluster && labels)
             for (int v = 0; v < Vc; v++)
                 if (hl[v] != 0xFFFFFFFFu) labels[(size_t)c * pl->G + pl->comp2grid[v]] = pl->comp2grid[hl[v]];
     }
+    pl->timing.pvalue_ties = total_ties;
     return BPG_OK;
 }

@@ -124,6 +124,9 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
                 const unsigned long long bal = __ballot(valid && stv > a.cdt);
                 if ((lane & 31) == 0 && k < a.K && vt >= 0)
                     a.bits[(size_t)k * a.nwords + vt] = hl ? (unsigned)(bal >> 32) : (unsigned)bal;
+                // a statistic inside the error band of the threshold (rare):
+                // the map's largest cluster may differ from an exact one's
+                if (a.ties && valid && k < a.K && stv > a.cdt_lo && stv <= a.cdt_hi) a.ties[k] = 1;
             }
         }
     }

@@ -52,10 +52,11 @@ struct TransformArgs {
     int N, q, Vc, Vpad;
     int KK;                // BF16 k-steps of 16 subjects (with Yb)
     int npy;               // BF16 fragments per subject in Yb: 2 (hi, lo) or 3 (hi, mid, lo)
-    int NS;                // FP32 operand rows, >= N (with Y32)
+    int NS;                // FP32 / FP64 operand rows, >= N (with Y32 or Y64)
     double* yy;            // [Vpad] |y*|^2 (padding voxels: NaN)
     unsigned* Yb;          // BF16 fragments, (Vpad/32)*KK*npy*64 uint4
     float* Y32;            // [NS][Vpad]
+    double* Y64;           // [NS][Vpad] (the FP64 form; dscale all ones)
 };
 // padded width of the nuisance basis for q columns (0 for none, -1 if q > 24)
 int transform_width(int q);
@@ -81,6 +82,10 @@ struct ContractArgs {
     const unsigned* Yb;    // BF16 data fragments
     const float* G32;      // FP32 operand
     const float* Y32;      // FP32 data [NS][Vpad]
+    const double* Y64;     // FP64 data [NS][Vpad] (unscaled y*)
+    const unsigned short* perm;  // FP64 form: [K][N] permutations (or null)
+    const float* sign;           // FP64 form: [K][N] signs (or null)
+    const double* Qt;            // FP64 form: rotated basis, Qt[s + i*N]
     const double* yy;      // [Vpad]
     float* partial;        // [n_vchunks][K] per-voxel-chunk maxima of the statistic
     float* stats;          // [K][Vc] or null
@@ -99,6 +104,13 @@ struct ContractArgs {
     float pinf;            // +infinity (a run-time value: the running max is one v_med3_f32)
     unsigned* unsafe;
     unsigned char* flags;  // [K]: 1 for a permutation with an ill-conditioned sample
+    // cluster inference: a permutation whose map has an in-mask statistic in
+    // (cdt_lo, cdt_hi] -- within the form's error band of the threshold --
+    // gets ties[k] = 1 (its largest cluster may differ from an exact
+    // computation's); cdt_lo == cdt_hi disables the check
+    float cdt_lo, cdt_hi;
+    unsigned char* ties;   // [K] or null
+    double dof;            // FP64 form: N - p
 };
 // SSE / |y*|^2 below which a sample is counted as ill-conditioned: the
 // split-BF16 z carries ~1e-5 relative error per unit of |z|^2 / SSE, the
@@ -140,6 +152,15 @@ int f32_perm_tile(int pfit);
 F32Launch plan_contract_f32(int K, int pfit, int Vpad);
 hipError_t launch_contract_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s);
 
+// FP64 form (the repeat of permutations with ill-conditioned samples):
+// blocks of f64_perm_tile(pfit) permutations over chunks of 256-voxel
+// steps, the operand rows built in LDS from the permutations / signs and
+// Qt, z and the residual sum of squares SSE = |y* - B z|^2 (B the permuted
+// basis) accumulated in FP64 from the FP64 y*.
+int f64_perm_tile(int pfit);
+F32Launch plan_contract_f64(int K, int pfit, int Vpad);
+hipError_t launch_contract_f64(const ContractArgs& a, const F32Launch& L, hipStream_t s);
+
 // out[k] = max over the n_part chunk maxima partial[c*K + k] (NaN-free:
 // chunks without in-mask voxels hold -inf).
 hipError_t launch_reduce_max(const float* partial, int n_part, int K, float* out, hipStream_t s);
@@ -166,6 +187,12 @@ struct PvalArgs {
     int K, Vc, inference;
     float cdt;
     float* P;                   // [Vc]
+    // optional: per voxel, the comparisons with the null inside the error
+    // band (voxel inference: null values within 1e-4 (2 + |T|) of T;
+    // cluster inference: cluster_ties, the permutations whose largest
+    // cluster may differ, for supra-threshold voxels; TFCE: 0)
+    unsigned* ties;             // [Vc] or null
+    unsigned cluster_ties;
 };
 hipError_t launch_pvalues(const PvalArgs& a, hipStream_t s);
 

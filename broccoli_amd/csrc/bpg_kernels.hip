@@ -126,7 +126,10 @@ __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
             residual8(s0, o, yy);
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                if (s0 + e < a.NS) a.Y32[(size_t)(s0 + e) * a.Vpad + v] = (float)o[e];
+                if (s0 + e < a.NS) {
+                    if (a.Y64) a.Y64[(size_t)(s0 + e) * a.Vpad + v] = o[e];
+                    else a.Y32[(size_t)(s0 + e) * a.Vpad + v] = (float)o[e];
+                }
         }
     }
     // padding voxels carry a NaN norm: their statistic is NaN, which the
@@ -464,6 +467,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(ContractArgs a, F32Launch 
                     if (w0 < a.nwords) a.bits[(size_t)kk * a.nwords + w0] = (unsigned)bal;
                     if (w0 + 1 < a.nwords) a.bits[(size_t)kk * a.nwords + w0 + 1] = (unsigned)(bal >> 32);
                 }
+                if (a.ties && valid && kk < a.K && st > a.cdt_lo && st <= a.cdt_hi) a.ties[kk] = 1;
             }
             float m = st;
             m = fmaxf(m, __shfl_xor(m, 32, 64));
@@ -526,6 +530,164 @@ F32Launch plan_contract_f32(int K, int pfit, int Vpad) {
 
 hipError_t launch_contract_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
     return a.ftest ? dispatch_f32<true>(a, L, s) : dispatch_f32<false>(a, L, s);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 contraction (the repeat of permutations with ill-conditioned samples)
+// ---------------------------------------------------------------------------
+// Block = 256 threads, one voxel per thread per step, a tile of PTT = 32/P
+// permutations.  The tile's permuted basis B_k(s, i) = Qt(pi_k(s), i) (or
+// sign_k(s) Qt(s, 0) for the group mean) sits in LDS as [s][k][i] doubles
+// and is read as broadcasts.  Per voxel and permutation, in FP64:
+//   z = B' y*,   SSE = |y* - B z|^2   (the residual itself, not
+//   |y*|^2 - |z|^2, so no cancellation however small SSE is),
+//   t = z_0 sqrt(dof / SSE),  F = (z_0^2 + .. + z_{r-1}^2) dof / (r SSE).
+// The data pass runs twice (z, then the residual); y* comes from HBM / L2.
+template <int P>
+struct F64Tile {
+    static constexpr int PTT = (32 / P) < 1 ? 1 : (32 / P);
+    static constexpr int ROW = PTT * P;  // doubles per subject
+};
+
+template <int P, bool FT>
+__global__ __launch_bounds__(256) void k_contract_f64(ContractArgs a, F32Launch L) {
+    using T = F64Tile<P>;
+    constexpr int PTT = T::PTT, ROW = T::ROW;
+    extern __shared__ double sB[];  // [N][ROW]
+    __shared__ unsigned sMax[PTT];
+    const int tid = threadIdx.x;
+    const int lb = xcd_logical_block(L.per_xcd);
+    if (lb >= L.n_pg * L.n_vchunks) return;
+    const int pg = lb % L.n_pg;
+    const int vc = lb / L.n_pg;
+    const int k0 = pg * PTT;
+    const int N = a.N;
+    for (int t = tid; t < N * ROW; t += 256) {
+        const int s = t / ROW, ki = t % ROW;
+        const int k = k0 + ki / P, i = ki % P;
+        double val = 0.0;
+        if (k < a.K) {
+            if (a.sign) val = (double)a.sign[(size_t)k * N + s] * a.Qt[s + (size_t)i * N];
+            else val = a.Qt[(size_t)a.perm[(size_t)k * N + s] + (size_t)i * N];
+        }
+        sB[t] = val;
+    }
+    if (tid < PTT) sMax[tid] = 0u;
+    __syncthreads();
+    const int v0 = vc * L.vox_per_chunk;
+    const int v1 = min(v0 + L.vox_per_chunk, a.Vpad);
+    const int lane = tid & 63;
+    for (int vb = v0; vb < v1; vb += 256) {
+        const int v = vb + tid;
+        const bool inb = v < a.Vpad;
+        const bool valid = v < a.Vc;
+        const double* yp = a.Y64 + (inb ? v : 0);
+        double z[ROW];
+#pragma unroll
+        for (int q = 0; q < ROW; q++) z[q] = 0.0;
+        for (int s = 0; s < N; s++) {
+            const double ys = inb ? yp[(size_t)s * a.Vpad] : 0.0;
+            const double* row = sB + s * ROW;
+#pragma unroll
+            for (int q = 0; q < ROW; q++) z[q] = fma(row[q], ys, z[q]);
+        }
+        double sse[PTT];
+#pragma unroll
+        for (int k = 0; k < PTT; k++) sse[k] = 0.0;
+        for (int s = 0; s < N; s++) {
+            const double ys = inb ? yp[(size_t)s * a.Vpad] : 0.0;
+            const double* row = sB + s * ROW;
+#pragma unroll
+            for (int k = 0; k < PTT; k++) {
+                double e = ys;
+#pragma unroll
+                for (int i = 0; i < P; i++) e = fma(-row[k * P + i], z[k * P + i], e);
+                sse[k] = fma(e, e, sse[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PTT; k++) {
+            double stat;
+            if (FT) {
+                double num = 0.0;
+#pragma unroll
+                for (int i = 0; i < P; i++)
+                    if (i < a.r) num = fma(z[k * P + i], z[k * P + i], num);
+                stat = num * a.dof / ((double)a.r * sse[k]);
+            } else {
+                stat = z[k * P] * sqrt(a.dof / sse[k]);
+            }
+            float st = (float)stat;
+            if (!valid) st = -INFINITY;
+            const int kk = k0 + k;
+            if (a.stats && valid && kk < a.K) a.stats[(size_t)kk * a.Vc + v] = st;
+            if (a.bits) {
+                const unsigned long long bal = __ballot(valid && st > a.cdt);
+                if (lane == 0 && kk < a.K) {
+                    const int w0 = (vb + (tid & ~63)) >> 5;
+                    if (w0 < a.nwords) a.bits[(size_t)kk * a.nwords + w0] = (unsigned)bal;
+                    if (w0 + 1 < a.nwords) a.bits[(size_t)kk * a.nwords + w0 + 1] = (unsigned)(bal >> 32);
+                }
+                if (a.ties && valid && kk < a.K && st > a.cdt_lo && st <= a.cdt_hi) a.ties[kk] = 1;
+            }
+            float m = st;
+            m = fmaxf(m, __shfl_xor(m, 32, 64));
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 8, 64));
+            m = fmaxf(m, __shfl_xor(m, 4, 64));
+            m = fmaxf(m, __shfl_xor(m, 2, 64));
+            m = fmaxf(m, __shfl_xor(m, 1, 64));
+            if (lane == 0 && m > -INFINITY) atomicMax(&sMax[k], enc_max(m));
+        }
+    }
+    __syncthreads();
+    if (tid < PTT && k0 + tid < a.K)
+        a.partial[(size_t)vc * a.K + k0 + tid] = sMax[tid] ? dec_max(sMax[tid]) : -INFINITY;
+}
+
+namespace {
+
+template <int P, bool FT>
+hipError_t run_f64(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
+    const size_t shmem = (size_t)a.N * F64Tile<P>::ROW * sizeof(double);
+    hipLaunchKernelGGL((k_contract_f64<P, FT>), dim3(L.per_xcd * 8), dim3(256), shmem, s, a, L);
+    return hipGetLastError();
+}
+
+template <bool FT>
+hipError_t dispatch_f64(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
+    switch (a.pfit) {
+#define BPG_P(n) case n: return run_f64<n, FT>(a, L, s);
+        BPG_P(1) BPG_P(2) BPG_P(3) BPG_P(4) BPG_P(5) BPG_P(6) BPG_P(7) BPG_P(8) BPG_P(9)
+        BPG_P(10) BPG_P(11) BPG_P(12) BPG_P(13) BPG_P(14) BPG_P(15) BPG_P(16) BPG_P(17)
+        BPG_P(18) BPG_P(19) BPG_P(20) BPG_P(21) BPG_P(22) BPG_P(23) BPG_P(24) BPG_P(25)
+#undef BPG_P
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int f64_perm_tile(int pfit) { return (32 / pfit) < 1 ? 1 : (32 / pfit); }
+
+F32Launch plan_contract_f64(int K, int pfit, int Vpad) {
+    F32Launch L{};
+    const int PTT = f64_perm_tile(pfit);
+    L.n_pg = (K + PTT - 1) / PTT;
+    const int nvsteps = (Vpad + 255) / 256;
+    int nvc = (2048 + L.n_pg - 1) / L.n_pg;
+    if (nvc > BPG_MAX_VCHUNKS) nvc = BPG_MAX_VCHUNKS;
+    if (nvc > nvsteps) nvc = nvsteps;
+    if (nvc < 1) nvc = 1;
+    const int steps_per = (nvsteps + nvc - 1) / nvc;
+    L.vox_per_chunk = steps_per * 256;
+    L.n_vchunks = (nvsteps + steps_per - 1) / steps_per;
+    L.per_xcd = (L.n_pg * L.n_vchunks + 7) / 8;
+    return L;
+}
+
+hipError_t launch_contract_f64(const ContractArgs& a, const F32Launch& L, hipStream_t s) {
+    return a.ftest ? dispatch_f64<true>(a, L, s) : dispatch_f64<false>(a, L, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,18 +761,53 @@ __global__ __launch_bounds__(256) void k_pvalues(PvalArgs a) {
     float T = a.obs[v];
     // TFCE compares the statistic itself with the TFCE null, as the
     // reference does (LIB:16824-16837)
-    if (a.inference == BPG_CLUSTER_EXTENT || a.inference == BPG_CLUSTER_MASS) {
-        if (!(T > a.cdt)) { a.P[v] = 0.0f; return; }
+    const bool cluster = a.inference == BPG_CLUSTER_EXTENT || a.inference == BPG_CLUSTER_MASS;
+    if (cluster) {
+        if (!(T > a.cdt)) {
+            a.P[v] = 0.0f;
+            if (a.ties) a.ties[v] = 0u;
+            return;
+        }
         T = (float)a.sizes[a.labels[v]];
         if (a.inference == BPG_CLUSTER_MASS) T = T / 10000.0f;
     }
-    int lo = 0, hi = a.K;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a.sorted_max[mid] < T) lo = mid + 1;
-        else hi = mid;
+    // first index whose null value is >= x (not < x)
+    auto lower = [&](float x) {
+        int lo = 0, hi = a.K;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.sorted_max[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int cnt = lower(T);
+    a.P[v] = (float)cnt / (float)a.K;
+    if (a.ties) {
+        unsigned t = 0u;
+        if (cluster) {
+            // every permutation whose largest cluster may differ
+            t = a.cluster_ties;
+        } else if (a.inference == BPG_VOXEL) {
+            // null values within 1e-4 (2 + |T|) of T: a null value's statistic
+            // is within 1e-4 (1 + |t|) of the exact one and its quantisation
+            // may move it by one more quantum, so the comparison T > m_k can
+            // differ from an exact computation's only inside this band
+            const float band = 1e-4f * (2.0f + fabsf(T)) * 1.0001f;
+            const int lo = lower(T - band);
+            int hi = lo;
+            {
+                int h = a.K;
+                while (hi < h) {
+                    const int mid = (hi + h) >> 1;
+                    if (a.sorted_max[mid] <= T + band) hi = mid + 1;
+                    else h = mid;
+                }
+            }
+            t = (unsigned)(hi - lo);
+        }
+        a.ties[v] = t;
     }
-    a.P[v] = (float)lo / (float)a.K;
 }
 
 hipError_t launch_pvalues(const PvalArgs& a, hipStream_t s) {

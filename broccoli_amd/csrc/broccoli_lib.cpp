@@ -111,6 +111,16 @@ void BROCCOLI_LIB::GetPermutationPhaseTimes(double* permutation_ms, double* obse
     if (pvalues_ms) *pvalues_ms = t_pval_;
 }
 
+void BROCCOLI_LIB::GetPermutationTies(long* pvalue_ties, long* cluster_tie_maps) {
+    if (pvalue_ties) *pvalue_ties = pvalue_ties_;
+    if (cluster_tie_maps) *cluster_tie_maps = cdt_tie_maps_;
+}
+
+void BROCCOLI_LIB::GetPermutationRepeats(long* fp32_permutations, long* fp64_permutations) {
+    if (fp32_permutations) *fp32_permutations = fp32_perms_;
+    if (fp64_permutations) *fp64_permutations = fp64_perms_;
+}
+
 void BROCCOLI_LIB::RecordError(int slot, int status, bool buffer) {
     if (status == BPG_OK) return;
     if (buffer || status == BPG_ERR_NOMEM) create_buffer_errors_[slot] = status;
@@ -142,6 +152,7 @@ void BROCCOLI_LIB::Perform(int test) {
     const int slot_perm = test == BPG_TTEST ? kSlotTTestPermutation
                           : test == BPG_FTEST ? kSlotFTestPermutation
                                               : kSlotMeanPermutation;
+    pvalue_ties_ = cdt_tie_maps_ = fp32_perms_ = fp64_perms_ = 0;
     if (!initiated_ || !ctx_) {
         run_kernel_errors_[slot_perm] = BPG_ERR_NODEVICE;
         return;
@@ -209,6 +220,14 @@ void BROCCOLI_LIB::Perform(int test) {
             return;
         }
         if (out_dist_ && out_dist_[c]) std::copy(dist, dist + K, out_dist_[c]);
+        {
+            bpg_timing tm{};
+            if (bpg_plan_last_timing(plan, &tm) == BPG_OK) {
+                cdt_tie_maps_ += tm.cdt_tie_maps;
+                fp32_perms_ += tm.precision_used == BPG_PREC_FP32 ? 0 : tm.fp32_permutations;
+                fp64_perms_ += tm.fp64_permutations;
+            }
+        }
         if (wrapper_ == 2) {
             std::vector<float> srt(dist, dist + K);
             std::sort(srt.begin(), srt.end());
@@ -245,6 +264,8 @@ void BROCCOLI_LIB::Perform(int test) {
         std::vector<const float*> md(nmaps);
         for (int c = 0; c < nmaps; c++) md[c] = dists[c].data();
         rc = bpg_plan_pvalues(plan, Ks.data(), md.data(), out_p_, nullptr);
+        bpg_timing tm{};
+        if (rc == BPG_OK && bpg_plan_last_timing(plan, &tm) == BPG_OK) pvalue_ties_ = tm.pvalue_ties;
         if (rc != BPG_OK) {
             const int slot = inference_ == 0 ? kSlotPValuesVoxel
                              : inference_ == 1 ? kSlotPValuesExtent

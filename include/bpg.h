@@ -53,7 +53,8 @@ extern "C" {
 /* arithmetic of the permutation contraction */
 #define BPG_PREC_FP32 0       /* FP32 VALU contraction                         */
 #define BPG_PREC_SPLIT_BF16 1 /* hi/lo BF16 MFMA with FP32 accumulation        */
-#define BPG_PREC_FP64 2       /* FP64 VALU (the first-level permutation kernel) */
+#define BPG_PREC_FP64 2       /* FP64 VALU, residual formed directly (also the
+                                 repeat of ill-conditioned permutations)      */
 #define BPG_PREC_SPLIT_F16 3  /* hi/lo FP16 MFMA, first-level plans only       */
 
 /* limits of the reference (RGL:630-636 p <= 25, RGL:726-732 F rows <= 10) */
@@ -97,11 +98,23 @@ typedef struct {
                              permutation-invariant and carries no signal)     */
     int slots_per_perm;   /* split-BF16 accumulator rows per permutation      */
     int precision_used;   /* BPG_PREC_* of the results                        */
-    int unsafe_waves;     /* waves that met a sample with SSE below 1/8 (split-
-                             BF16) or 1/256 (FP32) of |y*|^2; the split-BF16
+    int unsafe_waves;     /* waves of the first run that met a sample with
+                             SSE below 1/8 (split-BF16) or 1/256 (FP32, and
+                             split-BF16 sign flips at N <= 64) of |y*|^2.  The
                              permutations with such samples are repeated in
-                             FP32, and this counts the FP32 run's waves       */
+                             the next form (split-BF16 -> FP32 -> FP64) until
+                             none is flagged; FP64 forms the residual itself
+                             and is never flagged                            */
     long fp32_permutations; /* permutations whose results come from FP32      */
+    long fp64_permutations; /* permutations whose results come from FP64      */
+    long cdt_tie_maps;    /* cluster inference: permutations whose map has an
+                             in-mask statistic within 1e-4 (1 + |cdt|) of the
+                             threshold (not counted for FP64 results): their
+                             largest cluster may differ from an exact
+                             computation's                                  */
+    long pvalue_ties;     /* set by bpg_plan_pvalues: the (voxel, permutation)
+                             comparisons inside the error band (the total of
+                             its `ties` output)                              */
 } bpg_timing;
 
 /* ---- context: replaces BROCCOLI_LIB(platform, device, ...) +
@@ -119,6 +132,14 @@ const char* bpg_device_name(const bpg_ctx* ctx);
  *      LIB:13228-13261) and SetupPermutationTestSecondLevel (LIB:16128).
  *      Only in-mask voxels are uploaded (compacted, subject-major). */
 int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_plan** out);
+/* ---- the checks of bpg_plan_create on the host alone, no device needed (no
+ *      counterpart in the reference, which checks sizes in the CLI,
+ *      RGL:599-733): argument ranges, a design and contrasts of full rank,
+ *      and finite data inside the mask (a NaN or infinite value there is
+ *      refused: the reference's (int)(T * 1e4) of the resulting NaN, KM:299,
+ *      depends on the device).  Returns the status bpg_plan_create would
+ *      return for them; err (optional) receives the message. */
+int bpg_validate_study(const bpg_study_desc* desc, char* err, int err_len);
 void bpg_plan_destroy(bpg_plan* plan);
 long bpg_plan_num_voxels(const bpg_plan* plan); /* V_mask */
 
@@ -171,6 +192,25 @@ int bpg_plan_glm(bpg_plan* plan, float* betas, float* maps, float* residuals, fl
  *      outside clusters), nmaps*W*H*D. */
 int bpg_plan_pvalues(bpg_plan* plan, const long* K, const float* const* maxdist,
                      float* pvalues, unsigned* labels);
+/* ---- bpg_plan_pvalues with the ties of its counts (no counterpart in the
+ *      reference, whose FP32 counts report none).  ties (optional): host
+ *      grid, nmaps*W*H*D, per voxel the comparisons T > m_k that lie inside
+ *      the error band, i.e. where an exact computation could count
+ *      otherwise: voxel inference, the null values within 1e-4 (2 + |T|) of
+ *      the observed T (the statistic bound 1e-4 (1 + |t|) plus one
+ *      quantum of the max-T quantisation); cluster inference, for voxels
+ *      above the threshold, the permutations of the map's last phase whose
+ *      map had a statistic within 1e-4 (1 + |cdt|) of the threshold
+ *      (cdt_tie_maps; 0 when the null passed is not that phase's); TFCE: 0.
+ *      The total lands in bpg_timing.pvalue_ties. */
+int bpg_plan_pvalues_ties(bpg_plan* plan, const long* K, const float* const* maxdist,
+                          float* pvalues, unsigned* labels, unsigned* ties);
+/* ---- the per-permutation tie flags of the last permutation phase (no
+ *      counterpart in the reference): flags[k] = 1 when permutation k's map
+ *      had an in-mask statistic within 1e-4 (1 + |cdt|) of the
+ *      cluster-defining threshold (cluster inference; 0 otherwise and for
+ *      results of the FP64 form).  flags: host, K bytes. */
+int bpg_plan_last_tie_flags(const bpg_plan* plan, unsigned char* flags);
 
 /* ---- host-side pieces of the path -------------------------------------- */
 

@@ -22,7 +22,8 @@
  *   - a mean design without a permutation file gets the identity
  *     permutation (the reference reads the caller's uninitialised matrix);
  *   - extensions: SetPermutationPrecision (BPG_PREC_*; default split-BF16),
- *     SetClusterConnectivity (26 default, or 6), GetPermutationPhaseTimes.
+ *     SetClusterConnectivity (26 default, or 6), GetPermutationPhaseTimes,
+ *     GetPermutationTies, GetPermutationRepeats.
  */
 #ifndef BROCCOLI_LIB_H
 #define BROCCOLI_LIB_H
@@ -122,6 +123,14 @@ public:
     // milliseconds of the last Perform call: permutation phase (all
     // statistical maps), observed maps, p-values
     void GetPermutationPhaseTimes(double* permutation_ms, double* observed_ms, double* pvalues_ms);
+    // ties of the last Perform call (bpg_plan_pvalues_ties): p-value
+    // comparisons inside the error band, summed over the statistical maps,
+    // and (cluster inference) permutations whose map had a statistic within
+    // the error band of the cluster-defining threshold
+    void GetPermutationTies(long* pvalue_ties, long* cluster_tie_maps);
+    // permutations of the last Perform call whose results come from the
+    // FP32 and FP64 forms (repeats of ill-conditioned permutations)
+    void GetPermutationRepeats(long* fp32_permutations, long* fp64_permutations);
 
     static const int NUMBER_OF_OPENCL_KERNELS = 102;
 
@@ -170,6 +179,7 @@ private:
     int precision_ = 1;
     int connectivity_ = 26;
     double t_perm_ = 0.0, t_obs_ = 0.0, t_pval_ = 0.0;
+    long pvalue_ties_ = 0, cdt_tie_maps_ = 0, fp32_perms_ = 0, fp64_perms_ = 0;
 
     int create_kernel_errors_[NUMBER_OF_OPENCL_KERNELS];
     int create_buffer_errors_[NUMBER_OF_OPENCL_KERNELS];

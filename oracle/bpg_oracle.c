@@ -1130,3 +1130,154 @@ int orc_perm_stats(const orc_study* st, int c, int precision, long K, double* ou
     free(Xd); free(map); free(w32); free(w64);
     return 0;
 }
+
+
+/* FP64 maximum of the statistic over the in-mask voxels for the first K
+ * permutations of map c, not quantised: maxv[k] = max_v t_k(v), with
+ * orc_perm_map64's semantics (the reference's beta loop then residual loop,
+ * K2:1562-1669, K4:1943-2060, K2:1458-1559, in FP64 on the FP64-transformed
+ * data).  For the full-size accuracy check: the in-mask voxels are
+ * compacted, taken in blocks of OB that stay in cache while every
+ * permutation runs over them, OL voxels at a time as independent lanes;
+ * each thread keeps its own maxima, merged at the end.  NaN statistics
+ * (all-zero residualised data) are left out of the max. */
+#define OB 256
+#define OL 8
+/* beta and the residual sum of squares of OL voxels (lanes) for one
+ * permutation, the reference's two loops (K2:1624-1663) in FP64; PP is the
+ * compile-time regressor count (0: use P) so the accumulators stay in
+ * registers.  Contracted multiply-adds are allowed: the result is FP64. */
+#define ORC_LANES(PP)                                                              \
+    static void __attribute__((optimize("fp-contract=fast")))                     \
+    lanes##PP(int N, int P_, size_t Vp, const double* yc, const double* Xp,         \
+              const double* Xd, const unsigned short* pk, const float* sk,          \
+              double beta[MAXP][OL], double sse[OL]) {                             \
+        const int P = PP ? PP : P_;                                                \
+        double b[PP ? PP : MAXP][OL];                                              \
+        for (int r = 0; r < P; r++)                                                \
+            for (int l = 0; l < OL; l++) b[r][l] = 0.0;                            \
+        for (int s = 0; s < N; s++) {                                              \
+            const int ps = pk ? pk[s] : s;                                         \
+            const double sg = sk ? (double)sk[s] : 1.0;                            \
+            const double* ys = yc + (size_t)s * Vp;                                \
+            for (int r = 0; r < P; r++) {                                          \
+                const double w = Xp[ps + (size_t)r * N] * sg;                      \
+                for (int l = 0; l < OL; l++) b[r][l] += ys[l] * w;                 \
+            }                                                                      \
+        }                                                                          \
+        double q[OL];                                                              \
+        for (int l = 0; l < OL; l++) q[l] = 0.0;                                   \
+        for (int s = 0; s < N; s++) {                                              \
+            const int ps = pk ? pk[s] : s;                                         \
+            const double sg = sk ? (double)sk[s] : 1.0;                            \
+            const double* ys = yc + (size_t)s * Vp;                                \
+            double e[OL];                                                          \
+            for (int l = 0; l < OL; l++) e[l] = ys[l] * sg;                        \
+            for (int r = 0; r < P; r++) {                                          \
+                const double xv = Xd[ps + (size_t)r * N];                          \
+                for (int l = 0; l < OL; l++) e[l] -= xv * b[r][l];                 \
+            }                                                                      \
+            for (int l = 0; l < OL; l++) q[l] += e[l] * e[l];                      \
+        }                                                                          \
+        for (int r = 0; r < P; r++)                                                \
+            for (int l = 0; l < OL; l++) beta[r][l] = b[r][l];                     \
+        for (int l = 0; l < OL; l++) sse[l] = q[l];                                \
+    }
+ORC_LANES(0)
+ORC_LANES(1)
+ORC_LANES(2)
+ORC_LANES(3)
+ORC_LANES(4)
+ORC_LANES(5)
+ORC_LANES(6)
+ORC_LANES(7)
+ORC_LANES(8)
+typedef void (*orc_lanes_fn)(int, int, size_t, const double*, const double*, const double*,
+                             const unsigned short*, const float*, double[MAXP][OL], double[OL]);
+
+int orc_perm_max64(const orc_study* st, int c, long K, double* maxv) {
+    int N = st->N, p = st->p, nc = st->nc, test = st->test;
+    size_t V = (size_t)st->W * st->H * st->D;
+    int P = (test == ORC_GROUP_MEAN) ? 1 : p;
+#ifdef _OPENMP
+    if (st->nthreads > 0) omp_set_num_threads(st->nthreads);
+#endif
+    double* w64 = (double*)malloc(sizeof(double) * V * N);
+    double* Xd = (double*)malloc(sizeof(double) * (size_t)N * p);
+    model64* m = (model64*)malloc(sizeof(model64));
+    double Cd[MAXP * MAXC];
+    if (!w64 || !Xd || !m || N > 512) { free(w64); free(Xd); free(m); return -1; }
+    for (size_t i = 0; i < (size_t)N * p; i++) Xd[i] = (double)st->X[i];
+    for (int i = 0; i < p * nc; i++) Cd[i] = (double)st->C[i];
+    if (transformed(st, c, ORC_PREC_F64, NULL, w64) ||
+        model64_init(m, test == ORC_GROUP_MEAN ? ORC_TTEST : test, N, P, test == ORC_GROUP_MEAN ? 1 : nc, Xd, Cd)) {
+        free(w64); free(Xd); free(m);
+        return -1;
+    }
+    size_t Vc = 0;
+    for (size_t i = 0; i < V; i++) Vc += st->mask[i] == 1.0f;
+    size_t Vp = (Vc + OB - 1) / OB * OB;
+    double* yc = (double*)calloc((size_t)N * (Vp ? Vp : 1), sizeof(double));
+    if (!yc) { free(w64); free(Xd); free(m); return -1; }
+    {
+        size_t j = 0;
+        for (size_t i = 0; i < V; i++)
+            if (st->mask[i] == 1.0f) {
+                for (int s = 0; s < N; s++) yc[(size_t)s * Vp + j] = w64[i + (size_t)s * V];
+                j++;
+            }
+    }
+    free(w64);
+    for (long k = 0; k < K; k++) maxv[k] = -INFINITY;
+    long nblk = (long)(Vp / OB);
+    const unsigned short* perm0 = (test == ORC_GROUP_MEAN) ? NULL : st->perm[c];
+    double cvec[MAXP];
+    for (int r = 0; r < P; r++) cvec[r] = (test == ORC_GROUP_MEAN) ? 1.0 : Cd[r + (size_t)p * c];
+    static const orc_lanes_fn lanes_tab[9] = {lanes0, lanes1, lanes2, lanes3, lanes4,
+                                              lanes5, lanes6, lanes7, lanes8};
+    const orc_lanes_fn lanes = P <= 8 ? lanes_tab[P] : lanes0;
+#pragma omp parallel
+    {
+        double* lm = (double*)malloc(sizeof(double) * (size_t)K);
+        for (long k = 0; k < K; k++) lm[k] = -INFINITY;
+#pragma omp for schedule(dynamic, 1)
+        for (long b = 0; b < nblk; b++) {
+            size_t v0 = (size_t)b * OB;
+            for (long k = 0; k < K; k++) {
+                const unsigned short* pk = perm0 ? perm0 + (size_t)k * N : NULL;
+                const float* sk = (test == ORC_GROUP_MEAN) ? st->sign + (size_t)k * N : NULL;
+                double best = -INFINITY;
+                for (size_t l0 = v0; l0 < v0 + OB && l0 < Vc; l0 += OL) {
+                    double beta[MAXP][OL], sse[OL];
+                    lanes(N, P, Vp, yc + l0, m->Xp, Xd, pk, sk, beta, sse);
+                    for (int l = 0; l < OL; l++) {
+                        if (l0 + l >= Vc) break;
+                        double var = sse[l] / (double)(N - P), t;
+                        if (test == ORC_FTEST) {
+                            double cb[MAXC], q = 0.0;
+                            for (int a = 0; a < nc; a++) {
+                                cb[a] = 0.0;
+                                for (int r = 0; r < p; r++) cb[a] += Cd[r + (size_t)p * a] * beta[r][l];
+                            }
+                            for (int a = 0; a < nc; a++)
+                                for (int bb = 0; bb < nc; bb++) q += cb[a] * m->sc[bb + a * nc] * cb[bb];
+                            t = q / var / (double)nc;
+                        } else {
+                            double cv = 0.0;
+                            for (int r = 0; r < P; r++) cv += cvec[r] * beta[r][l];
+                            t = cv / sqrt(var * m->sc[test == ORC_GROUP_MEAN ? 0 : c]);
+                        }
+                        if (t > best) best = t;
+                    }
+                }
+                if (best > lm[k]) lm[k] = best;
+            }
+        }
+#pragma omp critical
+        for (long k = 0; k < K; k++)
+            if (lm[k] > maxv[k]) maxv[k] = lm[k];
+        free(lm);
+    }
+    free(yc); free(Xd); free(m);
+    return 0;
+}

@@ -137,6 +137,10 @@ int orc_randomise(const orc_study* st, int precision, float* maps,
 /* statistic maps of the first K permutations of map c, out[k*W*H*D + i] */
 int orc_perm_stats(const orc_study* st, int c, int precision, long K, double* out);
 
+/* FP64 max over the in-mask voxels of the statistic of the first K
+ * permutations of map c (not quantised), maxv[K] */
+int orc_perm_max64(const orc_study* st, int c, long K, double* maxv);
+
 /* the timed CPU restatement: permutation phase only, one contrast, voxel
  * inference, ref32 precision.  Returns max-T values in maxdist[K]. */
 int orc_permutation_phase32(const orc_study* st, int contrast, long K,

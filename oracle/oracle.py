@@ -233,6 +233,28 @@ def perm_stats(st: dict, perms: dict, K: int, c: int = 0, precision: int = F64) 
     return out
 
 
+def quantised_max(m) -> np.ndarray:
+    """The null value the reference stores for a maximum statistic m
+    (KM:299 + LIB:10988), vectorised: (float)(int)(float(m) * 10000.0f) /
+    10000.0f in FP32 arithmetic, the int conversion saturating as the GPU's
+    v_cvt_i32_f32 does (the library's choice for |t| > 214748.3647, where
+    C leaves the conversion undefined).  Monotone in m."""
+    mf = np.asarray(m, np.float64).astype(np.float32)
+    prod = (mf * np.float32(10000.0)).astype(np.float64)
+    q = np.clip(np.trunc(prod), -2.0 ** 31, 2.0 ** 31 - 1)
+    return (q.astype(np.float32) / np.float32(10000.0)).astype(np.float32)
+
+
+def perm_max64(st: dict, perms: dict, K: int, c: int = 0, nthreads: int = 0) -> np.ndarray:
+    """FP64 maximum statistic (unquantised) of the first K permutations of
+    map c over the in-mask voxels: (K,)."""
+    b = _Bound(st, perms, K, nthreads)
+    out = np.zeros(K)
+    rc = lib().orc_perm_max64(ctypes.byref(b.s), c, c_long(K), P(out))
+    assert rc == 0
+    return out
+
+
 def randomise(st: dict, perms: dict, K, precision: int = F64):
     """The whole path: (maps, pvalues, maxdist list)."""
     b = _Bound(st, perms, K)

@@ -165,6 +165,10 @@ int main(int argc, char** argv) {
         write_bin(dir + "/dist" + std::to_string(c) + ".bin", dist[c].data(), dist[c].size());
     write_bin(dir + "/perm_used.bin", pm[0].data(), pm[0].size());
     write_bin(dir + "/sign_used.bin", sign.data(), sign.size());
+    long pties = 0, cties = 0, r32 = 0, r64 = 0;
+    BROCCOLI.GetPermutationTies(&pties, &cties);
+    BROCCOLI.GetPermutationRepeats(&r32, &r64);
     printf("device %s, K %zu, errors %d\n", BROCCOLI.GetOpenCLDeviceName(), Kc[0], errors);
+    printf("pvalue ties %ld, cluster tie maps %ld, repeated fp32 %ld fp64 %ld\n", pties, cties, r32, r64);
     return errors ? 1 : 0;
 }

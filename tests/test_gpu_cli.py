@@ -94,6 +94,13 @@ def test_cli_end_to_end_matches_oracle(tmp_path, case):
     assert np.array_equal(null.astype(np.float32), md_c)
     if case != "ftest_cluster":
         assert np.all(np.abs(np.round(null * 1e4) - np.round(md64[0].astype(np.float64) * 1e4)) <= 1)
+    else:
+        # cluster sizes equal the FP64 path's except for permutations whose
+        # map has an FP64 statistic within the error band of the threshold
+        ref = oracle.perm_stats(st, perms, K)[:, st["mask"].ravel() == 1.0]
+        near = (np.abs(ref - st["cdt"]) <= 1e-4 * (1.0 + abs(st["cdt"]))).any(axis=1)
+        diff = null.astype(np.float32) != md64[0]
+        assert not np.any(diff & ~near), np.flatnonzero(diff & ~near)
     # p-values: exact counts against the null the run produced
     mask = st["mask"].ravel() == 1.0
     D, H, W = st["mask"].shape

@@ -1,5 +1,6 @@
 """The BROCCOLI_LIB drop-in, driven like the reference CLI (tests/cpp/
 dropin_driver.cpp), against the oracle's whole path on the same inputs."""
+import re
 import subprocess
 from pathlib import Path
 
@@ -64,15 +65,30 @@ def test_dropin_generated_permutations_match_reference_stream(tmp_path, wl):
     maps64, P64, md64 = oracle.randomise(st, used, K, precision=oracle.F64)
     err = np.abs(maps - maps64) / (1 + np.abs(maps64))
     assert err.max() <= 1e-6
+    pties, cties = (int(x) for x in re.search(r"pvalue ties (\d+), cluster tie maps (\d+)", out).groups())
+    m = st["mask"].ravel() == 1.0
     if st.get("inference", 0) == 0:
         q = np.round(dist.astype(np.float64) * 1e4)
         q64 = np.round(md64[0].astype(np.float64) * 1e4)
         assert np.all(np.abs(q - q64) <= 1)
-        m = st["mask"].ravel() == 1.0
-        assert (np.abs(pv[0][m] - P64[0][m]) * K).max() <= (q != q64).sum()
+        # p-value counts: equal to the FP64 path's except for comparisons
+        # inside the error band of the null value, which are reported
+        T = maps64[0][m].astype(np.float64)
+        band = 1e-4 * (2.0 + np.abs(T))
+        near = (np.abs(T[:, None] - md64[0][None, :].astype(np.float64)) <= band[:, None]).sum(1)
+        dcount = np.abs(pv[0][m] - P64[0][m]) * K
+        assert np.all(dcount <= near + 1e-3), "p-value count differs outside the tie band"
+        assert (dcount > 1e-3).sum() <= pties
     else:
-        # cluster sizes: integers, equal unless a voxel sits on the threshold
-        assert np.mean(dist == md64[0]) >= 0.9
+        # cluster sizes are integers: a permutation's largest cluster may
+        # differ from the FP64 one only where some FP64 statistic of its map
+        # lies within the error band of the threshold (a reported tie)
+        ref = oracle.perm_stats(st, used, K)[:, m]
+        cdt = st["cdt"]
+        near = (np.abs(ref - cdt) <= 1e-4 * (1.0 + abs(cdt))).any(axis=1)
+        diff = dist != md64[0]
+        assert not np.any(diff & ~near), np.flatnonzero(diff & ~near)
+        assert diff.sum() <= cties
 
 
 def test_dropin_permutation_file_and_fp32(tmp_path):

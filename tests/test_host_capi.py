@@ -193,3 +193,52 @@ def test_dropin_reports_initialisation_failure(tmp_path):
     res = subprocess.run([str(drv), str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert res.returncode == 4
     assert "Initialization error" in res.stdout
+
+
+def _small_study(rng):
+    from broccoli_amd import synth
+    st = synth.study(2, scale=0.1)
+    st["inference"] = capi.VOXEL
+    return st
+
+
+def test_validate_study_accepts_the_benchmark_designs():
+    from broccoli_amd import synth
+    for wl in (1, 2, 3):
+        st = synth.study(wl, scale=0.1)
+        rc, msg = capi.validate_study(st)
+        assert rc == capi.OK, (wl, msg)
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_validate_study_refuses_non_finite_data_inside_the_mask(bad):
+    """A NaN statistic's quantised value (int)(T * 1e4) (KM:299) depends on
+    the device, so non-finite data inside the mask are refused; outside the
+    mask they are never read."""
+    st = _small_study(None)
+    Y = st["Y"].copy()
+    inside = np.argwhere(st["mask"] == 1.0)[5]
+    Y[3][tuple(inside)] = bad
+    st["Y"] = Y
+    rc, msg = capi.validate_study(st)
+    assert rc == -1 and "NaN or infinite" in msg
+    Y = st["Y"].copy()
+    Y[3][tuple(inside)] = 0.5
+    outside = np.argwhere(st["mask"] != 1.0)[0]
+    Y[3][tuple(outside)] = bad
+    st["Y"] = Y
+    assert capi.validate_study(st)[0] == capi.OK
+
+
+def test_validate_study_refuses_bad_designs():
+    st = _small_study(None)
+    X = st["X"].copy()
+    X[:, 3] = X[:, 2]  # rank deficient
+    st["X"] = X
+    rc, msg = capi.validate_study(st)
+    assert rc == -4 and "rank deficient" in msg
+    st = _small_study(None)
+    st["N"] = st["p"]  # N <= p
+    assert capi.validate_study(st)[0] == -1
+    st = _small_study(None)
+    assert capi.validate_study(st, precision=7)[0] == -1

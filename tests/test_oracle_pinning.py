@@ -431,3 +431,32 @@ def test_pvalues_count():
     cnt = (mp.ravel()[m][:, None] > md[None, :]).sum(1)
     assert np.array_equal(P[m], cnt.astype(np.float32) / np.float32(50))
     assert np.all(P[~m] == 0)
+
+
+@pytest.mark.parametrize("wl", [1, 2, 3])
+def test_perm_max64_is_the_max_of_the_fp64_maps(wl):
+    """orc_perm_max64 (the full-size accuracy check's FP64 maxima) equals the
+    max over the in-mask voxels of orc_perm_stats' FP64 maps (pinned above
+    against scipy, closed forms and brute-force least squares)."""
+    from broccoli_amd import synth
+
+    st = synth.study(wl, scale=0.12)
+    K = 24
+    perms = synth.permutations(st, K, seed=41)
+    ref = oracle.perm_stats(st, perms, K)[:, st["mask"].ravel() == 1.0]
+    got = oracle.perm_max64(st, perms, K)
+    assert np.allclose(got, ref.max(axis=1), rtol=1e-12, atol=1e-12)
+
+
+def test_vectorised_quantised_max_is_the_oracle_quantisation():
+    """oracle.quantised_max equals orc_quantize (the C restatement of
+    (int)(T * 10000.0f), KM:299) followed by / 10000.0f, LIB:10988, on
+    values across the range, including products past 2^24 where FP32
+    rounding of T * 1e4 matters; beyond the int range it saturates."""
+    rng = np.random.default_rng(3)
+    t = np.concatenate([rng.normal(0, 5, 2000), rng.uniform(-3e3, 3e3, 2000), rng.uniform(1e3, 2e5, 2000),
+                        [0.0, 1e-5, -1e-5, 4.99995, 525.5731937, 214748.3]]).astype(np.float32)
+    lib = oracle.lib()
+    want = np.array([np.float32(lib.orc_quantize(float(x))) / np.float32(10000.0) for x in t], np.float32)
+    assert np.array_equal(oracle.quantised_max(t), want)
+    assert oracle.quantised_max(np.array([3e5]))[0] == np.float32(np.float32(2 ** 31 - 1) / np.float32(10000.0))
