@@ -462,14 +462,25 @@ def main():
     }
     md32 = None
     if not args.no_fp32:
-        plan, md32, dt32, kern32, _ = run_form(ctx, st, perms, "fp32", max(2, min(10, args.steps // 2)), 1)
+        plan, md32, dt32, kern32, tm32 = run_form(ctx, st, perms, "fp32", max(2, min(10, args.steps // 2)), 1)
         plan.close()
         print(f"# fp32: {dt32*1e3:.3f} ms/step, contraction {kern32:.3f} ms", file=sys.stderr, flush=True)
         a32 = flops / (kern32 * 1e-3) / 1e12
         res["fp32_form"] = {"value": K * V / dt32, "ms_per_step": dt32 * 1e3, "contract_ms": kern32,
                             "achieved_tflops": a32, "frac_of_fp32_peak": a32 / PEAK_FP32_TFLOPS}
+        res["fp32_form"]["kernel"] = ("k_contract_f32mx (FP32 matrix cores)" if tm32.fit_columns <= 16
+                                      else "k_contract_f32 (vector ALUs)")
         res["speedup_vs_fp32"] = dt32 / dt
         res["maxT_equal_fp32_vs_bf16"] = int(np.sum(md32 == md))
+        # the FP32 form on the vector ALUs as well (the plain per-voxel loop)
+        plan, mdv, dtv, kernv, _ = run_form(ctx, st, perms, "fp32_valu", max(2, min(10, args.steps // 2)), 1)
+        plan.close()
+        print(f"# fp32 (vector ALUs): {dtv*1e3:.3f} ms/step, contraction {kernv:.3f} ms", file=sys.stderr, flush=True)
+        av = flops / (kernv * 1e-3) / 1e12
+        res["fp32_valu_form"] = {"value": K * V / dtv, "ms_per_step": dtv * 1e3, "contract_ms": kernv,
+                                 "achieved_tflops": av, "frac_of_fp32_peak": av / PEAK_FP32_TFLOPS,
+                                 "maxT_equal_to_fp32_form": int(np.sum(mdv == md32))}
+        res["speedup_vs_fp32_valu"] = dtv / dt
     if not args.no_accuracy:
         res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8)
         if args.workload in (1, 2):

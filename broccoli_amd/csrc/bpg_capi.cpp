@@ -65,6 +65,7 @@ struct bpg_plan {
         long K = 0;
         bool cluster = false, mass = false, tfce = false;
         int form = BPG_PREC_FP32;  // arithmetic of this run (BPG_PREC_*)
+        bool mx = false;           // FP32 on the matrix cores
         bool exact = false;        // split-BF16 with every product exact (three data parts)
         int map = 0;
         long fp32_perms = 0;  // permutations whose results come from the FP32 form
@@ -83,6 +84,7 @@ struct bpg_plan {
     DBuf<double> yy, Qt, Qn, dscale;
     DBuf<unsigned> Yb, Gb;
     DBuf<float> Y32, G32;
+    DBuf<float> Ym, Gm;
     DBuf<double> Y64;
     DBuf<unsigned char> ties;
     HBuf<unsigned char> h_ties;
@@ -109,7 +111,7 @@ struct bpg_plan {
 
     void release_all() {
         h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release(); h_unsafe.release(); h_flags.release();
-        h_ties.release(); Y64.release(); ties.release();
+        h_ties.release(); Y64.release(); ties.release(); Ym.release(); Gm.release();
         Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
@@ -167,7 +169,8 @@ int check_study_args(const bpg_study_desc& d, std::string& err) {
     if (d.inference != BPG_VOXEL && d.inference != BPG_CLUSTER_EXTENT && d.inference != BPG_CLUSTER_MASS &&
         d.inference != BPG_TFCE)
         return fail("unknown inference mode");
-    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16 && d.precision != BPG_PREC_FP64)
+    if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16 && d.precision != BPG_PREC_FP64 &&
+        d.precision != BPG_PREC_FP32_VALU)
         return fail("unknown precision");
     if (d.connectivity != 26 && d.connectivity != 6) return fail("connectivity must be 26 or 6");
     if (d.W <= 0 || d.H <= 0 || d.D <= 0) return fail("grid dimensions must be positive");
@@ -304,6 +307,25 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
         ctx->err = "upload of the study failed";
         return fail(BPG_ERR_HIP);
     }
+    if (pl->Vc > 0) {
+        // the data-side buffers of the plan's form at their largest size: a
+        // phase then allocates only what depends on K
+        const int N = d.N, KK = (N + 15) / 16, KG = (N + 7) / 8;
+        size_t yb = 0;
+        int max_fit = 0;
+        for (const auto& m : pl->models) max_fit = std::max(max_fit, m.pfit);
+        for (const auto& m : pl->models)
+            yb = std::max(yb, (size_t)(pl->Vpad / 32) * KK * bpg::bf16_y_pieces(KK, m.sign_operand) * 64 * 4);
+        if (pl->yy.ensure(pl->Vpad) ||
+            (pl->d.precision == BPG_PREC_SPLIT_BF16 && pl->Yb.ensure(yb)) ||
+            (pl->d.precision == BPG_PREC_FP32 && max_fit <= 16 &&
+             pl->Ym.ensure((size_t)(pl->Vpad / 32) * KG * 64 * 4)) ||
+            ((pl->d.precision == BPG_PREC_FP32_VALU || (pl->d.precision == BPG_PREC_FP32 && max_fit > 16)) &&
+             pl->Y32.ensure((size_t)f32_rows(N) * pl->Vpad))) {
+            ctx->err = "device allocation of the study failed";
+            return fail(BPG_ERR_NOMEM);
+        }
+    }
     *out = pl;
     return BPG_OK;
 }
@@ -395,12 +417,21 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
                                          bpg::contract_bf16_blocks_per_cu(KK, PP, m.sign_operand,
                                                                           d.test == BPG_FTEST, want_stats || cluster))
            : bpg::Bf16Launch{};
-    const bpg::F32Launch Lf = bf ? bpg::F32Launch{}
+    // FP32 on the matrix cores up to 16 fit columns, else on the vector ALUs
+    const bool mx = form == BPG_PREC_FP32 && pfit <= 16;
+    const int KG = (N + 7) / 8;
+    const bpg::Bf16Launch Lm =
+        mx ? bpg::plan_contract_tiles((int)K, PP, bpg::f32mx_rows_per_wave(KG), Vpad,
+                                      ctx->cus / 8 *
+                                          bpg::contract_f32mx_blocks_per_cu(PP, KG, d.test == BPG_FTEST,
+                                                                            want_stats || cluster))
+           : bpg::Bf16Launch{};
+    const bpg::F32Launch Lf = bf || mx ? bpg::F32Launch{}
                               : f64 ? bpg::plan_contract_f64((int)K, pfit, Vpad)
                                     : bpg::plan_contract_f32((int)K, pfit, Vpad);
-    const int n_sub = Lb.n_sub;
-    const int n_part = bf ? Lb.n_vchunks : Lf.n_vchunks;
-    const int PT = bf || f64 ? 1 : bpg::f32_perm_tile(pfit);
+    const int n_sub = bf ? Lb.n_sub : Lm.n_sub;
+    const int n_part = bf ? Lb.n_vchunks : mx ? Lm.n_vchunks : Lf.n_vchunks;
+    const int PT = bf || f64 || mx ? 1 : bpg::f32_perm_tile(pfit);
     const long Kpad = (K + PT - 1) / PT * PT;
     const int NS64 = (N + 7) / 8 * 8;  // FP64 data rows (zero beyond N)
 
@@ -418,6 +449,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         BPG_HIP(ctx, pl->Gb.ensure((size_t)n_sub * KK * 2 * 64 * 4));
     } else if (f64) {
         BPG_HIP(ctx, pl->Y64.ensure((size_t)NS64 * Vpad));
+    } else if (mx) {
+        BPG_HIP(ctx, pl->Ym.ensure((size_t)(Vpad / 32) * KG * 64 * 4));
+        BPG_HIP(ctx, pl->Gm.ensure((size_t)n_sub * KG * 64 * 4));
     } else {
         BPG_HIP(ctx, pl->Y32.ensure((size_t)NS * Vpad));
         BPG_HIP(ctx, pl->G32.ensure((size_t)Kpad * pfit * NS));
@@ -461,8 +495,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;
         ta.N = N; ta.q = m.q; ta.Vc = Vc; ta.Vpad = Vpad;
         ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : f64 ? NS64 : NS; ta.npy = npy;
-        ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf || f64 ? nullptr : pl->Y32.p;
+        ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf || f64 || mx ? nullptr : pl->Y32.p;
         ta.Y64 = f64 ? pl->Y64.p : nullptr;
+        ta.Ym = mx ? pl->Ym.p : nullptr;
         BPG_HIP(ctx, bpg::launch_transform(ta, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
@@ -499,6 +534,10 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             BPG_HIP(ctx, bpg::launch_operand_bf16(oa, st));
         } else if (f64) {
             // the FP64 kernel builds its operand tiles itself
+        } else if (mx) {
+            oa.n_sub = n_sub;
+            oa.Gm = pl->Gm.p;
+            BPG_HIP(ctx, bpg::launch_operand_f32mx(oa, KG, st));
         } else {
             oa.n_sub = (int)Kpad;
             oa.G32 = pl->G32.p;
@@ -510,6 +549,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     {
         bpg::ContractArgs ca{};
         ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p; ca.Y64 = pl->Y64.p;
+        ca.Gm = pl->Gm.p; ca.Ym = pl->Ym.p;
         ca.perm = m.sign_operand ? nullptr : pl->perm.p;
         ca.sign = m.sign_operand ? pl->sign.p : nullptr;
         ca.Qt = pl->Qt.p;
@@ -539,6 +579,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         if (cluster) BPG_HIP(ctx, hipMemsetAsync(pl->ties.p, 0, K, st));
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
         else if (f64) BPG_HIP(ctx, bpg::launch_contract_f64(ca, Lf, st));
+        else if (mx) BPG_HIP(ctx, bpg::launch_contract_f32mx(ca, Lm, PP, KG, st));
         else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
         launches = 1;
         if (!cluster) BPG_HIP(ctx, bpg::launch_reduce_max(pl->partial.p, n_part, (int)K, pl->maxout.p, st));
@@ -631,7 +672,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     auto& ps = pl->pending;
     ps.active = true;
     ps.map = map;
-    ps.fp32_perms = form == BPG_PREC_FP32 ? K : 0;
+    ps.fp32_perms = form == BPG_PREC_FP32 || form == BPG_PREC_FP32_VALU ? K : 0;
     ps.fp64_perms = f64 ? K : 0;
     ps.have_times = false;
     ps.perm = perm;
@@ -644,6 +685,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     ps.form = form;
     ps.exact = bf && npy == 3;
     ps.pfit = pfit;
+    ps.mx = mx;
     ps.PP = PP;
     ps.launches = launches;
     ps.t0 = t0;
@@ -736,7 +778,7 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
         }
         ps.fp32_perms = ps.fp64_perms = 0;
         for (long k = 0; k < ps.K; k++) {
-            ps.fp32_perms += final_form[k] == BPG_PREC_FP32;
+            ps.fp32_perms += final_form[k] == BPG_PREC_FP32 || final_form[k] == BPG_PREC_FP32_VALU;
             ps.fp64_perms += final_form[k] == BPG_PREC_FP64;
         }
     }
@@ -782,7 +824,7 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     tm.contract_launches = ps.launches;
     tm.perm_voxels = K * (long)Vc;
     tm.fit_columns = ps.pfit;
-    tm.slots_per_perm = ps.form == BPG_PREC_SPLIT_BF16 ? ps.PP : ps.pfit;
+    tm.slots_per_perm = ps.form == BPG_PREC_SPLIT_BF16 || ps.mx ? ps.PP : ps.pfit;
     tm.precision_used = ps.form;
     tm.unsafe_waves = unsafe_first;
     tm.fp32_permutations = ps.fp32_perms;

@@ -57,6 +57,7 @@ struct TransformArgs {
     unsigned* Yb;          // BF16 fragments, (Vpad/32)*KK*npy*64 uint4
     float* Y32;            // [NS][Vpad]
     double* Y64;           // [NS][Vpad] (the FP64 form; dscale all ones)
+    float* Ym;             // FP32-MFMA data fragments [Vpad/32][KG][64] float4 (KG = ceil(N / 8))
 };
 // padded width of the nuisance basis for q columns (0 for none, -1 if q > 24)
 int transform_width(int q);
@@ -73,6 +74,7 @@ struct OperandArgs {
     int NS;                      // FP32 layout row length
     unsigned* Gb;                // BF16 fragments [n_sub][KK][2][64] uint4
     float* G32;                  // FP32 [K][pfit][NS]
+    float* Gm;                   // FP32-MFMA fragments [n_sub][KG][64] float4
 };
 hipError_t launch_operand_bf16(const OperandArgs& a, hipStream_t s);
 hipError_t launch_operand_f32(const OperandArgs& a, hipStream_t s);
@@ -82,6 +84,8 @@ struct ContractArgs {
     const unsigned* Yb;    // BF16 data fragments
     const float* G32;      // FP32 operand
     const float* Y32;      // FP32 data [NS][Vpad]
+    const float* Gm;       // FP32-MFMA operand fragments
+    const float* Ym;       // FP32-MFMA data fragments
     const double* Y64;     // FP64 data [NS][Vpad] (unscaled y*)
     const unsigned short* perm;  // FP64 form: [K][N] permutations (or null)
     const float* sign;           // FP64 form: [K][N] signs (or null)
@@ -151,6 +155,21 @@ struct F32Launch {
 int f32_perm_tile(int pfit);
 F32Launch plan_contract_f32(int K, int pfit, int Vpad);
 hipError_t launch_contract_f32(const ContractArgs& a, const F32Launch& L, hipStream_t s);
+
+// FP32 form on the matrix cores (bpg_contract_f32mx.hip): the split-BF16
+// kernel's grid and operand row order, one v_mfma_f32_32x32x2_f32 per
+// k-step of 2 subjects, operands streamed per group of 8 subjects.  For fit
+// columns up to 16 (the accumulator slots of one lane); wider designs use
+// the vector-ALU kernel above.
+// Operand subtiles per wave: 2 up to 80 subjects (each wave's subtiles take
+// 2 KG KiB of LDS, 64 KiB per block at KG = 8: two blocks per CU), 1 above.
+constexpr int f32mx_rows_per_wave(int KG) { return KG <= 10 ? 2 : 1; }
+constexpr size_t f32mx_lds_bytes(int KG) { return (size_t)4 * f32mx_rows_per_wave(KG) * KG * 64 * 16; }
+Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd);
+hipError_t launch_contract_f32mx(const ContractArgs& a, const Bf16Launch& L, int PP, int KG, hipStream_t s,
+                                 int* occ = nullptr);
+int contract_f32mx_blocks_per_cu(int PP, int KG, bool ftest, bool out);
+hipError_t launch_operand_f32mx(const OperandArgs& a, int KG, hipStream_t s);
 
 // FP64 form (the repeat of permutations with ill-conditioned samples):
 // blocks of f64_perm_tile(pfit) permutations over chunks of 256-voxel

@@ -120,6 +120,18 @@ __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
                     Yb4[base + 64] = make_uint4(mi[0], mi[1], mi[2], mi[3]);  // the two-part lo
                 }
             }
+    } else if (a.Ym) {
+        // FP32-MFMA fragments: per group of 8 subjects, lane (h, col) holds
+        // subjects 8g + h, 8g + 2 + h, 8g + 4 + h, 8g + 6 + h
+        const int vt = v >> 5, col = v & 31, KG = (a.N + 7) / 8;
+        float4* Ym4 = reinterpret_cast<float4*>(a.Ym);
+        for (int g = 0; g < KG; g++) {
+            double o[8];
+            residual8(8 * g, o, yy);
+            const size_t base = ((size_t)vt * KG + g) * 64 + col;
+            Ym4[base] = make_float4((float)o[0], (float)o[2], (float)o[4], (float)o[6]);
+            Ym4[base + 32] = make_float4((float)o[1], (float)o[3], (float)o[5], (float)o[7]);
+        }
     } else {
         for (int s0 = 0; s0 < a.NS; s0 += 8) {
             double o[8];
@@ -270,8 +282,11 @@ hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, 
 
 
 Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd) {
+    return plan_contract_tiles(K, PP, bf16_rows_per_wave(KK, sign), Vpad, slots_per_xcd);
+}
+
+Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd) {
     Bf16Launch L{};
-    const int R = bf16_rows_per_wave(KK, sign);
     L.n_sub = (K + 2 * (16 / PP) - 1) / (2 * (16 / PP));
     L.n_vt = Vpad / 32;
     L.n_mgroups = (L.n_sub + 4 * R - 1) / (4 * R);

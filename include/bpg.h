@@ -51,11 +51,14 @@ extern "C" {
 #define BPG_CORRELATION 1
 #define BPG_MEAN_DESIGN 2
 /* arithmetic of the permutation contraction */
-#define BPG_PREC_FP32 0       /* FP32 VALU contraction                         */
+#define BPG_PREC_FP32 0       /* FP32 contraction (FP32 MFMA)                  */
 #define BPG_PREC_SPLIT_BF16 1 /* hi/lo BF16 MFMA with FP32 accumulation        */
 #define BPG_PREC_FP64 2       /* FP64 VALU, residual formed directly (also the
                                  repeat of ill-conditioned permutations)      */
 #define BPG_PREC_SPLIT_F16 3  /* hi/lo FP16 MFMA, first-level plans only       */
+#define BPG_PREC_FP32_VALU 4  /* FP32 on the vector ALUs (second level; plain
+                                 BPG_PREC_FP32 runs on the FP32 matrix cores
+                                 up to 16 fit columns, here above)            */
 
 /* limits of the reference (RGL:630-636 p <= 25, RGL:726-732 F rows <= 10) */
 #define BPG_MAX_REGRESSORS 25
