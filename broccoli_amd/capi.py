@@ -32,7 +32,7 @@ ERRORS = {-1: "BPG_ERR_ARG", -2: "BPG_ERR_HIP", -3: "BPG_ERR_NOMEM", -4: "BPG_ER
 EXPORTS = [
     "bpg_create", "bpg_destroy", "bpg_last_error", "bpg_device_name",
     "bpg_plan_create", "bpg_validate_study", "bpg_plan_destroy", "bpg_plan_num_voxels",
-    "bpg_plan_permutation_phase", "bpg_plans_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
+    "bpg_plan_permutation_phase", "bpg_plan_permutation_phase_noalloc", "bpg_plans_permutation_phase", "bpg_plan_last_timing", "bpg_plan_observed_maps",
     "bpg_plan_glm",
     "bpg_plan_pvalues", "bpg_plan_pvalues_ties", "bpg_plan_last_tie_flags", "bpg_gen_sign_matrix", "bpg_gen_perm_two_sample",
     "bpg_gen_perm_correlation", "bpg_detect_designs", "bpg_max_permutations",
@@ -97,6 +97,7 @@ def load() -> ctypes.CDLL:
         "bpg_plan_num_voxels": (c_long, [vp]),
         "bpg_plan_permutation_phase": (c_int, [vp, c_int, vp, vp, c_long, vp, vp]),
         "bpg_plans_permutation_phase": (c_int, [vp, c_int, vp, vp, vp, vp, vp]),
+        "bpg_plan_permutation_phase_noalloc": (c_int, [vp, c_int, vp, vp, c_long, vp]),
         "bpg_plan_glm": (c_int, [vp, vp, vp, vp, vp]),
         "bpg_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
         "bpg_plan_observed_maps": (c_int, [vp, vp]),
@@ -281,6 +282,12 @@ class Plan:
         _check(self._lib.bpg_plan_permutation_phase(self.handle, map_index, _ptr(perm), _ptr(sign), K,
                                                     _ptr(maxdist), _ptr(stats)), self.ctx.handle)
         return maxdist, stats
+
+    def permutation_phase_noalloc(self, map_index: int, d_perm, d_sign, K: int, d_maxdist):
+        """bpg_plan_permutation_phase_noalloc on device pointers (ints or
+        c_void_p): permutations or signs in, K null values out."""
+        _check(self._lib.bpg_plan_permutation_phase_noalloc(self.handle, map_index, d_perm, d_sign, K, d_maxdist),
+               self.ctx.handle)
 
     def timing(self) -> Timing:
         t = Timing()

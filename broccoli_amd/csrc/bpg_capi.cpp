@@ -102,6 +102,11 @@ struct bpg_plan {
     DBuf<double> oX, oXp, oC, osc;
     DBuf<float> d_obs, sorted, P, glm_out;
     DBuf<unsigned> d_pties;
+    // per-map models on the device, [Qt | Qn (NR x QB, row-major) | dscale]
+    // at mdev_off[map] (uploaded once at creation; the no-allocation phase
+    // reads them)
+    DBuf<double> mdev;
+    std::vector<size_t> mdev_off;
     HBuf<double> h_model;
     HBuf<unsigned short> h_perm;
     HBuf<float> h_sign, h_max;
@@ -118,7 +123,7 @@ struct bpg_plan {
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
         cl.release(); maxsize.release(); labels.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
-        d_obs.release(); sorted.release(); P.release(); glm_out.release(); d_pties.release();
+        d_obs.release(); sorted.release(); P.release(); glm_out.release(); d_pties.release(); mdev.release();
     }
 };
 
@@ -149,6 +154,16 @@ extern "C" int bpg_create(int device, bpg_ctx** out) {
     if (c->arch.compare(0, 6, "gfx950") != 0) {
         delete c;
         return BPG_ERR_UNSUPPORTED;
+    }
+    // the no-allocation entry point's scratch: its address on this device,
+    // taken with the device current
+    {
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = bpg::scratch_address(device, &c->scratch);
+        if (e != hipSuccess) {
+            c->scratch = nullptr;
+            c->scratch_err = std::string("device scratch unavailable: ") + hipGetErrorString(e);
+        }
     }
     *out = c;
     return BPG_OK;
@@ -308,15 +323,26 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
         return fail(BPG_ERR_HIP);
     }
     if (pl->Vc > 0) {
-        // the data-side buffers of the plan's form at their largest size: a
-        // phase then allocates only what depends on K
-        const int N = d.N, KK = (N + 15) / 16, KG = (N + 7) / 8;
+        // every map's model on the device, and the data-side buffers of the
+        // plan's form at their largest size: a phase then allocates only
+        // what depends on K (and the no-allocation phase nothing)
+        const int N = d.N, NR = (N + 15) / 16 * 16, KK = (N + 15) / 16, KG = (N + 7) / 8;
+        std::vector<double> hm;
         size_t yb = 0;
         int max_fit = 0;
         for (const auto& m : pl->models) max_fit = std::max(max_fit, m.pfit);
-        for (const auto& m : pl->models)
+        for (const auto& m : pl->models) {
+            const int QB = std::max(bpg::transform_width(m.q), 1);
+            pl->mdev_off.push_back(hm.size());
+            hm.insert(hm.end(), m.Qt.begin(), m.Qt.end());
+            const size_t q0 = hm.size();
+            hm.resize(q0 + (size_t)NR * QB + NR, 0.0);
+            for (int j = 0; j < m.q; j++)
+                for (int s = 0; s < N; s++) hm[q0 + (size_t)s * QB + j] = m.Qn[(size_t)j * N + s];
+            std::copy(m.dscale.begin(), m.dscale.end(), hm.begin() + q0 + (size_t)NR * QB);
             yb = std::max(yb, (size_t)(pl->Vpad / 32) * KK * bpg::bf16_y_pieces(KK, m.sign_operand) * 64 * 4);
-        if (pl->yy.ensure(pl->Vpad) ||
+        }
+        if (pl->mdev.ensure(hm.size()) || pl->yy.ensure(pl->Vpad) ||
             (pl->d.precision == BPG_PREC_SPLIT_BF16 && pl->Yb.ensure(yb)) ||
             (pl->d.precision == BPG_PREC_FP32 && max_fit <= 16 &&
              pl->Ym.ensure((size_t)(pl->Vpad / 32) * KG * 64 * 4)) ||
@@ -324,6 +350,10 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
              pl->Y32.ensure((size_t)f32_rows(N) * pl->Vpad))) {
             ctx->err = "device allocation of the study failed";
             return fail(BPG_ERR_NOMEM);
+        }
+        if (hipMemcpy(pl->mdev.p, hm.data(), hm.size() * sizeof(double), hipMemcpyHostToDevice)) {
+            ctx->err = "upload of the model failed";
+            return fail(BPG_ERR_HIP);
         }
     }
     *out = pl;
@@ -843,6 +873,164 @@ extern "C" int bpg_plan_permutation_phase(bpg_plan* pl, int map, const unsigned 
     const int rc = phase_enqueue(pl, map, perm, sign, K, stats);
     if (rc != BPG_OK) return rc;
     return phase_finish(pl, maxdist);
+}
+
+extern "C" int bpg_plan_permutation_phase_noalloc(bpg_plan* pl, int map, const unsigned short* d_perm,
+                                                  const float* d_sign, long K, float* d_maxdist) {
+    if (!pl) return BPG_ERR_ARG;
+    bpg_ctx* ctx = pl->ctx;
+    const bpg_study_desc& d = pl->d;
+    if (!d_maxdist) BPG_FAIL(ctx, BPG_ERR_ARG, "null maxdist");
+    if (pl->pending.active) BPG_FAIL(ctx, BPG_ERR_ARG, "a permutation phase of this plan is in flight");
+    if (map < 0 || map >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
+    if (K <= 0 || K > (1L << 26)) BPG_FAIL(ctx, BPG_ERR_ARG, "number of permutations out of range");
+    if (d.inference != BPG_VOXEL)
+        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "the no-allocation phase supports voxel inference (max-T) only");
+    if (d.precision != BPG_PREC_SPLIT_BF16 && d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_FP32_VALU)
+        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "the no-allocation phase runs the split-BF16 and FP32 forms only");
+    if (!ctx->scratch) BPG_FAIL(ctx, BPG_ERR_HIP, ctx->scratch_err.c_str());
+    const bpg::MapModel& m = pl->models[map];
+    if (m.sign_operand ? !d_sign : !d_perm) BPG_FAIL(ctx, BPG_ERR_ARG, "missing permutation / sign matrix");
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = pl->stream;
+    const int N = d.N, Vc = pl->Vc, Vpad = pl->Vpad;
+    if (Vc == 0) {
+        // empty mask: the reference's empty max for every permutation
+        float e = empty_max();
+        unsigned bits;
+        std::memcpy(&bits, &e, sizeof bits);
+        BPG_HIP(ctx, bpg::launch_fill_u32(reinterpret_cast<unsigned*>(d_maxdist), (size_t)K, bits, st));
+        BPG_HIP(ctx, hipStreamSynchronize(st));
+        return BPG_OK;
+    }
+    const bool bf = d.precision == BPG_PREC_SPLIT_BF16;
+    const int pfit = m.pfit, KK = (N + 15) / 16, NS = f32_rows(N);
+    const int npy = bpg::bf16_y_pieces(KK, m.sign_operand);
+    const int PP = bf16_slots(pfit);
+    const bpg::Bf16Launch Lb =
+        bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad, m.sign_operand,
+                                     ctx->cus / 8 *
+                                         bpg::contract_bf16_blocks_per_cu(KK, PP, m.sign_operand,
+                                                                          d.test == BPG_FTEST, false))
+           : bpg::Bf16Launch{};
+    const bool mx = d.precision == BPG_PREC_FP32 && pfit <= 16;
+    const int KG = (N + 7) / 8;
+    const bpg::Bf16Launch Lm =
+        mx ? bpg::plan_contract_tiles((int)K, PP, bpg::f32mx_rows_per_wave(KG), Vpad,
+                                      ctx->cus / 8 *
+                                          bpg::contract_f32mx_blocks_per_cu(PP, KG, d.test == BPG_FTEST, false))
+           : bpg::Bf16Launch{};
+    const bpg::F32Launch Lf = bf || mx ? bpg::F32Launch{} : bpg::plan_contract_f32((int)K, pfit, Vpad);
+    const int n_part = bf ? Lb.n_vchunks : mx ? Lm.n_vchunks : Lf.n_vchunks;
+    const int PT = bf || mx ? 1 : bpg::f32_perm_tile(pfit);
+    const long Kpad = (K + PT - 1) / PT * PT;
+    // scratch layout, each part 256-byte aligned: operand | chunk maxima |
+    // maxima | flags | counters (unsafe waves, bad indices)
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t op_bytes = bf   ? (size_t)Lb.n_sub * KK * 2 * 64 * 16
+                            : mx ? (size_t)Lm.n_sub * KG * 64 * 16
+                                 : (size_t)Kpad * pfit * NS * sizeof(float);
+    const size_t o_part = up(op_bytes), o_max = o_part + up((size_t)n_part * K * sizeof(float));
+    const size_t o_flags = o_max + up((size_t)K * sizeof(float));
+    const size_t o_cnt = o_flags + up((size_t)K);
+    const size_t total = o_cnt + 256;
+    if (total > bpg::BPG_SCRATCH_BYTES) {
+        ctx->err = "the no-allocation phase needs " + std::to_string(total >> 20) + " MiB of scratch for K = " +
+                   std::to_string(K) + "; its device scratch holds " +
+                   std::to_string(bpg::BPG_SCRATCH_BYTES >> 20) + " MiB (use fewer permutations per call)";
+        return BPG_ERR_UNSUPPORTED;
+    }
+    unsigned char* sc = static_cast<unsigned char*>(ctx->scratch);
+    float* partial = reinterpret_cast<float*>(sc + o_part);
+    float* maxout = reinterpret_cast<float*>(sc + o_max);
+    unsigned char* flags = sc + o_flags;
+    unsigned* cnt = reinterpret_cast<unsigned*>(sc + o_cnt);  // [0] unsafe waves, [1] bad indices
+    const double* mq = pl->mdev.p + pl->mdev_off[map];
+    const int QB = bpg::transform_width(m.q), NR = (N + 15) / 16 * 16;
+    const auto t0 = std::chrono::steady_clock::now();
+    BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
+    {
+        bpg::TransformArgs ta{};
+        ta.Yc = pl->Yc.p;
+        ta.Qn = mq + m.Qt.size();
+        ta.dscale = mq + m.Qt.size() + (size_t)NR * std::max(QB, 1);
+        ta.N = N; ta.q = m.q; ta.Vc = Vc; ta.Vpad = Vpad;
+        ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : NS; ta.npy = npy;
+        ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf || mx ? nullptr : pl->Y32.p;
+        ta.Ym = mx ? pl->Ym.p : nullptr;
+        BPG_HIP(ctx, bpg::launch_transform(ta, st));
+    }
+    BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
+    // flags (K bytes, rounded up to words) and both counters start at zero
+    BPG_HIP(ctx, bpg::launch_fill_u32(reinterpret_cast<unsigned*>(flags), (K + 3) / 4, 0u, st));
+    BPG_HIP(ctx, bpg::launch_fill_u32(cnt, 2, 0u, st));
+    {
+        bpg::OperandArgs oa{};
+        oa.perm = m.sign_operand ? nullptr : d_perm;
+        oa.sign = m.sign_operand ? d_sign : nullptr;
+        oa.Qt = mq;
+        oa.N = N; oa.pfit = pfit; oa.K = (int)K; oa.PP = PP; oa.KK = KK; oa.NS = NS;
+        oa.bad_index = cnt + 1;
+        if (bf) {
+            oa.n_sub = Lb.n_sub;
+            oa.Gb = reinterpret_cast<unsigned*>(sc);
+            BPG_HIP(ctx, bpg::launch_operand_bf16(oa, st));
+        } else if (mx) {
+            oa.n_sub = Lm.n_sub;
+            oa.Gm = reinterpret_cast<float*>(sc);
+            BPG_HIP(ctx, bpg::launch_operand_f32mx(oa, KG, st));
+        } else {
+            oa.n_sub = (int)Kpad;
+            oa.G32 = reinterpret_cast<float*>(sc);
+            BPG_HIP(ctx, bpg::launch_operand_f32(oa, st));
+        }
+    }
+    BPG_HIP(ctx, hipEventRecord(pl->ev[2], st));
+    {
+        bpg::ContractArgs ca{};
+        ca.Gb = reinterpret_cast<const unsigned*>(sc); ca.Yb = pl->Yb.p;
+        ca.G32 = reinterpret_cast<const float*>(sc); ca.Y32 = pl->Y32.p;
+        ca.Gm = reinterpret_cast<const float*>(sc); ca.Ym = pl->Ym.p;
+        ca.yy = pl->yy.p; ca.partial = partial;
+        ca.K = (int)K; ca.Vc = Vc; ca.Vpad = Vpad; ca.N = N; ca.pfit = pfit; ca.r = m.r; ca.NS = NS;
+        ca.ftest = d.test == BPG_FTEST;
+        ca.sqrt_dof = (float)std::sqrt((double)m.dof);
+        ca.f_scale = (float)((double)m.dof / (double)m.r);
+        ca.cdt = d.cdt;
+        ca.tau = bf && npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+        ca.pinf = INFINITY;
+        ca.unsafe = cnt;
+        ca.flags = flags;
+        ca.cdt_lo = ca.cdt_hi = d.cdt;
+        if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
+        else if (mx) BPG_HIP(ctx, bpg::launch_contract_f32mx(ca, Lm, PP, KG, st));
+        else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
+        BPG_HIP(ctx, bpg::launch_reduce_max(partial, n_part, (int)K, maxout, st));
+        BPG_HIP(ctx, bpg::launch_quantize_max(maxout, (int)K, d_maxdist, empty_max(), st));
+    }
+    BPG_HIP(ctx, hipEventRecord(pl->ev[3], st));
+    unsigned hc[2] = {0u, 0u};
+    BPG_HIP(ctx, hipMemcpyAsync(hc, cnt, sizeof hc, hipMemcpyDeviceToHost, st));
+    BPG_HIP(ctx, hipStreamSynchronize(st));
+    bpg_timing& tm = pl->timing;
+    tm = bpg_timing{};
+    tm.phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    tm.transform_ms = elapsed(pl->ev[0], pl->ev[1]);
+    tm.operand_ms = elapsed(pl->ev[1], pl->ev[2]);
+    tm.contract_ms = elapsed(pl->ev[2], pl->ev[3]);
+    tm.contract_launches = 1;
+    tm.perm_voxels = K * (long)Vc;
+    tm.fit_columns = pfit;
+    tm.slots_per_perm = bf || mx ? PP : pfit;
+    tm.precision_used = d.precision;
+    tm.unsafe_waves = (int)hc[0];
+    tm.fp32_permutations = bf ? 0 : K;
+    if (hc[1]) BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range (maxdist is not valid)");
+    if (hc[0])
+        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED,
+                 "ill-conditioned samples: their permutations need repeats in a more accurate form, which "
+                 "allocates; run bpg_plan_permutation_phase (maxdist is not valid)");
+    return BPG_OK;
 }
 
 extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const int* maps,

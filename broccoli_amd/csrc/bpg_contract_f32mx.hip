@@ -216,7 +216,12 @@ __global__ __launch_bounds__(256) void k_operand_f32mx(OperandArgs a, int KG) {
         if (a.sign) {
             val = a.sign[row + sc];
         } else {
-            val = (float)a.Qt[(size_t)a.perm[row + sc] + col];
+            unsigned ix = a.perm[row + sc];
+            if (a.bad_index && ix >= (unsigned)a.N) {
+                atomicAdd(a.bad_index, 1u);
+                ix = 0;
+            }
+            val = (float)a.Qt[(size_t)ix + col];
         }
         v[q] = (live && s < a.N) ? val : 0.0f;
     }

@@ -75,6 +75,9 @@ struct OperandArgs {
     unsigned* Gb;                // BF16 fragments [n_sub][KK][2][64] uint4
     float* G32;                  // FP32 [K][pfit][NS]
     float* Gm;                   // FP32-MFMA fragments [n_sub][KG][64] float4
+    // optional: permutation indices >= N (which the host has not checked)
+    // are clamped and counted here
+    unsigned* bad_index;
 };
 hipError_t launch_operand_bf16(const OperandArgs& a, hipStream_t s);
 hipError_t launch_operand_f32(const OperandArgs& a, hipStream_t s);
@@ -258,6 +261,14 @@ hipError_t launch_level_bits(const float* stats, const int* nlev, int j, float l
 // out[m] = max over v < Vc of x[m][v]
 hipError_t launch_rowmax(const float* x, int nmaps, int Vc, float* out, hipStream_t s);
 hipError_t launch_ccl(const CclArgs& a, hipStream_t s);
+
+// The no-allocation permutation phase (bpg_noalloc.hip): the per-device
+// scratch array (128 MiB), its device address (call with the device
+// current), a fill kernel and the device-side quantisation of the maxima.
+constexpr size_t BPG_SCRATCH_BYTES = (size_t)128 << 20;
+hipError_t scratch_address(int device, void** out);
+hipError_t launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
+hipError_t launch_quantize_max(const float* m, int K, float* out, float empty, hipStream_t s);
 
 // obs > cdt -> compact bitmask [nwords]
 hipError_t launch_threshold_bits(const float* obs, int Vc, float cdt, unsigned* bits, int nwords,

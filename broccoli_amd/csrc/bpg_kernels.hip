@@ -206,6 +206,16 @@ __global__ __launch_bounds__(256) void k_operand_bf16(OperandArgs a) {
         unsigned short ix[8];
 #pragma unroll
         for (int e = 0; e < 8; e++) ix[e] = a.perm[row + min(s0 + e, a.N - 1)];
+        if (a.bad_index) {
+            bool bad = false;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (ix[e] >= a.N) {
+                    bad = true;
+                    ix[e] = 0;
+                }
+            if (bad) atomicAdd(a.bad_index, 1u);
+        }
 #pragma unroll
         for (int e = 0; e < 8; e++) g[e] = a.Qt[(size_t)ix[e] + col];
     }
@@ -248,7 +258,14 @@ __global__ __launch_bounds__(256) void k_operand_f32(OperandArgs a) {
     float val = 0.0f;
     if (k < a.K && s < a.N) {
         if (a.sign) val = a.sign[(size_t)k * a.N + s];
-        else val = (float)a.Qt[(size_t)a.perm[(size_t)k * a.N + s] + (size_t)i * a.N];
+        else {
+            unsigned ix = a.perm[(size_t)k * a.N + s];
+            if (a.bad_index && ix >= (unsigned)a.N) {
+                atomicAdd(a.bad_index, 1u);
+                ix = 0;
+            }
+            val = (float)a.Qt[(size_t)ix + (size_t)i * a.N];
+        }
     }
     a.G32[t] = val;
 }

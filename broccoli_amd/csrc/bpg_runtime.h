@@ -18,6 +18,10 @@ struct bpg_ctx {
     std::string arch;
     size_t lds_per_block = 0;  // shared memory one block may use
     int cus = 256;             // compute units (8 XCDs on gfx950)
+    // device address of this GPU's no-allocation scratch (bpg_noalloc.hip),
+    // or null with the reason in scratch_err
+    void* scratch = nullptr;
+    std::string scratch_err;
 };
 
 namespace bpg {

@@ -170,6 +170,26 @@ int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const int* maps,
                                 const unsigned short* const* perms, const float* const* signs,
                                 const long* K, float* const* maxdist);
 int bpg_plan_last_timing(const bpg_plan* plan, bpg_timing* out);
+/* ---- the permutation phase without any device allocation (no counterpart
+ *      in the reference): for callers that cannot allocate device memory
+ *      while they run (tight serving loops, code that must not call
+ *      hipMalloc / hipFree).  The same kernels as
+ *      bpg_plan_permutation_phase on the plan's device-resident study, for
+ *      voxel inference (max-T) in the plan's split-BF16 or FP32 form.
+ *      d_perm (t, F) or d_sign (group mean) and d_maxdist are DEVICE
+ *      buffers of the caller: [s + k*N] uint16 / float, and K floats that
+ *      receive the null distribution exactly as bpg_plan_permutation_phase
+ *      returns it.  The per-call scratch (operand fragments, chunk maxima,
+ *      flags, counters) lives in one static device array of 128 MiB per GPU
+ *      (allocated with the library's code object, held until the process
+ *      ends); a K whose scratch would exceed it is refused with
+ *      BPG_ERR_UNSUPPORTED.  Synchronous on the plan's stream.  Calls that
+ *      overlap on one GPU share that array and race.  Ill-conditioned
+ *      samples, which bpg_plan_permutation_phase repeats in a more accurate
+ *      form, are refused (BPG_ERR_UNSUPPORTED) and so are permutation
+ *      indices >= N (BPG_ERR_ARG); maxdist is then not valid. */
+int bpg_plan_permutation_phase_noalloc(bpg_plan* plan, int map, const unsigned short* d_perm,
+                                       const float* d_sign, long K, float* d_maxdist);
 
 /* ---- observed statistical maps on the original data and design
  *      (CalculateStatisticalMapsGLM{T,F}TestSecondLevel, LIB:15858-15950;
