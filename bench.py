@@ -109,6 +109,8 @@ WORKLOADS = {
        "(intercept, trend, block task, sinusoid, 8 nuisance), AR(4) noise, t on the task, AR(4) whitening "
        "(8 mm smoothing), 10000 time-point permutations, voxel max-T",
 }
+WORKLOAD_5F = ("2-row F test over group dummies 1 and 2, 182x218x182 grid (~1.83M voxels), N=200, p=8, "
+               "10000 row permutations, voxel max-T")
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector
 PEAK_FP32_MATRIX_TFLOPS = 157.3  # MI355X FP32 matrix
 
@@ -276,13 +278,13 @@ def mfma_busy(st, K, kern_ms, slots, clock_hz=2.4e9, simds=1024):
     return n_sub * n_vt * kk * prods * 32 / (kern_ms * 1e-3 * clock_hz * simds)
 
 
-def accuracy(ctx, st, perms, nsample):
-    """Sampled permutations of the split-BF16 form against the FP64 oracle."""
+def accuracy(ctx, st, perms, nsample, forms=("split_bf16", "fp32")):
+    """Sampled permutations of each form against the FP64 oracle."""
     from oracle import oracle
 
     sub = {k: v[:nsample] for k, v in perms.items() if k in ("perm", "sign")}
     out = {}
-    for prec in ("split_bf16", "fp32"):
+    for prec in forms:
         with capi.Plan(ctx, st, prec) as plan:
             md, stats = plan.permutation_phase(0, want_stats=True, **sub)
         ref = oracle.perm_stats(st, sub, nsample)[:, st["mask"].ravel() == 1.0]
@@ -386,6 +388,8 @@ def main():
                          "3: F test with cluster-extent inference; 4: 64 studies in one batched call; "
                          "5: 182x218x182 grid, N=200, p=8; 6: first-level permutation test")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = full size)")
+    ap.add_argument("--contrast", default="t", choices=["t", "F"],
+                    help="workload 5: the t contrast or the 2-row F contrast over group dummies 1 and 2")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, launch a plain elementwise copy of this many MiB three times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -402,13 +406,13 @@ def main():
     if args.workload == 6:
         run_first_level(args)
         return
-    st = synth.study(args.workload, scale=args.scale)
+    st = synth.study(args.workload, scale=args.scale, contrast=args.contrast)
     K = args.K if args.workload == 2 or args.K != 10000 else st["K"]
     perms = synth.permutations(st, K, seed=20260000 + args.workload)
     if args.workload != 2:
         # extra data points: no CPU baseline or oracle sample at these sizes
         args.no_cpu = True
-        args.no_accuracy = args.no_accuracy or args.workload in (3, 5)
+        args.no_accuracy = args.no_accuracy or args.workload == 3
     V = st["V"]
     N, p = st["N"], st["p"]
     ctx = capi.Context(0)
@@ -435,8 +439,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "accumulate": "fp32", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload] if args.scale == 1.0 else
-                   f"{WORKLOADS[args.workload]} (grid scale {args.scale})",
+        "config": {"workload": (WORKLOADS[args.workload] if args.contrast == "t" else WORKLOAD_5F) +
+                   ("" if args.scale == 1.0 else f" (grid scale {args.scale})"),
                    "V_mask": V, "N": N, "p": p, "K": K, "precision": "split_bf16"},
         "phase_breakdown_ms": {"transform": tm.transform_ms, "operand": tm.operand_ms,
                                "contract": tm.contract_ms, "cluster": tm.cluster_ms,
@@ -462,7 +466,8 @@ def main():
     }
     md32 = None
     if not args.no_fp32:
-        plan, md32, dt32, kern32, tm32 = run_form(ctx, st, perms, "fp32", max(2, min(10, args.steps // 2)), 1)
+        nc = max(2, min(3 if args.workload == 5 else 10, args.steps // 2))  # steps of the comparison forms
+        plan, md32, dt32, kern32, tm32 = run_form(ctx, st, perms, "fp32", nc, 1)
         plan.close()
         print(f"# fp32: {dt32*1e3:.3f} ms/step, contraction {kern32:.3f} ms", file=sys.stderr, flush=True)
         a32 = flops / (kern32 * 1e-3) / 1e12
@@ -473,7 +478,7 @@ def main():
         res["speedup_vs_fp32"] = dt32 / dt
         res["maxT_equal_fp32_vs_bf16"] = int(np.sum(md32 == md))
         # the FP32 form on the vector ALUs as well (the plain per-voxel loop)
-        plan, mdv, dtv, kernv, _ = run_form(ctx, st, perms, "fp32_valu", max(2, min(10, args.steps // 2)), 1)
+        plan, mdv, dtv, kernv, _ = run_form(ctx, st, perms, "fp32_valu", nc, 1)
         plan.close()
         print(f"# fp32 (vector ALUs): {dtv*1e3:.3f} ms/step, contraction {kernv:.3f} ms", file=sys.stderr, flush=True)
         av = flops / (kernv * 1e-3) / 1e12
@@ -481,8 +486,20 @@ def main():
                                  "achieved_tflops": av, "frac_of_fp32_peak": av / PEAK_FP32_TFLOPS,
                                  "maxT_equal_to_fp32_form": int(np.sum(mdv == md32))}
         res["speedup_vs_fp32_valu"] = dtv / dt
+        if args.workload == 5:
+            # the split-FP8 form (e4m3 terms, E8M0 scales per 32 subjects)
+            plan, md8, dt8, kern8, tm8 = run_form(ctx, st, perms, "split_fp8", nc, 1)
+            plan.close()
+            print(f"# split_fp8: {dt8*1e3:.3f} ms/step, contraction {kern8:.3f} ms", file=sys.stderr, flush=True)
+            a8 = flops / (kern8 * 1e-3) / 1e12
+            res["fp8_form"] = {"value": K * V / dt8, "ms_per_step": dt8 * 1e3, "contract_ms": kern8,
+                               "achieved_tflops": a8, "frac_of_fp8_peak": a8 / 5000.0,
+                               "permutations_repeated_in_fp32": tm8.fp32_permutations,
+                               "maxT_equal_to_split_bf16_form": int(np.sum(md8 == md))}
+            res["speedup_vs_fp8"] = dt8 / dt
     if not args.no_accuracy:
-        res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8)
+        forms = ("split_bf16", "fp32", "split_fp8") if args.workload == 5 else ("split_bf16", "fp32")
+        res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8, forms)
         if args.workload in (1, 2):
             full = full_k_check(st, perms, K, md, md32, obs, Pv, pties, host_cores())
             res["accuracy_vs_fp64"]["full_K"] = full

@@ -20,9 +20,10 @@ LIB_PATH = Path(os.environ.get("BPG_LIB", Path(__file__).resolve().with_name("li
 TTEST, FTEST, GROUP_MEAN = 0, 1, 2
 VOXEL, CLUSTER_EXTENT, CLUSTER_MASS, TFCE = 0, 1, 2, 3
 TWOSAMPLE, CORRELATION, MEAN_DESIGN = 0, 1, 2
-PREC_FP32, PREC_SPLIT_BF16, PREC_FP64, PREC_SPLIT_F16, PREC_FP32_VALU = 0, 1, 2, 3, 4
+PREC_FP32, PREC_SPLIT_BF16, PREC_FP64, PREC_SPLIT_F16, PREC_FP32_VALU, PREC_SPLIT_FP8 = 0, 1, 2, 3, 4, 5
 FL_PRECISIONS = {"split_f16": PREC_SPLIT_F16, "split_bf16": PREC_SPLIT_BF16, "fp32": PREC_FP32, "fp64": PREC_FP64}
-PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp64": PREC_FP64, "fp32_valu": PREC_FP32_VALU}
+PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp64": PREC_FP64, "fp32_valu": PREC_FP32_VALU,
+              "split_fp8": PREC_SPLIT_FP8}
 
 OK = 0
 ERRORS = {-1: "BPG_ERR_ARG", -2: "BPG_ERR_HIP", -3: "BPG_ERR_NOMEM", -4: "BPG_ERR_DESIGN",

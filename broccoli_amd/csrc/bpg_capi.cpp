@@ -85,6 +85,7 @@ struct bpg_plan {
     DBuf<unsigned> Yb, Gb;
     DBuf<float> Y32, G32;
     DBuf<float> Ym, Gm;
+    DBuf<unsigned> Yf8, Ysc, Gf8, Gsc;
     DBuf<double> Y64;
     DBuf<unsigned char> ties;
     HBuf<unsigned char> h_ties;
@@ -117,6 +118,7 @@ struct bpg_plan {
     void release_all() {
         h_model.release(); h_perm.release(); h_sign.release(); h_max.release(); h_nlev.release(); h_unsafe.release(); h_flags.release();
         h_ties.release(); Y64.release(); ties.release(); Ym.release(); Gm.release();
+        Yf8.release(); Ysc.release(); Gf8.release(); Gsc.release();
         Yc.release(); d_nbr.release();
         yy.release(); Qt.release(); Qn.release(); dscale.release();
         Yb.release(); Gb.release(); Y32.release(); G32.release();
@@ -185,7 +187,7 @@ int check_study_args(const bpg_study_desc& d, std::string& err) {
         d.inference != BPG_TFCE)
         return fail("unknown inference mode");
     if (d.precision != BPG_PREC_FP32 && d.precision != BPG_PREC_SPLIT_BF16 && d.precision != BPG_PREC_FP64 &&
-        d.precision != BPG_PREC_FP32_VALU)
+        d.precision != BPG_PREC_FP32_VALU && d.precision != BPG_PREC_SPLIT_FP8)
         return fail("unknown precision");
     if (d.connectivity != 26 && d.connectivity != 6) return fail("connectivity must be 26 or 6");
     if (d.W <= 0 || d.H <= 0 || d.D <= 0) return fail("grid dimensions must be positive");
@@ -282,7 +284,8 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
     // (one lane's accumulator slots); wider designs run the FP32 form
     int max_pfit = 0;
     for (const auto& m : pl->models) max_pfit = std::max(max_pfit, m.pfit);
-    if (d.precision == BPG_PREC_SPLIT_BF16 && max_pfit > 16) pl->d.precision = BPG_PREC_FP32;
+    if ((d.precision == BPG_PREC_SPLIT_BF16 || d.precision == BPG_PREC_SPLIT_FP8) && max_pfit > 16)
+        pl->d.precision = BPG_PREC_FP32;
     // compaction: in-mask voxels in linear order (mask == 1.0f, K2:1586)
     pl->G = (size_t)d.W * d.H * d.D;
     std::vector<int> grid2comp;
@@ -346,7 +349,8 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
             (pl->d.precision == BPG_PREC_SPLIT_BF16 && pl->Yb.ensure(yb)) ||
             (pl->d.precision == BPG_PREC_FP32 && max_fit <= 16 &&
              pl->Ym.ensure((size_t)(pl->Vpad / 32) * KG * 64 * 4)) ||
-            ((pl->d.precision == BPG_PREC_FP32_VALU || (pl->d.precision == BPG_PREC_FP32 && max_fit > 16)) &&
+            ((pl->d.precision == BPG_PREC_FP32_VALU || pl->d.precision == BPG_PREC_SPLIT_FP8 ||
+              (pl->d.precision == BPG_PREC_FP32 && max_fit > 16)) &&
              pl->Y32.ensure((size_t)f32_rows(N) * pl->Vpad))) {
             ctx->err = "device allocation of the study failed";
             return fail(BPG_ERR_NOMEM);
@@ -450,18 +454,27 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     // FP32 on the matrix cores up to 16 fit columns, else on the vector ALUs
     const bool mx = form == BPG_PREC_FP32 && pfit <= 16;
     const int KG = (N + 7) / 8;
+    // split-FP8 (the data split from the FP32 rows of the vector-ALU form)
+    const bool f8 = form == BPG_PREC_SPLIT_FP8;
+    const int KS = (N + 63) / 64;
+    const bpg::Bf16Launch L8 =
+        f8 ? bpg::plan_contract_tiles((int)K, PP, bpg::f8_rows_per_wave(KS), Vpad,
+                                      ctx->cus / 8 *
+                                          bpg::contract_f8_blocks_per_cu(PP, KS, m.sign_operand, d.test == BPG_FTEST,
+                                                                         want_stats || cluster))
+           : bpg::Bf16Launch{};
     const bpg::Bf16Launch Lm =
         mx ? bpg::plan_contract_tiles((int)K, PP, bpg::f32mx_rows_per_wave(KG), Vpad,
                                       ctx->cus / 8 *
                                           bpg::contract_f32mx_blocks_per_cu(PP, KG, d.test == BPG_FTEST,
                                                                             want_stats || cluster))
            : bpg::Bf16Launch{};
-    const bpg::F32Launch Lf = bf || mx ? bpg::F32Launch{}
+    const bpg::F32Launch Lf = bf || mx || f8 ? bpg::F32Launch{}
                               : f64 ? bpg::plan_contract_f64((int)K, pfit, Vpad)
                                     : bpg::plan_contract_f32((int)K, pfit, Vpad);
-    const int n_sub = bf ? Lb.n_sub : Lm.n_sub;
-    const int n_part = bf ? Lb.n_vchunks : mx ? Lm.n_vchunks : Lf.n_vchunks;
-    const int PT = bf || f64 || mx ? 1 : bpg::f32_perm_tile(pfit);
+    const int n_sub = bf ? Lb.n_sub : f8 ? L8.n_sub : Lm.n_sub;
+    const int n_part = bf ? Lb.n_vchunks : mx ? Lm.n_vchunks : f8 ? L8.n_vchunks : Lf.n_vchunks;
+    const int PT = bf || f64 || mx || f8 ? 1 : bpg::f32_perm_tile(pfit);
     const long Kpad = (K + PT - 1) / PT * PT;
     const int NS64 = (N + 7) / 8 * 8;  // FP64 data rows (zero beyond N)
 
@@ -482,6 +495,12 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     } else if (mx) {
         BPG_HIP(ctx, pl->Ym.ensure((size_t)(Vpad / 32) * KG * 64 * 4));
         BPG_HIP(ctx, pl->Gm.ensure((size_t)n_sub * KG * 64 * 4));
+    } else if (f8) {
+        BPG_HIP(ctx, pl->Y32.ensure((size_t)NS * Vpad));
+        BPG_HIP(ctx, pl->Yf8.ensure((size_t)(Vpad / 32) * KS * BPG_F8_TERMS * 64 * 8));
+        BPG_HIP(ctx, pl->Ysc.ensure((size_t)(Vpad / 32) * KS * 64));
+        BPG_HIP(ctx, pl->Gf8.ensure((size_t)n_sub * KS * BPG_F8_TERMS * 64 * 8));
+        BPG_HIP(ctx, pl->Gsc.ensure((size_t)n_sub * KS * 64));
     } else {
         BPG_HIP(ctx, pl->Y32.ensure((size_t)NS * Vpad));
         BPG_HIP(ctx, pl->G32.ensure((size_t)Kpad * pfit * NS));
@@ -529,6 +548,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ta.Y64 = f64 ? pl->Y64.p : nullptr;
         ta.Ym = mx ? pl->Ym.p : nullptr;
         BPG_HIP(ctx, bpg::launch_transform(ta, st));
+        if (f8) BPG_HIP(ctx, bpg::launch_data_f8(pl->Y32.p, NS, N, Vpad, KS, pl->Yf8.p, pl->Ysc.p, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
     // while the transform runs: stage (and check) the permutations
@@ -568,6 +588,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             oa.n_sub = n_sub;
             oa.Gm = pl->Gm.p;
             BPG_HIP(ctx, bpg::launch_operand_f32mx(oa, KG, st));
+        } else if (f8) {
+            oa.n_sub = n_sub;
+            BPG_HIP(ctx, bpg::launch_operand_f8(oa, KS, m.sign_operand, pl->Gf8.p, pl->Gsc.p, st));
         } else {
             oa.n_sub = (int)Kpad;
             oa.G32 = pl->G32.p;
@@ -580,6 +603,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         bpg::ContractArgs ca{};
         ca.Gb = pl->Gb.p; ca.Yb = pl->Yb.p; ca.G32 = pl->G32.p; ca.Y32 = pl->Y32.p; ca.Y64 = pl->Y64.p;
         ca.Gm = pl->Gm.p; ca.Ym = pl->Ym.p;
+        ca.Gf8 = pl->Gf8.p; ca.Gsc = pl->Gsc.p; ca.Yf8 = pl->Yf8.p; ca.Ysc = pl->Ysc.p;
         ca.perm = m.sign_operand ? nullptr : pl->perm.p;
         ca.sign = m.sign_operand ? pl->sign.p : nullptr;
         ca.Qt = pl->Qt.p;
@@ -594,7 +618,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.f_scale = (float)((double)m.dof / (double)m.r);
         ca.cdt = d.cdt;
         // with three data fragments every product is exact: the FP32 bound
-        ca.tau = bf && npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+        ca.tau = f8 ? bpg::BPG_TAU_FP8 : bf && npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
         ca.pinf = INFINITY;
         ca.unsafe = pl->unsafe.p;
         ca.flags = pl->flags.p;
@@ -610,6 +634,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
         else if (f64) BPG_HIP(ctx, bpg::launch_contract_f64(ca, Lf, st));
         else if (mx) BPG_HIP(ctx, bpg::launch_contract_f32mx(ca, Lm, PP, KG, st));
+        else if (f8) BPG_HIP(ctx, bpg::launch_contract_f8(ca, L8, PP, KS, m.sign_operand, st));
         else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
         launches = 1;
         if (!cluster) BPG_HIP(ctx, bpg::launch_reduce_max(pl->partial.p, n_part, (int)K, pl->maxout.p, st));
@@ -715,7 +740,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     ps.form = form;
     ps.exact = bf && npy == 3;
     ps.pfit = pfit;
-    ps.mx = mx;
+    ps.mx = mx || f8;
     ps.PP = PP;
     ps.launches = launches;
     ps.t0 = t0;
@@ -802,7 +827,8 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
             if (pl->h_flags.p[k]) S.push_back(k);
         int form = ps.form;
         while (!S.empty() && form != BPG_PREC_FP64) {
-            form = (form == BPG_PREC_SPLIT_BF16 && !ps.exact) ? BPG_PREC_FP32 : BPG_PREC_FP64;
+            form = ((form == BPG_PREC_SPLIT_BF16 && !ps.exact) || form == BPG_PREC_SPLIT_FP8) ? BPG_PREC_FP32
+                                                                                              : BPG_PREC_FP64;
             const int rc = repeat_subset(pl, S, form, final_form);
             if (rc != BPG_OK) return rc;
         }

@@ -10,7 +10,7 @@
 // so LDS is register overflow here, not a shared tile) and read per group
 // of 4 k-steps (8 subjects); the data tile streams from L2 through L1, one
 // float4 per lane and group, shared by the block's 4 waves, for 4 R MFMAs
-// (256 R cycles of matrix work).  Streaming the operand from L2 instead, as
+// (256 R cycles of matrix work), three groups ahead.  Streaming the operand from L2 instead, as
 // a first version did, fetched 57 GB per launch from HBM on the headline
 // workload (the 8 MB operand exceeds an XCD's 4 MB L2) and kept the matrix
 // pipe 63% busy.
@@ -71,40 +71,56 @@ __global__ __launch_bounds__(256) void k_contract_f32mx(ContractArgs a, Bf16Laun
     const int vt0 = vc * L.vt_per_chunk;
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
     for (int vt = vt0; vt < vt1; vt++) {
-        f32x16 acc[R];
+        // two accumulators per subtile (k-steps 0, 1 and 2, 3 of each group),
+        // so that with one subtile per wave consecutive MFMAs do not wait on
+        // each other
+        f32x16 acc[R][2];
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int e = 0; e < 16; e++) acc[r][e] = 0.0f;
-        float4 b = Y[((size_t)vt * KG) * 64 + lane];
-        float4 av[R];
+            for (int e = 0; e < 16; e++) acc[r][0][e] = acc[r][1][e] = 0.0f;
+        // a ring of 4 fragment buffers, 3 groups of prefetch: the loop is
+        // unrolled over the ring so that no buffer is moved (a register move
+        // of a loaded value would wait for its load); with one wave per SIMD
+        // (the large-N form) 3 groups of MFMAs (768 cycles) cover an L2 read
+        float4 bq[4], aq[4][R];
 #pragma unroll
-        for (int r = 0; r < R; r++) av[r] = myA[((size_t)r * KG) * 64];
-        for (int g = 0; g < KG; g++) {
-            // the next group's fragments load while this group multiplies
-            const int gn = min(g + 1, KG - 1);
-            const float4 bn = Y[((size_t)vt * KG + gn) * 64 + lane];
-            float4 an[R];
+        for (int u = 0; u < 3; u++) {
+            const int gl = min(u, KG - 1);
+            bq[u] = Y[((size_t)vt * KG + gl) * 64 + lane];
 #pragma unroll
-            for (int r = 0; r < R; r++) an[r] = myA[((size_t)r * KG + gn) * 64];
+            for (int r = 0; r < R; r++) aq[u][r] = myA[((size_t)r * KG + gl) * 64];
+        }
+        for (int g = 0; g < KG; g += 4) {
 #pragma unroll
-            for (int r = 0; r < R; r++) acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r].x, b.x, acc[r], 0, 0, 0);
+            for (int u = 0; u < 4; u++) {
+                const int gl = min(g + u + 3, KG - 1);
+                bq[(u + 3) & 3] = Y[((size_t)vt * KG + gl) * 64 + lane];
 #pragma unroll
-            for (int r = 0; r < R; r++) acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r].y, b.y, acc[r], 0, 0, 0);
+                for (int r = 0; r < R; r++) aq[(u + 3) & 3][r] = myA[((size_t)r * KG + gl) * 64];
+                if (g + u < KG) {
+                    const float4 b = bq[u];
 #pragma unroll
-            for (int r = 0; r < R; r++) acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r].z, b.z, acc[r], 0, 0, 0);
+                    for (int r = 0; r < R; r++)
+                        acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(aq[u][r].x, b.x, acc[r][0], 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < R; r++) acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r].w, b.w, acc[r], 0, 0, 0);
-            b = bn;
+                    for (int r = 0; r < R; r++)
+                        acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(aq[u][r].y, b.y, acc[r][1], 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < R; r++) av[r] = an[r];
+                    for (int r = 0; r < R; r++)
+                        acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(aq[u][r].z, b.z, acc[r][0], 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+                        acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(aq[u][r].w, b.w, acc[r][1], 0, 0, 0);
+                }
+            }
         }
         const double yv = a.yy[vt * 32 + (lane & 31)];
         const float yhi = (float)yv;  // NaN for padding voxels
         const float ylo = (float)(yv - (double)yhi);
 #pragma unroll
         for (int r = 0; r < R; r++)
-            if (live[r]) epi<PP, FT, OUT>(a, acc[r], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r]);
+            if (live[r]) epi<PP, FT, OUT>(a, acc[r][0] + acc[r][1], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r]);
     }
     // flags of the (subtile, half-wave) pairs that met an ill-conditioned
     // sample, as in the split-BF16 kernel

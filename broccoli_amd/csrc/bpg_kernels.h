@@ -89,6 +89,10 @@ struct ContractArgs {
     const float* Y32;      // FP32 data [NS][Vpad]
     const float* Gm;       // FP32-MFMA operand fragments
     const float* Ym;       // FP32-MFMA data fragments
+    const unsigned* Gf8;   // split-FP8 operand fragments and scale words
+    const unsigned* Gsc;
+    const unsigned* Yf8;   // split-FP8 data fragments and scale words
+    const unsigned* Ysc;
     const double* Y64;     // FP64 data [NS][Vpad] (unscaled y*)
     const unsigned short* perm;  // FP64 form: [K][N] permutations (or null)
     const float* sign;           // FP64 form: [K][N] signs (or null)
@@ -124,6 +128,9 @@ struct ContractArgs {
 // FP32 z ~4e-7
 constexpr float BPG_TAU_BF16 = 0.125f;
 constexpr float BPG_TAU_FP32 = 1.0f / 256.0f;
+// the split-FP8 z carries the FP8 matrix cores' summation error, ~3e-5 of
+// the sum of |products| per instruction
+constexpr float BPG_TAU_FP8 = 0.5f;
 // Grid of the split-BF16 contraction (see bpg_contract_bf16.h): blocks of
 // 4 waves x R operand subtiles (32 permutation rows each) over chunks of
 // 32-voxel tiles.
@@ -173,6 +180,27 @@ hipError_t launch_contract_f32mx(const ContractArgs& a, const Bf16Launch& L, int
                                  int* occ = nullptr);
 int contract_f32mx_blocks_per_cu(int PP, int KG, bool ftest, bool out);
 hipError_t launch_operand_f32mx(const OperandArgs& a, int KG, hipStream_t s);
+
+// Split-FP8 form (bpg_contract_f8.hip): BPG_F8_TERMS e4m3 terms per operand
+// with an E8M0 scale per 32 subjects, products of terms i + j < T on
+// v_mfma_scale_f32_32x32x64_f8f6f4, one operand term for exact +-1 signs.
+// Up to 256 subjects (4 k-steps of 64) and 16 fit columns.
+#ifndef BPG_F8_TERMS
+#define BPG_F8_TERMS 4
+#endif
+constexpr int f8_rows_per_wave(int KS) { return KS <= 2 ? 2 : 1; }
+constexpr size_t f8_lds_bytes(int KS, bool sign) {
+    return (size_t)4 * f8_rows_per_wave(KS) * KS * ((sign ? 1 : BPG_F8_TERMS) * 128 + 64) * 16;
+}
+hipError_t launch_contract_f8(const ContractArgs& a, const Bf16Launch& L, int PP, int KS, bool sign, hipStream_t s,
+                              int* occ = nullptr);
+int contract_f8_blocks_per_cu(int PP, int KS, bool sign, bool ftest, bool out);
+// data fragments from the FP32 operand rows Y32 [NS][Vpad] (k_transform's
+// FP32 output): Yf8 [Vpad/32][KS][T][64][32 bytes], Ysc [Vpad/32][KS][64]
+hipError_t launch_data_f8(const float* Y32, int NS, int N, int Vpad, int KS, unsigned* Yf8, unsigned* Ysc,
+                          hipStream_t s);
+// operand fragments Gf8 [n_sub][KS][T][64][32 bytes], Gsc [n_sub][KS][64]
+hipError_t launch_operand_f8(const OperandArgs& a, int KS, bool sign, unsigned* Gf8, unsigned* Gsc, hipStream_t s);
 
 // FP64 form (the repeat of permutations with ill-conditioned samples):
 // blocks of f64_perm_tile(pfit) permutations over chunks of 256-voxel

@@ -51,8 +51,11 @@ def _scaled(grid, scale):
     return tuple(max(8, int(round(g * scale))) for g in grid)
 
 
-def study(workload: int, scale: float = 1.0, seed: int | None = None) -> dict:
-    """Synthetic study for workloads 1, 2, 3 and 5 (see the module doc)."""
+def study(workload: int, scale: float = 1.0, seed: int | None = None, contrast: str = "t") -> dict:
+    """Synthetic study for workloads 1, 2, 3 and 5 (see the module doc).
+    Workload 5 has two statistical maps on the same data: contrast "t" (the
+    first covariate) or "F" (the 2-row F contrast over group dummies 1 and
+    2)."""
     seed = 20260000 + workload if seed is None else seed
     ss = np.random.SeedSequence(seed)
     r_data, r_cov, r_perm = (np.random.default_rng(c) for c in ss.spawn(3))
@@ -103,10 +106,15 @@ def study(workload: int, scale: float = 1.0, seed: int | None = None) -> dict:
         dum = np.stack([(grp == j).astype(np.float64) for j in (1, 2, 3)], 1)
         cov = r_cov.standard_normal((N, 4))
         X = np.c_[np.ones(N), dum, cov].astype(np.float32)
-        C = np.zeros((1, 8), np.float32)
-        C[0, 4] = 1
         Y = r_data.standard_normal((N, D, H, W), dtype=np.float32)
-        st.update(N=N, p=8, nc=1, test=capi.TTEST, X=X, C=C, Y=Y, K=10000)
+        if contrast == "F":
+            C = np.zeros((2, 8), np.float32)
+            C[0, 1] = C[1, 2] = 1
+            st.update(N=N, p=8, nc=2, test=capi.FTEST, X=X, C=C, Y=Y, K=10000)
+        else:
+            C = np.zeros((1, 8), np.float32)
+            C[0, 4] = 1
+            st.update(N=N, p=8, nc=1, test=capi.TTEST, X=X, C=C, Y=Y, K=10000)
     else:
         raise ValueError(f"unknown workload {workload}")
     st["V"] = int(mask.sum())

@@ -56,6 +56,9 @@ extern "C" {
 #define BPG_PREC_FP64 2       /* FP64 VALU, residual formed directly (also the
                                  repeat of ill-conditioned permutations)      */
 #define BPG_PREC_SPLIT_F16 3  /* hi/lo FP16 MFMA, first-level plans only       */
+#define BPG_PREC_SPLIT_FP8 5  /* second level: e4m3 terms with an E8M0 scale per
+                                 32 subjects on the block-scaled FP8 matrix
+                                 cores (N <= 256, 16 fit columns)              */
 #define BPG_PREC_FP32_VALU 4  /* FP32 on the vector ALUs (second level; plain
                                  BPG_PREC_FP32 runs on the FP32 matrix cores
                                  up to 16 fit columns, here above)            */

@@ -20,8 +20,9 @@ GOLD = np.load(Path(__file__).resolve().parent / "golden" / "reference_vectors.n
 
 TOL = 1e-4
 
-# (workload, grid scale, permutations): the benchmark shapes on shrunken grids
-CASES = [(1, 0.25, 128), (2, 0.2, 96), (3, 0.2, 64), (5, 0.1, 40)]
+# (workload, grid scale, permutations, contrast): the benchmark shapes on
+# shrunken grids; workload 5 with its t contrast and its 2-row F contrast
+CASES = [(1, 0.25, 128, "t"), (2, 0.2, 96, "t"), (3, 0.2, 64, "t"), (5, 0.1, 40, "t"), (5, 0.1, 40, "F")]
 
 
 def vox(st):
@@ -54,13 +55,13 @@ def check_maxT(md_gpu, ref, stats):
     return int(d.sum())
 
 
-@pytest.mark.parametrize("prec", ["fp32", "split_bf16", "fp64"])
-@pytest.mark.parametrize("wl,scale,K", CASES)
-def test_permuted_statistics_match_fp64(ctx, wl, scale, K, prec):
-    st = synth.study(wl, scale=scale)
+@pytest.mark.parametrize("prec", ["fp32", "split_bf16", "fp64", "fp32_valu"])
+@pytest.mark.parametrize("wl,scale,K,con", CASES)
+def test_permuted_statistics_match_fp64(ctx, wl, scale, K, con, prec):
+    st = synth.study(wl, scale=scale, contrast=con)
     st["inference"] = capi.VOXEL
     perms = synth.permutations(st, K, seed=11)
-    print(f"workload {wl} scale {scale} K {K} {prec} V={st['V']}", flush=True)
+    print(f"workload {wl} ({con}) scale {scale} K {K} {prec} V={st['V']}", flush=True)
     with capi.Plan(ctx, st, prec) as plan:
         md, stats = plan.permutation_phase(0, want_stats=True, **perms)
         tm = plan.timing()
@@ -626,3 +627,26 @@ def test_non_finite_data_are_refused(ctx):
     st["Y"] = Y
     with pytest.raises(capi.BpgError, match="NaN or infinite"):
         capi.Plan(ctx, st, "split_bf16")
+
+
+@pytest.mark.parametrize("wl,scale,K,con", CASES)
+def test_split_fp8_form(ctx, wl, scale, K, con):
+    """The split-FP8 form (4 e4m3 terms per operand, an E8M0 scale per 32
+    subjects, v_mfma_scale_f32_32x32x64_f8f6f4): statistics within 2e-4
+    (1 + |t|) of the FP64 oracle -- not the 1e-4 gate, which it does not
+    meet: the FP8 matrix cores do not sum a 64-term product exactly (up to
+    2^-12.6 of the largest product, probed on the device) -- and a null
+    whose quanta differ from the FP64 one only inside the measured error."""
+    st = synth.study(wl, scale=scale, contrast=con)
+    st["inference"] = capi.VOXEL
+    perms = synth.permutations(st, K, seed=11)
+    print(f"split-FP8 workload {wl} ({con}) V={st['V']} N={st['N']}", flush=True)
+    with capi.Plan(ctx, st, "split_fp8") as plan:
+        md, stats = plan.permutation_phase(0, want_stats=True, **perms)
+        tm = plan.timing()
+    assert tm.precision_used == capi.PREC_SPLIT_FP8
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+    print(f"  max |t-t64|/(1+|t|) = {err.max():.3e}, repeated fp32 {tm.fp32_permutations} fp64 {tm.fp64_permutations}")
+    assert err.max() <= 2e-4
+    check_maxT(md, ref, stats)
