@@ -54,6 +54,10 @@ def run_batch(args):
         Ks.append(K)
         perms.append(synth.permutations(st, K, seed=20260400 + i))
         plans.append(capi.Plan(ctx, st, "split_bf16"))
+    # the studies' shapes from the autotuner (set-up, outside the timed region)
+    t0 = time.perf_counter()
+    tuned = [p.tune(0, K) for p, K in zip(plans, Ks)] if not args.no_tune else []
+    tune_s = time.perf_counter() - t0
     pv = sum(K * st["V"] for K, st in zip(Ks, studies))
     flops = sum(2.0 * st["N"] * st["p"] * K * st["V"] for K, st in zip(Ks, studies))
     print(f"# 64 studies ready: {pv:.3e} perm*voxels per step", file=sys.stderr, flush=True)
@@ -83,6 +87,10 @@ def run_batch(args):
         "config": {"workload": WORKLOADS[4], "studies": len(plans), "perm_voxels": pv,
                    "V_mean": float(np.mean([st["V"] for st in studies])), "precision": "split_bf16"},
         "one_after_another_ms": seq * 1e3,
+        "autotuner": {"buckets": ctx.tune_entries, "seconds": tune_s,
+                      "studies_off_default": int(sum(1 for r in tuned if r.variant or r.chunk_pow)),
+                      "sum_default_ms": float(sum(r.default_ms for r in tuned)),
+                      "sum_tuned_ms": float(sum(r.best_ms for r in tuned))} if tuned else None,
         "studies_rerun_in_fp32": int(n_fp32),
         "permutations_repeated_in_fp32": int(p_fp32), "permutations_total": int(sum(Ks)),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
@@ -248,8 +256,14 @@ def hip_sync():
         pass
 
 
-def run_form(ctx, st, perms, precision, steps, warmup):
+def run_form(ctx, st, perms, precision, steps, warmup, tune=False):
     plan = capi.Plan(ctx, st, precision)
+    if tune:
+        # the autotuner's shape for this bucket (set-up, outside the timing)
+        key = "sign" if "sign" in perms else "perm"
+        r = plan.tune(0, perms[key].shape[0])
+        run_form.tuned = {"rows_per_wave": r.rows_per_wave, "chunk_pow": r.chunk_pow,
+                          "vt_per_chunk": r.vt_per_chunk, "default_ms": r.default_ms, "tuned_ms": r.best_ms}
     for _ in range(warmup):
         plan.permutation_phase(0, **perms)
     hip_sync()
@@ -383,6 +397,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the planner's default shapes")
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5, 6],
                     help="2 (default, the metric's workload); 1: sign-flip group mean, K=1000; "
                          "3: F test with cluster-extent inference; 4: 64 studies in one batched call; "
@@ -418,7 +433,8 @@ def main():
     ctx = capi.Context(0)
     print(f"# study ready: V={V} N={N} p={p} K={K} device={ctx.device_name}", file=sys.stderr, flush=True)
 
-    plan, md, dt, kern_ms, tm = run_form(ctx, st, perms, "split_bf16", args.steps, args.warmup)
+    run_form.tuned = None
+    plan, md, dt, kern_ms, tm = run_form(ctx, st, perms, "split_bf16", args.steps, args.warmup, tune=not args.no_tune)
     # the observed map and its p-values (with ties) against this null
     t0 = time.perf_counter()
     obs = plan.observed_maps()
@@ -463,6 +479,7 @@ def main():
         "permutations_repeated_in_fp32": tm.fp32_permutations,
         "permutations_repeated_in_fp64": tm.fp64_permutations,
         "observed_and_pvalues_ms": pval_ms,
+        "autotuner": run_form.tuned,
     }
     md32 = None
     if not args.no_fp32:

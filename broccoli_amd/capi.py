@@ -41,6 +41,7 @@ EXPORTS = [
     "bpg_gen_perm_first_level", "bpg_fl_plan_create", "bpg_fl_plan_destroy", "bpg_fl_plan_num_voxels",
     "bpg_fl_plan_ar_estimates", "bpg_fl_plan_whitened", "bpg_fl_plan_permutation_phase",
     "bpg_fl_plan_last_timing",
+    "bpg_plan_tune", "bpg_tune_save", "bpg_tune_load", "bpg_tune_entries",
 ]
 
 
@@ -66,6 +67,11 @@ class Timing(ctypes.Structure):
                 ("fit_columns", c_int), ("slots_per_perm", c_int), ("precision_used", c_int),
                 ("unsafe_waves", c_int), ("fp32_permutations", c_long), ("fp64_permutations", c_long),
                 ("cdt_tie_maps", c_long), ("pvalue_ties", c_long)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("variant", c_int), ("rows_per_wave", c_int), ("chunk_pow", c_int), ("vt_per_chunk", c_int),
+                ("default_ms", c_double), ("best_ms", c_double), ("candidates", c_int)]
 
 
 class BpgError(RuntimeError):
@@ -119,6 +125,10 @@ def load() -> ctypes.CDLL:
         "bpg_fl_plan_whitened": (c_int, [vp, vp]),
         "bpg_fl_plan_permutation_phase": (c_int, [vp, c_int, vp, c_long, vp, vp]),
         "bpg_fl_plan_last_timing": (c_int, [vp, POINTER(Timing)]),
+        "bpg_plan_tune": (c_int, [vp, c_int, c_long, POINTER(TuneResult)]),
+        "bpg_tune_save": (c_int, [vp, ctypes.c_char_p]),
+        "bpg_tune_load": (c_int, [vp, ctypes.c_char_p]),
+        "bpg_tune_entries": (c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -205,6 +215,19 @@ class Context:
     @property
     def device_name(self) -> str:
         return self._lib.bpg_device_name(self.handle).decode()
+
+    def tune_save(self, path) -> None:
+        _check(self._lib.bpg_tune_save(self.handle, str(path).encode()), self.handle)
+
+    def tune_load(self, path) -> int:
+        n = self._lib.bpg_tune_load(self.handle, str(path).encode())
+        if n < 0:
+            _check(n, self.handle)
+        return n
+
+    @property
+    def tune_entries(self) -> int:
+        return self._lib.bpg_tune_entries(self.handle)
 
     def close(self):
         if self.handle:
@@ -294,6 +317,13 @@ class Plan:
         t = Timing()
         _check(self._lib.bpg_plan_last_timing(self.handle, ctypes.byref(t)))
         return t
+
+    def tune(self, map_index: int, K: int) -> TuneResult:
+        """bpg_plan_tune: searches the split-BF16 contraction's shape for
+        this map and K on the device and keeps the fastest in the context."""
+        r = TuneResult()
+        _check(self._lib.bpg_plan_tune(self.handle, map_index, K, ctypes.byref(r)), self.ctx.handle)
+        return r
 
     def observed_maps(self) -> np.ndarray:
         out = np.zeros(self.nmaps * self.G, np.float32)

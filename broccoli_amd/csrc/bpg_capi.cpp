@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -39,6 +41,29 @@ int bf16_slots(int pfit) { return (pfit == 3 || pfit == 5) ? pfit : pow2ceil(pfi
 
 // FP32 operand row count for N subjects: multiples of 4 up to 64, then of 16.
 int f32_rows(int N) { return N <= 64 ? (N + 3) / 4 * 4 : (N + 15) / 16 * 16; }
+
+// The autotuner's bucket of a split-BF16 contraction.
+bpg::TuneKey tune_key(int KK, int PP, bool sign, bool ftest, bool out, int Vpad) {
+    int vb = 0;
+    while ((1 << vb) < Vpad) vb++;
+    return bpg::TuneKey(KK, PP, sign ? 1 : 0, ftest ? 1 : 0, out ? 1 : 0, vb);
+}
+
+// The split-BF16 grid: the candidate bpg_plan_tune has in force, else the
+// tuned entry of this bucket, else the planner's default.  The alternative
+// subtile count exists for the max-T-only kernels only.
+bpg::Bf16Launch bf16_launch(const bpg_ctx* ctx, long K, int PP, int KK, int Vpad, bool sign, bool ftest, bool out) {
+    bpg::TuneEntry e;
+    if (ctx->tune_forced) {
+        e = ctx->tune_force;
+    } else {
+        const auto it = ctx->tune.find(tune_key(KK, PP, sign, ftest, out, Vpad));
+        if (it != ctx->tune.end()) e = it->second;
+    }
+    const int variant = out ? 0 : e.variant;
+    const int slots = ctx->cus / 8 * bpg::contract_bf16_blocks_per_cu(KK, PP, sign, ftest, out, variant);
+    return bpg::plan_contract_bf16((int)K, PP, KK, Vpad, sign, slots, variant, e.chunk_pow);
+}
 
 }  // namespace
 
@@ -167,6 +192,8 @@ extern "C" int bpg_create(int device, bpg_ctx** out) {
             c->scratch_err = std::string("device scratch unavailable: ") + hipGetErrorString(e);
         }
     }
+    // tuned shapes cached for this device (bpg_tune_save)
+    if (const char* tc = std::getenv("BPG_TUNE_CACHE")) (void)bpg_tune_load(c, tc);
     *out = c;
     return BPG_OK;
 }
@@ -446,10 +473,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const int PP = bf16_slots(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
     const bpg::Bf16Launch Lb =
-        bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad, m.sign_operand,
-                                     ctx->cus / 8 *
-                                         bpg::contract_bf16_blocks_per_cu(KK, PP, m.sign_operand,
-                                                                          d.test == BPG_FTEST, want_stats || cluster))
+        bf ? bf16_launch(ctx, K, PP, KK, Vpad, m.sign_operand, d.test == BPG_FTEST, want_stats || cluster)
            : bpg::Bf16Launch{};
     // FP32 on the matrix cores up to 16 fit columns, else on the vector ALUs
     const bool mx = form == BPG_PREC_FP32 && pfit <= 16;
@@ -934,11 +958,7 @@ extern "C" int bpg_plan_permutation_phase_noalloc(bpg_plan* pl, int map, const u
     const int npy = bpg::bf16_y_pieces(KK, m.sign_operand);
     const int PP = bf16_slots(pfit);
     const bpg::Bf16Launch Lb =
-        bf ? bpg::plan_contract_bf16((int)K, PP, KK, Vpad, m.sign_operand,
-                                     ctx->cus / 8 *
-                                         bpg::contract_bf16_blocks_per_cu(KK, PP, m.sign_operand,
-                                                                          d.test == BPG_FTEST, false))
-           : bpg::Bf16Launch{};
+        bf ? bf16_launch(ctx, K, PP, KK, Vpad, m.sign_operand, d.test == BPG_FTEST, false) : bpg::Bf16Launch{};
     const bool mx = d.precision == BPG_PREC_FP32 && pfit <= 16;
     const int KG = (N + 7) / 8;
     const bpg::Bf16Launch Lm =
@@ -1268,3 +1288,142 @@ extern "C" int bpg_plan_pvalues_ties(bpg_plan* pl, const long* K, const float* c
     pl->timing.pvalue_ties = total_ties;
     return BPG_OK;
 }
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+extern "C" int bpg_plan_tune(bpg_plan* pl, int map, long K, bpg_tune_result* out) {
+    if (!pl || !out) return BPG_ERR_ARG;
+    bpg_ctx* ctx = pl->ctx;
+    const bpg_study_desc& d = pl->d;
+    *out = bpg_tune_result{};
+    if (map < 0 || map >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
+    if (K <= 0 || K > (1L << 26)) BPG_FAIL(ctx, BPG_ERR_ARG, "number of permutations out of range");
+    if (d.precision != BPG_PREC_SPLIT_BF16)
+        BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED, "the autotuner searches the split-BF16 contraction's shapes");
+    if (pl->pending.active) BPG_FAIL(ctx, BPG_ERR_ARG, "a permutation phase of this plan is in flight");
+    if (pl->Vc == 0) return BPG_OK;
+    const bpg::MapModel& m = pl->models[map];
+    const int N = d.N, KK = (N + 15) / 16, PP = bf16_slots(m.pfit);
+    const bool cluster = d.inference == BPG_CLUSTER_EXTENT || d.inference == BPG_CLUSTER_MASS;
+    const bool out_stats = cluster || d.inference == BPG_CLUSTER_MASS || d.inference == BPG_TFCE;
+    // seeded permutations of the phase's shape (the grid does not depend on
+    // their values)
+    std::vector<unsigned short> perm;
+    std::vector<float> sign;
+    int rc = m.sign_operand ? (sign.resize((size_t)K * N), bpg_gen_sign_matrix(N, K, 0, 20261203, sign.data()))
+                            : (perm.resize((size_t)K * N), bpg_gen_perm_correlation(N, K, 0, 20261203, perm.data()));
+    if (rc != BPG_OK) {
+        // fewer distinct permutations than K (tiny N): repeat the identity
+        rc = BPG_OK;
+        if (m.sign_operand) std::fill(sign.begin(), sign.end(), 1.0f);
+        else
+            for (long k = 0; k < K; k++)
+                for (int s = 0; s < N; s++) perm[(size_t)k * N + s] = (unsigned short)s;
+    }
+    std::vector<float> md(K);
+    std::vector<bpg::TuneEntry> cands;
+    const bool alt = KK <= 8 && !out_stats;
+    for (int v = 0; v <= (alt ? 1 : 0); v++)
+        for (int cp = -1; cp <= 1; cp++) {
+            bpg::TuneEntry e;
+            e.variant = v;
+            e.chunk_pow = cp;
+            cands.push_back(e);
+        }
+    double def_ms = 0.0, best_ms = 1e30;
+    bpg::TuneEntry best;
+    ctx->tune_forced = true;
+    for (auto& e : cands) {
+        ctx->tune_force = e;
+        double ms = 1e30;
+        for (int rep = 0; rep < 3 && rc == BPG_OK; rep++) {
+            rc = bpg_plan_permutation_phase(pl, map, m.sign_operand ? nullptr : perm.data(),
+                                            m.sign_operand ? sign.data() : nullptr, K, md.data(), nullptr);
+            if (rc == BPG_OK && rep > 0) ms = std::min(ms, pl->timing.contract_ms);
+        }
+        if (rc != BPG_OK) break;
+        e.best_ms = ms;
+        if (e.variant == 0 && e.chunk_pow == 0) def_ms = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = e;
+        }
+    }
+    ctx->tune_forced = false;
+    if (rc != BPG_OK) return rc;
+    best.default_ms = def_ms;
+    best.best_ms = best_ms;
+    const int Vpad = pl->Vpad;
+    ctx->tune[tune_key(KK, PP, m.sign_operand, d.test == BPG_FTEST, out_stats, Vpad)] = best;
+    const bpg::Bf16Launch L = bf16_launch(ctx, K, PP, KK, Vpad, m.sign_operand, d.test == BPG_FTEST, out_stats);
+    out->variant = best.variant;
+    out->rows_per_wave = bpg::bf16_rows_per_wave_v(KK, m.sign_operand, best.variant);
+    out->chunk_pow = best.chunk_pow;
+    out->vt_per_chunk = L.vt_per_chunk;
+    out->default_ms = def_ms;
+    out->best_ms = best_ms;
+    out->candidates = (int)cands.size();
+    return BPG_OK;
+}
+
+namespace {
+std::string tune_device_tag(const bpg_ctx* ctx) {
+    std::string t = ctx->name + "/" + ctx->arch;
+    for (char& c : t)
+        if (c == ' ' || c == '\t') c = '_';
+    return t;
+}
+}  // namespace
+
+extern "C" int bpg_tune_save(const bpg_ctx* ctx, const char* path) {
+    if (!ctx || !path) return BPG_ERR_ARG;
+    // keep the entries of other devices already in the file
+    std::vector<std::string> keep;
+    const std::string tag = tune_device_tag(ctx);
+    if (FILE* f = std::fopen(path, "r")) {
+        char line[512];
+        while (std::fgets(line, sizeof line, f))
+            if (line[0] != '#' && std::strncmp(line, tag.c_str(), tag.size()) != 0) keep.push_back(line);
+        std::fclose(f);
+    }
+    FILE* f = std::fopen(path, "w");
+    if (!f) return BPG_ERR_ARG;
+    std::fprintf(f, "# bpg split-BF16 tuning: device KK PP sign ftest out log2(Vpad) variant chunk_pow default_ms best_ms\n");
+    for (const auto& l : keep) std::fputs(l.c_str(), f);
+    for (const auto& [k, e] : ctx->tune)
+        std::fprintf(f, "%s %d %d %d %d %d %d %d %d %.6f %.6f\n", tag.c_str(), std::get<0>(k), std::get<1>(k),
+                     std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), e.variant, e.chunk_pow,
+                     e.default_ms, e.best_ms);
+    std::fclose(f);
+    return BPG_OK;
+}
+
+extern "C" int bpg_tune_load(bpg_ctx* ctx, const char* path) {
+    if (!ctx || !path) return BPG_ERR_ARG;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return 0;
+    const std::string tag = tune_device_tag(ctx);
+    char line[512], dev[256];
+    int n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int kk, pp, sg, ft, ou, vb, va, cp;
+        double dm, bm;
+        if (std::sscanf(line, "%255s %d %d %d %d %d %d %d %d %lf %lf", dev, &kk, &pp, &sg, &ft, &ou, &vb, &va, &cp,
+                        &dm, &bm) != 11)
+            continue;
+        if (tag != dev || va < 0 || va > 1 || cp < -1 || cp > 1) continue;
+        bpg::TuneEntry e;
+        e.variant = va;
+        e.chunk_pow = cp;
+        e.default_ms = dm;
+        e.best_ms = bm;
+        ctx->tune[bpg::TuneKey(kk, pp, sg, ft, ou, vb)] = e;
+        n++;
+    }
+    std::fclose(f);
+    return n;
+}
+
+extern "C" int bpg_tune_entries(const bpg_ctx* ctx) { return ctx ? (int)ctx->tune.size() : BPG_ERR_ARG; }

@@ -357,10 +357,25 @@ hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool s
     constexpr int RS = bf16_rows_per_wave(KK, true);
     constexpr int R = bf16_rows_per_wave(KK, false);
     constexpr bool PIPE = bf16_pipelined(KK, false);
+    // the autotuner's alternative subtile count, compiled for the max-T-only
+    // kernels up to 8 k-steps
+    constexpr bool ALT = KK <= 8;
+    constexpr int RSA = bf16_rows_per_wave_alt(KK, true);
+    constexpr int RA = bf16_rows_per_wave_alt(KK, false);
+    const bool out = occ ? *occ != 0 : (a.stats || a.bits);
+    const bool alt = ALT && L.variant == 1 && !out;
     if (sign) {
         if (a.ftest || PP != 1) return hipErrorInvalidValue;
-        if constexpr (PP == 1) return run_bf16<KK, RS, 1, true, false, bf16_pipelined(KK, true)>(a, L, s, occ);
+        if constexpr (PP == 1) {
+            if constexpr (ALT)
+                if (alt) return run_bf16<KK, RSA, 1, true, false, bf16_pipelined(KK, true)>(a, L, s, occ);
+            return run_bf16<KK, RS, 1, true, false, bf16_pipelined(KK, true)>(a, L, s, occ);
+        }
     }
+    if constexpr (ALT)
+        if (alt)
+            return a.ftest ? run_bf16<KK, RA, PP, false, true, PIPE>(a, L, s, occ)
+                           : run_bf16<KK, RA, PP, false, false, PIPE>(a, L, s, occ);
     return a.ftest ? run_bf16<KK, R, PP, false, true, PIPE>(a, L, s, occ)
                    : run_bf16<KK, R, PP, false, false, PIPE>(a, L, s, occ);
 }

@@ -136,10 +136,20 @@ constexpr float BPG_TAU_FP8 = 0.5f;
 // 32-voxel tiles.
 struct Bf16Launch {
     int n_mgroups, n_vchunks, vt_per_chunk, per_xcd, n_sub, n_vt;
+    int variant;  // 0: the default subtiles per wave; 1: the tuner's alternative
 };
 // operand subtiles per wave (the planner and the kernels share this rule)
 constexpr int bf16_rows_per_wave(int KK, bool sign) {
     return KK <= 4 ? (sign ? BPG_R_SIGN : BPG_R_SMALL) : (KK <= 8 ? BPG_R_MID : (KK <= BPG_R2_KK ? 2 : 1));
+}
+// The alternative the autotuner may choose for the max-T-only kernels up to
+// 8 k-steps (N <= 128): 2 subtiles instead of 4 (signs) or 3 (N <= 64),
+// 1 instead of 2 at 5..8 k-steps; elsewhere the default.
+constexpr int bf16_rows_per_wave_alt(int KK, bool sign) {
+    return KK <= 4 ? 2 : (KK <= 8 ? 1 : bf16_rows_per_wave(KK, sign));
+}
+constexpr int bf16_rows_per_wave_v(int KK, bool sign, int variant) {
+    return variant ? bf16_rows_per_wave_alt(KK, sign) : bf16_rows_per_wave(KK, sign);
 }
 // data fragments per subject: hi + mid + lo for exact +-1 operands at
 // KK <= 4 (every product exact, so the FP32 form's conditioning bound
@@ -148,14 +158,17 @@ constexpr int bf16_y_pieces(int KK, bool sign) { return sign && KK <= 4 ? 3 : 2;
 constexpr bool bf16_pipelined(int KK, bool sign) {
     return !sign && (KK <= 4 ? BPG_PIPE_SMALL : (KK <= 8 ? BPG_PIPE_MID : BPG_PIPE_LARGE)) != 0;
 }
-// slots_per_xcd: blocks of the contraction that one XCD holds at once
-Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd);
+// slots_per_xcd: blocks of the contraction that one XCD holds at once;
+// variant: subtiles per wave (bf16_rows_per_wave_v); chunk_pow: the
+// voxel-chunk length scaled by 2^chunk_pow from the planner's (autotuner)
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd, int variant = 0,
+                              int chunk_pow = 0);
 // with occ set, stores the blocks per CU of the instantiation instead of
 // launching (see run_bf16)
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
                                 bool sign_operand, hipStream_t s, int* occ = nullptr);
 // blocks of the contraction instantiation for these operands per CU
-int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out);
+int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out, int variant = 0);
 
 // Grid of the FP32 contraction: blocks of PTT permutations over chunks of
 // 256-voxel steps.
@@ -175,7 +188,7 @@ hipError_t launch_contract_f32(const ContractArgs& a, const F32Launch& L, hipStr
 // 2 KG KiB of LDS, 64 KiB per block at KG = 8: two blocks per CU), 1 above.
 constexpr int f32mx_rows_per_wave(int KG) { return KG <= 10 ? 2 : 1; }
 constexpr size_t f32mx_lds_bytes(int KG) { return (size_t)4 * f32mx_rows_per_wave(KG) * KG * 64 * 16; }
-Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd);
+Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd, int chunk_pow = 0);
 hipError_t launch_contract_f32mx(const ContractArgs& a, const Bf16Launch& L, int PP, int KG, hipStream_t s,
                                  int* occ = nullptr);
 int contract_f32mx_blocks_per_cu(int PP, int KG, bool ftest, bool out);

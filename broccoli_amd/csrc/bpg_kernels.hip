@@ -298,11 +298,14 @@ template <int KK>
 hipError_t dispatch_bf16_pp(const ContractArgs& a, const Bf16Launch& L, int PP, bool sign, hipStream_t s, int* occ);
 
 
-Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd) {
-    return plan_contract_tiles(K, PP, bf16_rows_per_wave(KK, sign), Vpad, slots_per_xcd);
+Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd, int variant,
+                              int chunk_pow) {
+    Bf16Launch L = plan_contract_tiles(K, PP, bf16_rows_per_wave_v(KK, sign, variant), Vpad, slots_per_xcd, chunk_pow);
+    L.variant = variant;
+    return L;
 }
 
-Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd) {
+Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd, int chunk_pow) {
     Bf16Launch L{};
     L.n_sub = (K + 2 * (16 / PP) - 1) / (2 * (16 / PP));
     L.n_vt = Vpad / 32;
@@ -330,17 +333,22 @@ Bf16Launch plan_contract_tiles(int K, int PP, int R, int Vpad, int slots_per_xcd
             len = s;
         }
     }
+    // the autotuner's scaling of the chunk length, within the chunk limit
+    if (chunk_pow > 0) len = std::min(std::max(1, L.n_vt), len << chunk_pow);
+    if (chunk_pow < 0) len = std::max((L.n_vt + maxc - 1) / maxc, std::max(1, len >> -chunk_pow));
     L.vt_per_chunk = len;
     L.n_vchunks = (L.n_vt + len - 1) / len;
     L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
     return L;
 }
 
-int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out) {
+int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out, int variant) {
     ContractArgs a{};
     a.ftest = ftest;
     int occ = out ? 1 : 0;
-    if (launch_contract_bf16(a, Bf16Launch{}, KK, PP, sign, nullptr, &occ) != hipSuccess || occ < 1) return 1;
+    Bf16Launch L{};
+    L.variant = variant;
+    if (launch_contract_bf16(a, L, KK, PP, sign, nullptr, &occ) != hipSuccess || occ < 1) return 1;
     return occ;
 }
 

@@ -5,14 +5,30 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "bpg.h"
 #include "bpg_kernels.h"
 
+namespace bpg {
+// Autotuner entries: per (k-steps, slots per permutation, sign operand,
+// F test, statistic output, log2 of the padded voxel count) the subtile
+// variant and the chunk-length scaling that ran fastest on this device.
+using TuneKey = std::tuple<int, int, int, int, int, int>;
+struct TuneEntry {
+    int variant = 0, chunk_pow = 0;
+    double default_ms = 0.0, best_ms = 0.0;
+};
+}  // namespace bpg
+
 struct bpg_ctx {
     int device = 0;
+    std::map<bpg::TuneKey, bpg::TuneEntry> tune;
+    bool tune_forced = false;      // bpg_plan_tune's candidate in force
+    bpg::TuneEntry tune_force;
     std::string err;
     std::string name;
     std::string arch;

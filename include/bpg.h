@@ -238,6 +238,34 @@ int bpg_plan_pvalues_ties(bpg_plan* plan, const long* K, const float* const* max
  *      results of the FP64 form).  flags: host, K bytes. */
 int bpg_plan_last_tie_flags(const bpg_plan* plan, unsigned char* flags);
 
+/* ---- autotuner (no counterpart in the reference, whose launch shapes are
+ *      fixed: LIB:2949-2973, LIB:3117-3180).  bpg_plan_tune times the
+ *      split-BF16 contraction of the plan's map `map` with K permutations
+ *      (seeded, of the phase's shape) in each candidate shape -- operand
+ *      subtiles per wave (the default or, for the max-T kernels up to 128
+ *      subjects, the alternative) x voxel-chunk length (the planner's,
+ *      halved, doubled) -- on the device, HIP-event kernel time, best of 2
+ *      after a warm-up, and keeps the fastest for the bucket (k-steps of 16
+ *      subjects, accumulator slots per permutation, sign operand, F test,
+ *      statistic output, log2 of the padded voxel count) in the context:
+ *      every later phase in that bucket uses it.  bpg_tune_save writes the
+ *      context's entries to a text file (entries of other devices already
+ *      there are kept), bpg_tune_load reads this device's (returns their
+ *      number); bpg_create loads the file named by BPG_TUNE_CACHE. */
+typedef struct {
+    int variant;        /* 0: the default subtiles per wave, 1: the alternative */
+    int rows_per_wave;  /* subtiles per wave of the choice                      */
+    int chunk_pow;      /* voxel-chunk length x 2^chunk_pow (-1, 0, 1)          */
+    int vt_per_chunk;   /* 32-voxel tiles per chunk of the choice               */
+    double default_ms;  /* contraction time of the planner's default shape      */
+    double best_ms;     /* of the choice                                        */
+    int candidates;
+} bpg_tune_result;
+int bpg_plan_tune(bpg_plan* plan, int map, long K, bpg_tune_result* out);
+int bpg_tune_save(const bpg_ctx* ctx, const char* path);
+int bpg_tune_load(bpg_ctx* ctx, const char* path);
+int bpg_tune_entries(const bpg_ctx* ctx);
+
 /* ---- host-side pieces of the path -------------------------------------- */
 
 /* Generators (LIB:17069-17122, LIB:16925-17011, LIB:17014-17066).
