@@ -42,6 +42,7 @@ EXPORTS = [
     "bpg_fl_plan_ar_estimates", "bpg_fl_plan_whitened", "bpg_fl_plan_permutation_phase",
     "bpg_fl_plan_last_timing",
     "bpg_plan_tune", "bpg_tune_save", "bpg_tune_load", "bpg_tune_entries",
+    "bpg_batch_create", "bpg_batch_run", "bpg_batch_last_timing", "bpg_batch_destroy",
 ]
 
 
@@ -129,6 +130,10 @@ def load() -> ctypes.CDLL:
         "bpg_tune_save": (c_int, [vp, ctypes.c_char_p]),
         "bpg_tune_load": (c_int, [vp, ctypes.c_char_p]),
         "bpg_tune_entries": (c_int, [vp]),
+        "bpg_batch_create": (c_int, [vp, c_int, vp, vp, POINTER(c_void_p)]),
+        "bpg_batch_run": (c_int, [vp, vp, vp, vp]),
+        "bpg_batch_last_timing": (c_int, [vp, POINTER(c_double), POINTER(c_double), POINTER(c_int), POINTER(c_int)]),
+        "bpg_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -409,6 +414,63 @@ def permutation_phases(plans, maps, perms):
     O = (c_void_p * n)(*[o.ctypes.data for o in outs])
     _check(lib.bpg_plans_permutation_phase(P, n, M, pp, ss, Ks, O), plans[0].ctx.handle)
     return outs
+
+
+class Batch:
+    """bpg_batch: the permutation phases of several studies captured once as
+    a hipGraph (persistent contraction per class); run() replays it."""
+
+    def __init__(self, plans, maps, Ks):
+        self.plans = list(plans)
+        self._lib = self.plans[0]._lib
+        n = len(self.plans)
+        self.Ks = [int(k) for k in Ks]
+        P = (c_void_p * n)(*[p.handle for p in self.plans])
+        M = (c_int * n)(*maps)
+        Kc = (c_long * n)(*self.Ks)
+        h = c_void_p()
+        _check(self._lib.bpg_batch_create(P, n, M, Kc, ctypes.byref(h)), self.plans[0].ctx.handle)
+        self.handle = h
+
+    def run(self, perms):
+        """perms[i]: dict with "perm" or "sign" of K[i] rows; returns the
+        list of maxdist arrays."""
+        n = len(self.plans)
+        pp = (c_void_p * n)()
+        ss = (c_void_p * n)()
+        keep, outs = [], []
+        for i, d in enumerate(perms):
+            if "sign" in d:
+                a = np.ascontiguousarray(d["sign"], np.float32)
+                ss[i] = a.ctypes.data
+            else:
+                a = np.ascontiguousarray(d["perm"], np.uint16)
+                pp[i] = a.ctypes.data
+            assert a.shape[0] == self.Ks[i]
+            keep.append(a)
+            outs.append(np.zeros(self.Ks[i], np.float32))
+        O = (c_void_p * n)(*[o.ctypes.data for o in outs])
+        _check(self._lib.bpg_batch_run(self.handle, pp, ss, O), self.plans[0].ctx.handle)
+        return outs
+
+    def last_timing(self):
+        """(run ms, graph ms, studies with repeated permutations, persistent
+        launches) of the last run."""
+        ms, gms, fb, nc = c_double(), c_double(), c_int(), c_int()
+        _check(self._lib.bpg_batch_last_timing(self.handle, ctypes.byref(ms), ctypes.byref(gms), ctypes.byref(fb),
+                                               ctypes.byref(nc)))
+        return ms.value, gms.value, fb.value, nc.value
+
+    def close(self):
+        if self.handle:
+            self._lib.bpg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # ------------------------------------------------------------- first level

@@ -1427,3 +1427,504 @@ extern "C" int bpg_tune_load(bpg_ctx* ctx, const char* path) {
 }
 
 extern "C" int bpg_tune_entries(const bpg_ctx* ctx) { return ctx ? (int)ctx->tune.size() : BPG_ERR_ARG; }
+
+// ---------------------------------------------------------------------------
+// batches of studies: one persistent contraction per class, two hipGraphs
+// ---------------------------------------------------------------------------
+namespace {
+
+// The regularised incomplete beta function I_x(a, b) from its continued
+// fraction (Abramowitz & Stegun 26.5.8), evaluated by the modified Lentz
+// method; on the side x > (a+1)/(a+b+2) through I_x(a,b) = 1 - I_{1-x}(b,a).
+double inc_beta(double x, double a, double b) {
+    if (!(x > 0.0)) return 0.0;
+    if (!(x < 1.0)) return 1.0;
+    const bool flip = x > (a + 1.0) / (a + b + 2.0);
+    if (flip) {
+        std::swap(a, b);
+        x = 1.0 - x;
+    }
+    const double front =
+        std::exp(std::lgamma(a + b) - std::lgamma(a) - std::lgamma(b) + a * std::log(x) + b * std::log1p(-x)) / a;
+    const double tiny = 1e-300;
+    auto clamp = [&](double v) { return std::fabs(v) < tiny ? tiny : v; };
+    double c = 1.0, d = 1.0 / clamp(1.0 - (a + b) * x / (a + 1.0)), h = d;
+    for (int m = 1; m <= 400; m++) {
+        const double m2 = 2.0 * m;
+        double t = m * (b - m) * x / ((a + m2 - 1.0) * (a + m2));
+        d = 1.0 / clamp(1.0 + t * d);
+        c = clamp(1.0 + t / c);
+        h *= d * c;
+        t = -(a + m) * (a + b + m) * x / ((a + m2) * (a + m2 + 1.0));
+        d = 1.0 / clamp(1.0 + t * d);
+        c = clamp(1.0 + t / c);
+        const double step = d * c;
+        h *= step;
+        if (std::fabs(step - 1.0) < 1e-13) break;
+    }
+    const double r = front * h;
+    return flip ? 1.0 - r : r;
+}
+
+// The chance that a permutation of a study meets an ill-conditioned sample
+// (SSE < tau |y*|^2) at one of its V voxels, were the data Gaussian noise:
+// SSE / |y*|^2 is then Beta(dof / 2, pfit / 2) at every voxel.  Orders the
+// batch's work only; results never depend on it.
+double ill_conditioned_chance(double dof, int pfit, long V, double tau) {
+    if (dof <= 0.0 || pfit <= 0 || V <= 0) return 0.0;
+    const double pv = inc_beta(tau, 0.5 * dof, 0.5 * pfit);
+    if (pv >= 1.0) return 1.0;
+    return -std::expm1((double)V * std::log1p(-pv));
+}
+
+}  // namespace
+
+struct bpg_batch {
+    bpg_ctx* ctx = nullptr;
+    struct Study {
+        bpg_plan* plan = nullptr;
+        int map = 0;
+        long K = 0;
+        int N = 0, KK = 0, PP = 0, npy = 2;
+        int stage = 1;        // 0: likely to meet ill-conditioned samples (runs first)
+        bool sign = false;
+        bpg::Bf16Launch L{};
+        DBuf<double> yy;
+        DBuf<unsigned> Yb, Gb;
+        DBuf<float> partial;
+        size_t in_off = 0;    // its permutations / signs in the staging and device copies
+        size_t max_off = 0;   // its maxima in max_all / h_max
+        size_t flag_off = 0;  // its flags (bytes, a multiple of 4) in the word buffer / h_flags
+        int slot = 0;         // its ill-conditioned count in the word buffer / h_unsafe
+        // repeats of its flagged permutations (host copies that live until
+        // the repeat has finished)
+        std::vector<long> rep;
+        std::vector<unsigned short> rperm;
+        std::vector<float> rsign, rmax;
+    };
+    std::vector<Study> st;
+    struct Class {
+        int KK, PP, sign, variant, stage;
+        std::vector<int> members;
+        int n_items[bpg::BPG_XCDS] = {};  // per XCD queue
+        int grid = 0;
+        size_t first = 0;  // index of its first study in the device table
+    };
+    std::vector<Class> cls;
+    DBuf<bpg::BatchStudy> table;
+    DBuf<bpg::ReduceJob> jobs;
+    // one word buffer cleared by one fill: queue heads (8 per class), the
+    // studies' ill-conditioned counts, then their flag bytes
+    DBuf<unsigned> words;
+    size_t n_heads = 0, flag_bytes = 0;
+    DBuf<unsigned short> perm_all;
+    DBuf<float> sign_all, max_all;
+    size_t n_perm = 0, n_sign = 0, n_max = 0;
+    int max_K = 0;
+    // per stage: its studies' ranges of maxima, flag bytes and counts (the
+    // offsets are assigned stage by stage), its reduction jobs
+    size_t max_end[2] = {}, flag_end[2] = {};
+    int slot_end[2] = {}, job_end[2] = {};
+    HBuf<unsigned short> h_perm;
+    HBuf<float> h_sign, h_max;
+    HBuf<unsigned> h_unsafe;
+    HBuf<unsigned char> h_flags;
+    hipStream_t stream = nullptr;
+    hipEvent_t first_done = nullptr;
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    double last_ms = 0.0, last_graph_ms = 0.0;
+    int last_fallback = 0;
+
+    unsigned* heads() const { return words.p; }
+    unsigned* unsafe() const { return words.p + n_heads; }
+    unsigned char* flags() const { return reinterpret_cast<unsigned char*>(words.p + n_heads + st.size()); }
+
+    ~bpg_batch() {
+        for (int g = 0; g < 2; g++) {
+            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+            if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        }
+        if (first_done) (void)hipEventDestroy(first_done);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (auto& s : st) {
+            s.yy.release(); s.Yb.release(); s.Gb.release(); s.partial.release();
+        }
+        table.release(); jobs.release(); words.release(); perm_all.release(); sign_all.release(); max_all.release();
+        h_perm.release(); h_sign.release(); h_max.release(); h_unsafe.release(); h_flags.release();
+    }
+};
+
+extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, const long* K, bpg_batch** out) {
+    if (!plans || n <= 0 || !maps || !K || !out) return BPG_ERR_ARG;
+    *out = nullptr;
+    bpg_ctx* ctx = plans[0] ? plans[0]->ctx : nullptr;
+    if (!ctx) return BPG_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        bpg_plan* pl = plans[i];
+        if (!pl || pl->ctx != ctx) BPG_FAIL(ctx, BPG_ERR_ARG, "the plans of a batch must share one context");
+        for (int j = 0; j < i; j++)
+            if (plans[j] == pl) BPG_FAIL(ctx, BPG_ERR_ARG, "a plan appears twice in one batch");
+        const bpg_study_desc& d = pl->d;
+        if (d.precision != BPG_PREC_SPLIT_BF16 || d.inference != BPG_VOXEL || d.test == BPG_FTEST || d.N > 128)
+            BPG_FAIL(ctx, BPG_ERR_UNSUPPORTED,
+                     "a batch runs split-BF16 t tests and group means with voxel inference, up to 128 subjects");
+        if (maps[i] < 0 || maps[i] >= pl->nmaps) BPG_FAIL(ctx, BPG_ERR_ARG, "statistical map index out of range");
+        if (K[i] <= 0 || K[i] > (1L << 26)) BPG_FAIL(ctx, BPG_ERR_ARG, "number of permutations out of range");
+    }
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    bpg_batch* b = new (std::nothrow) bpg_batch();
+    if (!b) return BPG_ERR_NOMEM;
+    b->ctx = ctx;
+    auto fail = [&](int rc) { delete b; return rc; };
+#define BPG_BHIP(call)                                                              \
+    do {                                                                            \
+        const hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                                     \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+            return fail(e_ == hipErrorOutOfMemory ? BPG_ERR_NOMEM : BPG_ERR_HIP);   \
+        }                                                                           \
+    } while (0)
+    b->st.resize(n);
+    for (int i = 0; i < n; i++) {
+        auto& s = b->st[i];
+        bpg_plan* pl = plans[i];
+        const bpg::MapModel& m = pl->models[maps[i]];
+        s.plan = pl;
+        s.map = maps[i];
+        s.K = K[i];
+        s.N = pl->d.N;
+        s.KK = (s.N + 15) / 16;
+        s.PP = bf16_slots(m.pfit);
+        s.sign = m.sign_operand;
+        s.npy = bpg::bf16_y_pieces(s.KK, s.sign);
+        const double tau = s.npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+        s.stage = ill_conditioned_chance((double)m.dof, m.pfit, pl->Vc, tau) >= 1e-3 ? 0 : 1;
+        s.in_off = s.sign ? b->n_sign : b->n_perm;
+        (s.sign ? b->n_sign : b->n_perm) += (size_t)s.K * s.N;
+        b->max_K = std::max<int>(b->max_K, (int)s.K);
+        if (pl->Vc == 0) continue;
+        s.L = bf16_launch(ctx, s.K, s.PP, s.KK, pl->Vpad, s.sign, false, false);
+        BPG_BHIP(s.yy.ensure(pl->Vpad));
+        BPG_BHIP(s.Yb.ensure((size_t)(pl->Vpad / 32) * s.KK * s.npy * 64 * 4));
+        BPG_BHIP(s.Gb.ensure((size_t)s.L.n_sub * s.KK * 2 * 64 * 4));
+        BPG_BHIP(s.partial.ensure((size_t)s.L.n_vchunks * s.K));
+    }
+    // result offsets stage by stage, so each stage's results are one range
+    for (int g = 0; g < 2; g++) {
+        for (auto& s : b->st) {
+            if (s.stage != g) continue;
+            s.max_off = b->n_max;
+            b->n_max += (size_t)s.K;
+            s.flag_off = b->flag_bytes;
+            b->flag_bytes += ((size_t)s.K + 3) / 4 * 4;
+            s.slot = b->slot_end[1]++;
+        }
+        b->max_end[g] = b->n_max;
+        b->flag_end[g] = b->flag_bytes;
+        if (g == 0) b->slot_end[0] = b->slot_end[1];
+    }
+    // classes: same instantiation (k-steps, slots, operand kind, subtile
+    // variant) and stage
+    for (int i = 0; i < n; i++) {
+        const auto& s = b->st[i];
+        if (s.plan->Vc == 0) continue;
+        bpg_batch::Class* c = nullptr;
+        for (auto& x : b->cls)
+            if (x.KK == s.KK && x.PP == s.PP && x.sign == (int)s.sign && x.variant == s.L.variant &&
+                x.stage == s.stage)
+                c = &x;
+        if (!c) {
+            b->cls.push_back(bpg_batch::Class{s.KK, s.PP, (int)s.sign, s.L.variant, s.stage, {}, {}, 0, 0});
+            c = &b->cls.back();
+        }
+        c->members.push_back(i);
+    }
+    b->n_heads = b->cls.size() * bpg::BPG_XCDS;
+    const size_t n_words = b->n_heads + (size_t)n + b->flag_bytes / 4;
+    BPG_BHIP(b->words.ensure(n_words));
+    BPG_BHIP(b->perm_all.ensure(std::max<size_t>(b->n_perm, 1)));
+    BPG_BHIP(b->sign_all.ensure(std::max<size_t>(b->n_sign, 1)));
+    BPG_BHIP(b->max_all.ensure(std::max<size_t>(b->n_max, 1)));
+    BPG_BHIP(b->h_perm.ensure(std::max<size_t>(b->n_perm, 1)));
+    BPG_BHIP(b->h_sign.ensure(std::max<size_t>(b->n_sign, 1)));
+    BPG_BHIP(b->h_max.ensure(std::max<size_t>(b->n_max, 1)));
+    BPG_BHIP(b->h_unsafe.ensure(n));
+    BPG_BHIP(b->h_flags.ensure(std::max<size_t>(b->flag_bytes, 1)));
+    std::fill(b->h_unsafe.p, b->h_unsafe.p + n, 0u);
+    std::vector<bpg::BatchStudy> host;
+    for (size_t ci = 0; ci < b->cls.size(); ci++) {
+        auto& c = b->cls[ci];
+        c.first = host.size();
+        for (int i : c.members) {
+            auto& s = b->st[i];
+            bpg_plan* pl = s.plan;
+            const bpg::MapModel& m = pl->models[s.map];
+            bpg::BatchStudy bs{};
+            bpg::ContractArgs& a = bs.a;
+            a.Gb = s.Gb.p; a.Yb = s.Yb.p; a.yy = s.yy.p; a.partial = s.partial.p;
+            a.K = (int)s.K; a.Vc = pl->Vc; a.Vpad = pl->Vpad; a.N = s.N; a.pfit = m.pfit; a.r = m.r;
+            a.nwords = pl->nwords;
+            a.sqrt_dof = (float)std::sqrt((double)m.dof);
+            a.f_scale = (float)((double)m.dof / (double)m.r);
+            a.cdt = pl->d.cdt;
+            a.tau = s.npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
+            a.pinf = INFINITY;
+            a.unsafe = b->unsafe() + s.slot;
+            a.flags = b->flags() + s.flag_off;
+            a.cdt_lo = a.cdt_hi = pl->d.cdt;
+            bs.L = s.L;
+            for (int x = 0; x < bpg::BPG_XCDS; x++) {
+                // voxel chunks x, x + 8, ... of this study go to queue x
+                const int nvc = s.L.n_vchunks > x ? (s.L.n_vchunks - 1 - x) / bpg::BPG_XCDS + 1 : 0;
+                bs.item0[x] = c.n_items[x];
+                c.n_items[x] += nvc * s.L.n_mgroups;
+            }
+            host.push_back(bs);
+        }
+        int occ = 0, total = 0;
+        for (int x = 0; x < bpg::BPG_XCDS; x++) total += c.n_items[x];
+        BPG_BHIP(bpg::launch_contract_bf16_persist(bpg::BatchClass{}, c.KK, c.PP, c.sign != 0, c.variant, 0, nullptr,
+                                                   &occ));
+        c.grid = std::max(1, std::min(total, ctx->cus * std::max(occ, 1)));
+    }
+    // the reduction jobs stage by stage
+    std::vector<bpg::ReduceJob> rjobs;
+    for (int g = 0; g < 2; g++) {
+        for (const auto& s : b->st)
+            if (s.stage == g && s.plan->Vc > 0)
+                rjobs.push_back(bpg::ReduceJob{s.partial.p, b->max_all.p + s.max_off, s.L.n_vchunks, (int)s.K});
+        b->job_end[g] = (int)rjobs.size();
+    }
+    BPG_BHIP(b->table.ensure(std::max<size_t>(host.size(), 1)));
+    if (!host.empty())
+        BPG_BHIP(hipMemcpy(b->table.p, host.data(), host.size() * sizeof(bpg::BatchStudy), hipMemcpyHostToDevice));
+    BPG_BHIP(b->jobs.ensure(std::max<size_t>(rjobs.size(), 1)));
+    if (!rjobs.empty())
+        BPG_BHIP(hipMemcpy(b->jobs.p, rjobs.data(), rjobs.size() * sizeof(bpg::ReduceJob), hipMemcpyHostToDevice));
+    BPG_BHIP(hipStreamCreate(&b->stream));
+    BPG_BHIP(hipEventCreateWithFlags(&b->first_done, hipEventDisableTiming));
+    // Two graphs, each one chain on one stream.  The first: the upload of
+    // all permutations / signs and one fill clearing the queue heads,
+    // ill-conditioned counts and flags (a kernel, not a memset node); then,
+    // as the second, the stage's classes, the largest work first, each its
+    // members' nuisance transforms and operand builds and then its
+    // persistent contraction; one launch of the stage's max reductions; the
+    // copies of its maxima, counts and flags to the host.  Stage 0 holds
+    // the studies likely to meet ill-conditioned samples, so that their
+    // repeats run beside the second graph.
+    std::vector<double> work(b->cls.size(), 0.0);
+    for (size_t ci = 0; ci < b->cls.size(); ci++)
+        for (int i : b->cls[ci].members) {
+            const auto& s = b->st[i];
+            work[ci] += (double)s.K * s.plan->Vpad * s.KK * s.PP;
+        }
+    std::vector<int> order(b->cls.size());
+    for (size_t ci = 0; ci < order.size(); ci++) order[ci] = (int)ci;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    hipStream_t cs = b->stream;
+    for (int g = 0; g < 2; g++) {
+        const size_t m0 = g ? b->max_end[0] : 0, f0 = g ? b->flag_end[0] : 0;
+        const int s0 = g ? b->slot_end[0] : 0, j0 = g ? b->job_end[0] : 0;
+        const bool empty = b->job_end[g] == j0;
+        if (g == 1 && empty) break;
+        BPG_BHIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        int rc = BPG_OK;
+        auto cap = [&](hipError_t e) {
+            if (e != hipSuccess && rc == BPG_OK) {
+                ctx->err = std::string("graph capture: ") + hipGetErrorString(e);
+                rc = BPG_ERR_HIP;
+            }
+        };
+        if (g == 0) {
+            if (b->n_perm)
+                cap(hipMemcpyAsync(b->perm_all.p, b->h_perm.p, b->n_perm * sizeof(unsigned short),
+                                   hipMemcpyHostToDevice, cs));
+            if (b->n_sign)
+                cap(hipMemcpyAsync(b->sign_all.p, b->h_sign.p, b->n_sign * sizeof(float), hipMemcpyHostToDevice, cs));
+            cap(bpg::launch_fill_u32(b->words.p, n_words, 0u, cs));
+        }
+        for (int ci : order) {
+            if (rc != BPG_OK) break;
+            const auto& c = b->cls[ci];
+            if (c.stage != g) continue;
+            // the members' data and operand fragments shortly before their
+            // contraction, while they are still in the last-level cache
+            for (int i : c.members) {
+                auto& s = b->st[i];
+                bpg_plan* pl = s.plan;
+                const bpg::MapModel& m = pl->models[s.map];
+                const double* mq = pl->mdev.p + pl->mdev_off[s.map];
+                const int QB = bpg::transform_width(m.q), NR = (s.N + 15) / 16 * 16;
+                bpg::TransformArgs ta{};
+                ta.Yc = pl->Yc.p;
+                ta.Qn = mq + m.Qt.size();
+                ta.dscale = mq + m.Qt.size() + (size_t)NR * std::max(QB, 1);
+                ta.N = s.N; ta.q = m.q; ta.Vc = pl->Vc; ta.Vpad = pl->Vpad;
+                ta.KK = s.KK; ta.NS = 0; ta.npy = s.npy;
+                ta.yy = s.yy.p; ta.Yb = s.Yb.p;
+                cap(bpg::launch_transform(ta, cs));
+                bpg::OperandArgs oa{};
+                oa.perm = s.sign ? nullptr : b->perm_all.p + s.in_off;
+                oa.sign = s.sign ? b->sign_all.p + s.in_off : nullptr;
+                oa.Qt = mq;
+                oa.N = s.N; oa.pfit = m.pfit; oa.K = (int)s.K; oa.PP = s.PP; oa.KK = s.KK;
+                oa.n_sub = s.L.n_sub;
+                oa.Gb = s.Gb.p;
+                cap(bpg::launch_operand_bf16(oa, cs));
+            }
+            bpg::BatchClass bc{};
+            bc.studies = b->table.p + c.first;
+            bc.n_studies = (int)c.members.size();
+            std::copy(c.n_items, c.n_items + bpg::BPG_XCDS, bc.n_items);
+            bc.counter = b->heads() + (size_t)ci * bpg::BPG_XCDS;
+            cap(bpg::launch_contract_bf16_persist(bc, c.KK, c.PP, c.sign != 0, c.variant, c.grid, cs));
+        }
+        if (!empty) {
+            cap(bpg::launch_reduce_max_jobs(b->jobs.p + j0, b->job_end[g] - j0, b->max_K, cs));
+            cap(hipMemcpyAsync(b->h_max.p + m0, b->max_all.p + m0, (b->max_end[g] - m0) * sizeof(float),
+                               hipMemcpyDeviceToHost, cs));
+            cap(hipMemcpyAsync(b->h_unsafe.p + s0, b->unsafe() + s0, (size_t)(b->slot_end[g] - s0) * sizeof(unsigned),
+                               hipMemcpyDeviceToHost, cs));
+            cap(hipMemcpyAsync(b->h_flags.p + f0, b->flags() + f0, b->flag_end[g] - f0, hipMemcpyDeviceToHost, cs));
+        }
+        const hipError_t ec = hipStreamEndCapture(cs, &b->graph[g]);
+        if (rc != BPG_OK) return fail(rc);
+        BPG_BHIP(ec);
+        BPG_BHIP(hipGraphInstantiate(&b->exec[g], b->graph[g], nullptr, nullptr, 0));
+    }
+#undef BPG_BHIP
+    *out = b;
+    return BPG_OK;
+}
+
+namespace {
+
+// The flagged permutations of study i of a finished stage, enqueued for a
+// repeat through its plan in the next form (split-BF16 -> FP32, an exact
+// split -> FP64); the plan cascades further when the repeat is collected.
+int batch_enqueue_repeat(bpg_batch* b, int i, const unsigned short* const* perms, const float* const* signs) {
+    auto& s = b->st[i];
+    s.rep.clear();
+    if (s.plan->Vc == 0 || b->h_unsafe.p[s.slot] == 0) return BPG_OK;
+    const unsigned char* fl = b->h_flags.p + s.flag_off;
+    for (long k = 0; k < s.K; k++)
+        if (fl[k]) s.rep.push_back(k);
+    if (s.rep.empty()) return BPG_OK;
+    const long Ks = (long)s.rep.size();
+    const unsigned short* rp = nullptr;
+    const float* rs = nullptr;
+    if (Ks == s.K) {
+        rp = s.sign ? nullptr : perms[i];
+        rs = s.sign ? signs[i] : nullptr;
+    } else if (s.sign) {
+        s.rsign.resize((size_t)Ks * s.N);
+        for (long j = 0; j < Ks; j++)
+            std::copy(signs[i] + s.rep[j] * s.N, signs[i] + (s.rep[j] + 1) * s.N, s.rsign.begin() + j * s.N);
+        rs = s.rsign.data();
+    } else {
+        s.rperm.resize((size_t)Ks * s.N);
+        for (long j = 0; j < Ks; j++)
+            std::copy(perms[i] + s.rep[j] * s.N, perms[i] + (s.rep[j] + 1) * s.N, s.rperm.begin() + j * s.N);
+        rp = s.rperm.data();
+    }
+    if (s.plan->pending.active) {
+        s.rep.clear();
+        BPG_FAIL(b->ctx, BPG_ERR_ARG, "a permutation phase of a batched plan is in flight");
+    }
+    const int rc = phase_enqueue(s.plan, s.map, rp, rs, Ks, nullptr, s.npy == 3 ? BPG_PREC_FP64 : BPG_PREC_FP32);
+    if (rc != BPG_OK) s.rep.clear();
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int bpg_batch_run(bpg_batch* b, const unsigned short* const* perms, const float* const* signs,
+                             float* const* maxdist) {
+    if (!b || !maxdist) return BPG_ERR_ARG;
+    bpg_ctx* ctx = b->ctx;
+    const int n = (int)b->st.size();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; i++) {
+        const auto& s = b->st[i];
+        if (!maxdist[i]) BPG_FAIL(ctx, BPG_ERR_ARG, "null maxdist");
+        const void* src = s.sign ? (const void*)(signs ? signs[i] : nullptr) : (const void*)(perms ? perms[i] : nullptr);
+        if (!src) BPG_FAIL(ctx, BPG_ERR_ARG, "missing permutation / sign matrix");
+        if (s.plan->pending.active) BPG_FAIL(ctx, BPG_ERR_ARG, "a permutation phase of a batched plan is in flight");
+        if (s.sign) {
+            std::memcpy(b->h_sign.p + s.in_off, signs[i], (size_t)s.K * s.N * sizeof(float));
+        } else {
+            unsigned short worst = 0;
+            for (size_t q = 0; q < (size_t)s.K * s.N; q++) {
+                b->h_perm.p[s.in_off + q] = perms[i][q];
+                worst = std::max(worst, perms[i][q]);
+            }
+            if (worst >= s.N) BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
+        }
+    }
+    BPG_HIP(ctx, hipSetDevice(ctx->device));
+    BPG_HIP(ctx, hipGraphLaunch(b->exec[0], b->stream));
+    BPG_HIP(ctx, hipEventRecord(b->first_done, b->stream));
+    if (b->exec[1]) BPG_HIP(ctx, hipGraphLaunch(b->exec[1], b->stream));
+    // the first stage's repeats are enqueued (on the plans' streams) while
+    // the second graph runs; then the second stage's
+    int rc = BPG_OK;
+    std::vector<int> pending;
+    for (int g = 0; g < 2 && rc == BPG_OK; g++) {
+        const hipError_t e = g == 0 ? hipEventSynchronize(b->first_done) : hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("batch graph: ") + hipGetErrorString(e);
+            rc = BPG_ERR_HIP;
+            break;
+        }
+        if (g == 1)
+            b->last_graph_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int i = 0; i < n && rc == BPG_OK; i++) {
+            if (b->st[i].stage != g) continue;
+            rc = batch_enqueue_repeat(b, i, perms, signs);
+            if (rc == BPG_OK && !b->st[i].rep.empty()) pending.push_back(i);
+        }
+    }
+    if (rc != BPG_OK) (void)hipStreamSynchronize(b->stream);
+    int fallback = 0;
+    for (int i : pending) {
+        auto& s = b->st[i];
+        s.rmax.resize(s.rep.size());
+        const int r = phase_finish(s.plan, s.rmax.data());
+        if (rc == BPG_OK) rc = r;
+        fallback++;
+    }
+    if (rc != BPG_OK) return rc;
+    for (int i = 0; i < n; i++) {
+        const auto& s = b->st[i];
+        float* md = maxdist[i];
+        if (s.plan->Vc == 0) {
+            for (long k = 0; k < s.K; k++) md[k] = empty_max();
+            continue;
+        }
+        const float* hm = b->h_max.p + s.max_off;
+        for (long k = 0; k < s.K; k++)
+            md[k] = hm[k] > -INFINITY ? (float)((float)quantize(hm[k]) / 10000.0f) : empty_max();
+        for (size_t j = 0; j < s.rep.size(); j++) md[s.rep[j]] = s.rmax[j];
+    }
+    b->last_fallback = fallback;
+    b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return BPG_OK;
+}
+
+extern "C" int bpg_batch_last_timing(const bpg_batch* b, double* run_ms, double* graph_ms, int* repeated_studies,
+                                     int* classes) {
+    if (!b) return BPG_ERR_ARG;
+    if (run_ms) *run_ms = b->last_ms;
+    if (graph_ms) *graph_ms = b->last_graph_ms;
+    if (repeated_studies) *repeated_studies = b->last_fallback;
+    if (classes) *classes = (int)b->cls.size();
+    return BPG_OK;
+}
+
+extern "C" void bpg_batch_destroy(bpg_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b;
+}

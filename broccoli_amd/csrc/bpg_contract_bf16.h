@@ -30,10 +30,19 @@
 
 namespace bpg {
 
+// Views of the fragment and norm arrays in the global address space: the
+// kernels read them through pointers that may come from memory (the
+// persistent form's study table), which the compiler would otherwise treat
+// as generic and access with flat instructions.
+typedef const bf16x8 __attribute__((address_space(1))) gbf16x8;
+typedef const double __attribute__((address_space(1))) gdouble;
+typedef float __attribute__((address_space(1))) gfloat;
+typedef unsigned char __attribute__((address_space(1))) gbyte;
+
 // The data tile of voxel tile vt: NPY fragments per k-step (hi, lo; or hi,
 // mid, lo for exact +-1 operands at small N), bm unused with NPY = 2.
 template <int KK, int NPY>
-__device__ __forceinline__ void load_tile(const bf16x8* Y, const double* yy, int vt, int lane,
+__device__ __forceinline__ void load_tile(const gbf16x8* Y, const gdouble* yy, int vt, int lane,
                                           bf16x8 (&bh)[KK], bf16x8 (&bm)[NPY == 3 ? KK : 1], bf16x8 (&bl)[KK],
                                           double& yv) {
 #pragma unroll
@@ -181,19 +190,18 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 // L2; the 4 waves share the lines through the CU's L1) while tile t is
 // multiplied.  At the end, a 32-lane butterfly per permutation and one
 // partial maximum per (voxel chunk, permutation).
+// One work item: permutation group mg (4 waves x R subtiles) over voxel
+// chunk vc.
 template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
-__device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16Launch& L) {
+__device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16Launch& L, int mg, int vc) {
     constexpr int PER = 16 / PP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int lb = xcd_logical_block(L.per_xcd);
-    if (lb >= L.n_mgroups * L.n_vchunks) return;
-    const int mg = lb % L.n_mgroups;
-    const int vc = lb / L.n_mgroups;
     const int sub0 = (mg * 4 + wave) * R;
 
-    const bf16x8* G = reinterpret_cast<const bf16x8*>(a.Gb);
-    const bf16x8* Y = reinterpret_cast<const bf16x8*>(a.Yb);
+    const gbf16x8* G = (const gbf16x8*)a.Gb;
+    const gbf16x8* Y = (const gbf16x8*)a.Yb;
+    const gdouble* YY = (const gdouble*)a.yy;
 
     bf16x8 ahi[R][KK], alo[R][KK];
 #pragma unroll
@@ -244,7 +252,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
     // registers); the R accumulation chains interleave.
     constexpr bool ROLL = !PIPE && NPY == 2 && KK >= BPG_ROLL_MIN_KK;
     if constexpr (ROLL) {
-        if (vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
+        if (vt0 < vt1) load_tile<KK, NPY>(Y, YY, vt0, lane, b0h, b0m, b0l, y0);
         for (int vt = vt0; vt < vt1; vt++) {
             const int vn = min(vt + 1, vt1 - 1);  // the last tile reloads itself
             f32x16 acc[R];
@@ -264,7 +272,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
                 b0h[kk] = Y[base];
                 b0l[kk] = Y[base + 64];
             }
-            const double yn = a.yy[vn * 32 + (lane & 31)];
+            const double yn = YY[vn * 32 + (lane & 31)];
             const float yhi = (float)y0;  // NaN for padding voxels
             const float ylo = (float)(y0 - (double)yhi);
 #pragma unroll
@@ -272,13 +280,13 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
             y0 = yn;
         }
     }
-    if (!ROLL && vt0 < vt1) load_tile<KK, NPY>(Y, a.yy, vt0, lane, b0h, b0m, b0l, y0);
+    if (!ROLL && vt0 < vt1) load_tile<KK, NPY>(Y, YY, vt0, lane, b0h, b0m, b0l, y0);
     for (int vt = vt0; vt < vt1 && !ROLL; vt += 2) {
-        if (vt + 1 < vt1) load_tile<KK, NPY>(Y, a.yy, vt + 1, lane, b1h, b1m, b1l, y1);
+        if (vt + 1 < vt1) load_tile<KK, NPY>(Y, YY, vt + 1, lane, b1h, b1m, b1l, y1);
         contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b0h, b0m, b0l, y0, vt, lane, sub0, runmax,
                                                            pd, bad);
         if (vt + 1 >= vt1) break;
-        if (vt + 2 < vt1) load_tile<KK, NPY>(Y, a.yy, vt + 2, lane, b0h, b0m, b0l, y0);
+        if (vt + 2 < vt1) load_tile<KK, NPY>(Y, YY, vt + 2, lane, b0h, b0m, b0l, y0);
         contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b1h, b1m, b1l, y1, vt + 1, lane, sub0,
                                                            runmax, pd, bad);
     }
@@ -299,7 +307,7 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
                 for (int r = (PIPE ? 0 : b); r < (PIPE ? R : b + 1); r++)
                     for (int P = 0; P < PER; P++) {
                         const int k = (sub0 + r) * (2 * PER) + hl * PER + P;
-                        if (k < a.K) a.flags[k] = 1;
+                        if (k < a.K) ((gbyte*)a.flags)[k] = 1;
                     }
         }
     }
@@ -319,13 +327,61 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
             m = fmaxf(m, __shfl_xor(m, 1, 64));
             const int k = (sub0 + r) * (2 * PER) + hl * PER + P;
             // one writer per (voxel chunk, permutation): no atomics
-            if ((lane & 31) == 0 && k < a.K) a.partial[(size_t)vc * a.K + k] = m * scale;
+            if ((lane & 31) == 0 && k < a.K) ((gfloat*)a.partial)[(size_t)vc * a.K + k] = m * scale;
         }
+}
+
+template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+__device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16Launch& L) {
+    const int lb = xcd_logical_block(L.per_xcd);
+    if (lb >= L.n_mgroups * L.n_vchunks) return;
+    contract_item<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L, lb % L.n_mgroups, lb / L.n_mgroups);
 }
 
 template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
 __global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
+}
+
+// Persistent form over the studies of one batch class (same KK, PP, operand
+// kind and subtile count).  A block starts on the queue of the XCD it runs
+// on (its XCC_ID hardware register): item i of study j in queue x is
+// (mg, vc) = (i % n_mgroups, x + 8 * (i / n_mgroups)) after the items of
+// studies 0..j-1.  A block whose queue is empty takes items from the other
+// XCDs' queues in turn, and leaves when all eight are empty (every block
+// reaches that exit).
+// (the rolling schedule held to 256 registers as in k_contract_bf16_roll:
+// the accumulators then stay in VGPRs, and two waves share each SIMD)
+template <int KK, int R, bool SIGN, bool PIPE>
+constexpr int persist_min_blocks() {
+    return (!PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK) ? (R >= 3 ? 1 : 2) : 1;
+}
+
+template <int KK, int R, int PP, bool SIGN, bool PIPE>
+__global__ __launch_bounds__(256, (persist_min_blocks<KK, R, SIGN, PIPE>())) void k_contract_bf16_persist(BatchClass c) {
+    __shared__ int s_item;
+    // s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4): the XCD of this block
+    const int home = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (BPG_XCDS - 1);
+    for (int q = 0; q < BPG_XCDS; q++) {
+        const int x = (home + q) & (BPG_XCDS - 1);
+        for (;;) {
+            if (threadIdx.x == 0) s_item = (int)atomicAdd(c.counter + x, 1u);
+            __syncthreads();
+            // one value for the whole block: scalar registers from here on
+            const int item = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+            if (item >= c.n_items[x]) break;
+            int j = 0;
+            while (j + 1 < c.n_studies && c.studies[j + 1].item0[x] <= item) j++;
+            // the study's arguments as values (uniform: scalar registers),
+            // not references into memory the item's stores might alias
+            const ContractArgs a = c.studies[j].a;
+            const Bf16Launch L = c.studies[j].L;
+            const int local = item - c.studies[j].item0[x];
+            contract_item<KK, R, PP, SIGN, false, false, PIPE>(a, L, local % L.n_mgroups,
+                                                               x + BPG_XCDS * (local / L.n_mgroups));
+        }
+    }
 }
 
 // The rolling schedule (two data fragments, KK 5..8), held to 256 registers
@@ -378,6 +434,50 @@ hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool s
                            : run_bf16<KK, RA, PP, false, false, PIPE>(a, L, s, occ);
     return a.ftest ? run_bf16<KK, R, PP, false, true, PIPE>(a, L, s, occ)
                    : run_bf16<KK, R, PP, false, false, PIPE>(a, L, s, occ);
+}
+
+template <int KK, int R, int PP, bool SIGN>
+hipError_t run_bf16_persist(const BatchClass& c, int grid, hipStream_t s, int* occ) {
+    constexpr bool PIPE = bf16_pipelined(KK, SIGN);
+    void (*kern)(BatchClass) = k_contract_bf16_persist<KK, R, PP, SIGN, PIPE>;
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, reinterpret_cast<const void*>(kern), 256, 0);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
+
+template <int KK, int PP>
+hipError_t dispatch_persist_kind(const BatchClass& c, bool sign, int variant, int grid, hipStream_t s, int* occ) {
+    constexpr int RS = bf16_rows_per_wave(KK, true), RSA = bf16_rows_per_wave_alt(KK, true);
+    constexpr int R = bf16_rows_per_wave(KK, false), RA = bf16_rows_per_wave_alt(KK, false);
+    if (sign) {
+        if constexpr (PP == 1)
+            return variant ? run_bf16_persist<KK, RSA, 1, true>(c, grid, s, occ)
+                           : run_bf16_persist<KK, RS, 1, true>(c, grid, s, occ);
+        return hipErrorInvalidValue;
+    }
+    return variant ? run_bf16_persist<KK, RA, PP, false>(c, grid, s, occ)
+                   : run_bf16_persist<KK, R, PP, false>(c, grid, s, occ);
+}
+
+// The persistent contraction of a batch class (t statistics, max-T only),
+// compiled up to 8 k-steps (N <= 128).
+template <int KK>
+hipError_t dispatch_persist_pp(const BatchClass& c, int PP, bool sign, int variant, int grid, hipStream_t s,
+                               int* occ) {
+    if constexpr (KK > 8) {
+        return hipErrorInvalidValue;
+    } else {
+        switch (PP) {
+            case 1: return dispatch_persist_kind<KK, 1>(c, sign, variant, grid, s, occ);
+            case 2: return dispatch_persist_kind<KK, 2>(c, sign, variant, grid, s, occ);
+            case 3: return dispatch_persist_kind<KK, 3>(c, sign, variant, grid, s, occ);
+            case 4: return dispatch_persist_kind<KK, 4>(c, sign, variant, grid, s, occ);
+            case 5: return dispatch_persist_kind<KK, 5>(c, sign, variant, grid, s, occ);
+            case 8: return dispatch_persist_kind<KK, 8>(c, sign, variant, grid, s, occ);
+            case 16: return dispatch_persist_kind<KK, 16>(c, sign, variant, grid, s, occ);
+        }
+        return hipErrorInvalidValue;
+    }
 }
 
 template <int KK>

@@ -163,6 +163,30 @@ constexpr bool bf16_pipelined(int KK, bool sign) {
 // voxel-chunk length scaled by 2^chunk_pow from the planner's (autotuner)
 Bf16Launch plan_contract_bf16(int K, int PP, int KK, int Vpad, bool sign, int slots_per_xcd, int variant = 0,
                               int chunk_pow = 0);
+// A batch of studies in one persistent contraction per class: the studies'
+// launch arguments and grids in device memory, item0 the first work item of
+// each study (prefix sums of n_mgroups x n_vchunks), the queue counter
+// cleared before each launch.
+// The work items of a batch class sit in one queue per XCD: queue x holds,
+// study after study, the (permutation group, voxel chunk) items of the voxel
+// chunks vc = x, x + 8, ..., so the blocks of one XCD share their chunks' data
+// in that XCD's L2.
+constexpr int BPG_XCDS = 8;
+struct BatchStudy {
+    ContractArgs a;
+    Bf16Launch L;
+    int item0[BPG_XCDS];  // its first item in each queue
+};
+struct BatchClass {
+    const BatchStudy* studies;
+    int n_studies;
+    int n_items[BPG_XCDS];
+    unsigned* counter;  // BPG_XCDS queue heads
+};
+// persistent launch of a class (t statistics, max-T only, KK <= 8); with
+// occ set, stores the blocks per CU of the instantiation instead
+hipError_t launch_contract_bf16_persist(const BatchClass& c, int KK, int PP, bool sign, int variant, int grid,
+                                        hipStream_t s, int* occ = nullptr);
 // with occ set, stores the blocks per CU of the instantiation instead of
 // launching (see run_bf16)
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
@@ -227,6 +251,13 @@ hipError_t launch_contract_f64(const ContractArgs& a, const F32Launch& L, hipStr
 // out[k] = max over the n_part chunk maxima partial[c*K + k] (NaN-free:
 // chunks without in-mask voxels hold -inf).
 hipError_t launch_reduce_max(const float* partial, int n_part, int K, float* out, hipStream_t s);
+// several per-permutation max reductions in one launch (K <= max_K each)
+struct ReduceJob {
+    const float* partial;
+    float* out;
+    int n_part, K;
+};
+hipError_t launch_reduce_max_jobs(const ReduceJob* jobs, int n_jobs, int max_K, hipStream_t s);
 
 struct ObsArgs {
     const float* Yc;       // [N][Vc]

@@ -352,6 +352,24 @@ int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out,
     return occ;
 }
 
+template <int KK>
+hipError_t dispatch_persist_pp(const BatchClass& c, int PP, bool sign, int variant, int grid, hipStream_t s, int* occ);
+
+hipError_t launch_contract_bf16_persist(const BatchClass& c, int KK, int PP, bool sign, int variant, int grid,
+                                        hipStream_t s, int* occ) {
+    switch (KK) {
+        case 1: return dispatch_persist_pp<1>(c, PP, sign, variant, grid, s, occ);
+        case 2: return dispatch_persist_pp<2>(c, PP, sign, variant, grid, s, occ);
+        case 3: return dispatch_persist_pp<3>(c, PP, sign, variant, grid, s, occ);
+        case 4: return dispatch_persist_pp<4>(c, PP, sign, variant, grid, s, occ);
+        case 5: return dispatch_persist_pp<5>(c, PP, sign, variant, grid, s, occ);
+        case 6: return dispatch_persist_pp<6>(c, PP, sign, variant, grid, s, occ);
+        case 7: return dispatch_persist_pp<7>(c, PP, sign, variant, grid, s, occ);
+        case 8: return dispatch_persist_pp<8>(c, PP, sign, variant, grid, s, occ);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
                                 bool sign_operand, hipStream_t s, int* occ) {
     switch (KK) {
@@ -380,11 +398,13 @@ hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int 
 // chunk, 8 loads in flight); the slices meet in LDS.  With few permutations
 // there are up to 1024 chunks, which one thread per permutation would walk
 // one dependent load at a time.
-__global__ __launch_bounds__(1024) void k_reduce_max(const float* __restrict__ partial, int n_part, int K,
-                                                     float* __restrict__ out) {
+// The per-permutation max over voxel chunks: block bx covers permutations
+// 32 bx .. 32 bx + 31, 32 threads per chunk stride.
+__device__ __forceinline__ void reduce_max_block(const float* __restrict__ partial, int n_part, int K,
+                                                 float* __restrict__ out, int bx) {
     __shared__ float red[32][33];
     const int kl = threadIdx.x & 31, sl = threadIdx.x >> 5;
-    const int k = blockIdx.x * 32 + kl;
+    const int k = bx * 32 + kl;
     float m = -INFINITY;
     if (k < K) {
         int c = sl;
@@ -406,9 +426,28 @@ __global__ __launch_bounds__(1024) void k_reduce_max(const float* __restrict__ p
     }
 }
 
+__global__ __launch_bounds__(1024) void k_reduce_max(const float* __restrict__ partial, int n_part, int K,
+                                                     float* __restrict__ out) {
+    reduce_max_block(partial, n_part, K, out, blockIdx.x);
+}
+
 hipError_t launch_reduce_max(const float* partial, int n_part, int K, float* out, hipStream_t s) {
     if (K <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_max, dim3((K + 31) / 32), dim3(1024), 0, s, partial, n_part, K, out);
+    return hipGetLastError();
+}
+
+// One launch for several reductions: job blockIdx.y, its permutations
+// 32 blockIdx.x .. (blocks past its K return at once).
+__global__ __launch_bounds__(1024) void k_reduce_max_jobs(const ReduceJob* __restrict__ jobs) {
+    const ReduceJob j = jobs[blockIdx.y];
+    if ((int)blockIdx.x * 32 >= j.K) return;
+    reduce_max_block(j.partial, j.n_part, j.K, j.out, blockIdx.x);
+}
+
+hipError_t launch_reduce_max_jobs(const ReduceJob* jobs, int n_jobs, int max_K, hipStream_t s) {
+    if (n_jobs <= 0 || max_K <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_max_jobs, dim3((max_K + 31) / 32, n_jobs), dim3(1024), 0, s, jobs);
     return hipGetLastError();
 }
 

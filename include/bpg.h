@@ -173,6 +173,44 @@ int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const int* maps,
                                 const unsigned short* const* perms, const float* const* signs,
                                 const long* K, float* const* maxdist);
 int bpg_plan_last_timing(const bpg_plan* plan, bpg_timing* out);
+/* ---- a batch of studies as hipGraphs (no counterpart in the reference,
+ *      which analyses one study per RandomiseGroupLevel run, RGL:44-1715;
+ *      the per-study work it replaces is LIB:16589-16790).
+ *      bpg_batch_create takes n distinct plans of one context (split-BF16 t
+ *      tests or group means, voxel inference, up to 128 subjects), their
+ *      maps and permutation counts, and captures two graphs.  The studies
+ *      are split in two stages: stage 0 holds those likely to meet
+ *      ill-conditioned samples (a Gaussian-noise estimate from their
+ *      degrees of freedom, fit columns and voxel count, which orders the
+ *      work and never changes a result), stage 1 the rest.  The first graph
+ *      uploads all permutations / signs and clears the work queues,
+ *      ill-conditioned counts and flags with one fill kernel (not a
+ *      memset); each graph then runs its stage: per class of studies that
+ *      share a contraction instantiation (k-steps, accumulator slots,
+ *      operand kind and the autotuner's subtile count), largest work first,
+ *      its members' nuisance transforms and operand builds, then one
+ *      persistent contraction whose blocks take (study, permutation group,
+ *      voxel chunk) work items from per-XCD device queues until all are
+ *      empty; one launch of the stage's max reductions; the copies of its
+ *      maxima, counts and flags to the host.  The studies' device data and
+ *      models are the plans' (uploaded at their creation); every per-run
+ *      buffer is the batch's own.  bpg_batch_run replays both graphs for new
+ *      permutations (perms[i] or signs[i], [s + k*N], K[i] rows as at
+ *      creation) and returns each study's null in maxdist[i] exactly as
+ *      bpg_plan_permutation_phase does: the flagged (ill-conditioned)
+ *      permutations of each study are repeated through its plan in FP32 /
+ *      FP64, stage 0's while the second graph runs.  A plan may be used on
+ *      its own between runs. */
+typedef struct bpg_batch bpg_batch;
+int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, const long* K, bpg_batch** out);
+int bpg_batch_run(bpg_batch* batch, const unsigned short* const* perms, const float* const* signs,
+                  float* const* maxdist);
+/* host wall time of the last run (graph launch to results) and of its graph
+ * launches (launch to their completion), the studies whose flagged permutations were
+ * repeated, the number of persistent launches */
+int bpg_batch_last_timing(const bpg_batch* batch, double* run_ms, double* graph_ms, int* repeated_studies,
+                          int* classes);
+void bpg_batch_destroy(bpg_batch* batch);
 /* ---- the permutation phase without any device allocation (no counterpart
  *      in the reference): for callers that cannot allocate device memory
  *      while they run (tight serving loops, code that must not call

@@ -650,3 +650,48 @@ def test_split_fp8_form(ctx, wl, scale, K, con):
     print(f"  max |t-t64|/(1+|t|) = {err.max():.3e}, repeated fp32 {tm.fp32_permutations} fp64 {tm.fp64_permutations}")
     assert err.max() <= 2e-4
     check_maxT(md, ref, stats)
+
+
+def test_graph_batch_of_64_studies(ctx):
+    """Workload 4 at reduced scale: the 64 heterogeneous studies (own masks,
+    N 12..120, p 1..6, sign flips for p = 1) in one hipGraph with a
+    persistent contraction per class.  Each study's null equals the
+    single-study call's (bit-exact: the same arithmetic, another schedule),
+    whose statistics meet the FP64 bound; a second replay with other
+    permutations does too."""
+    studies = synth.studies_batch(64, scale=0.25, K=64)
+    plans = [capi.Plan(ctx, st, "split_bf16") for st in studies]
+    try:
+        Ks, perms = [], []
+        for i, st in enumerate(studies):
+            K = int(min(64, capi.max_permutations(2 if st["test"] == capi.GROUP_MEAN else 0, capi.CORRELATION,
+                                                  st["N"], 0)))
+            Ks.append(K)
+            perms.append(synth.permutations(st, K, seed=300 + i))
+        print(f"graph batch: 64 studies, N {min(s['N'] for s in studies)}..{max(s['N'] for s in studies)}",
+              flush=True)
+        with capi.Batch(plans, [0] * 64, Ks) as batch:
+            for rep in range(2):
+                if rep == 1:
+                    perms = [synth.permutations(st, K, seed=900 + i) for i, (st, K) in enumerate(zip(studies, Ks))]
+                got = batch.run(perms)
+                ms, gms, repeated, classes = batch.last_timing()
+                print(f"  replay {rep}: {ms:.2f} ms (graph {gms:.2f}), {classes} classes, {repeated} studies with "
+                      f"repeated permutations", flush=True)
+                for i, (pl, d) in enumerate(zip(plans, perms)):
+                    single, stats = pl.permutation_phase(0, want_stats=True, **d)
+                    assert np.array_equal(got[i], single), i
+                    if rep == 0 and i % 8 == 0:
+                        ref = oracle.perm_stats(studies[i], d, Ks[i])[:, vox(studies[i])]
+                        assert (np.abs(stats - ref) / (1.0 + np.abs(ref))).max() <= TOL
+                        check_maxT(single, ref, stats)
+    finally:
+        for pl in plans:
+            pl.close()
+
+
+def test_graph_batch_refuses_what_it_does_not_run(ctx):
+    st = synth.study(3, scale=0.1)  # F test, cluster inference
+    with capi.Plan(ctx, st, "split_bf16") as pl:
+        with pytest.raises(capi.BpgError, match="batch runs"):
+            capi.Batch([pl], [0], [16])

@@ -29,6 +29,18 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\bT {name}\b", out), name
 
 
+def test_binding_argument_counts_match_declarations():
+    """Each ctypes signature has as many arguments as the C declaration."""
+    lib = capi.load()
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "bpg.h").read_text(), flags=re.S)
+    decls = re.findall(r"\b(bpg_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text)
+    assert len(decls) >= len(capi.EXPORTS)
+    for name, params in decls:
+        params = params.strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        assert len(getattr(lib, name).argtypes) == n, name
+
+
 def test_dropin_class_symbols_exported():
     out = subprocess.run(["nm", "-DC", "--defined-only", str(capi.LIB_PATH)], capture_output=True,
                          text=True).stdout
