@@ -62,7 +62,15 @@ bpg::Bf16Launch bf16_launch(const bpg_ctx* ctx, long K, int PP, int KK, int Vpad
     }
     const int variant = out ? 0 : e.variant;
     const int slots = ctx->cus / 8 * bpg::contract_bf16_blocks_per_cu(KK, PP, sign, ftest, out, variant);
-    return bpg::plan_contract_bf16((int)K, PP, KK, Vpad, sign, slots, variant, e.chunk_pow);
+    bpg::Bf16Launch L = bpg::plan_contract_bf16((int)K, PP, KK, Vpad, sign, slots, variant, e.chunk_pow);
+    if (out && L.n_vt > 32) {
+        // chunks of whole 32-tile groups: the bitmask runs a block stores
+        // then never share a 128-byte run with another block's
+        L.vt_per_chunk = std::min(L.n_vt, (L.vt_per_chunk + 31) / 32 * 32);
+        L.n_vchunks = (L.n_vt + L.vt_per_chunk - 1) / L.vt_per_chunk;
+        L.per_xcd = (L.n_mgroups * L.n_vchunks + 7) / 8;
+    }
+    return L;
 }
 
 }  // namespace

@@ -39,6 +39,10 @@ typedef const double __attribute__((address_space(1))) gdouble;
 typedef float __attribute__((address_space(1))) gfloat;
 typedef unsigned char __attribute__((address_space(1))) gbyte;
 
+// v_writelane_b32: old with lane `lane` replaced by the uniform `value` (the
+// LLVM intrinsic; this compiler exposes no builtin for it)
+__device__ int lane_write(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
 // The data tile of voxel tile vt: NPY fragments per k-step (hi, lo; or hi,
 // mid, lo for exact +-1 operands at small N), bm unused with NPY = 2.
 template <int KK, int NPY>
@@ -74,14 +78,24 @@ __device__ __forceinline__ f32x16 chain(const bf16x8 (&ah)[KK], const bf16x8 (&a
     return acc;
 }
 
+// The voxel tiles of one work item, first..last.
+struct TileSpan {
+    int first, last;
+};
+
 // Statistic and running max of global subtile st over voxel tile vt; vt < 0
 // marks the empty item that opens the pipelined schedule (its NaN norm is
-// dropped by fmaxf and it writes nothing).  OUT: also write the statistic
-// of every voxel (stats) and/or the supra-threshold bitmask (bits); the
-// max-T-only form carries no branches.
-template <int PP, bool FT, bool OUT>
+// dropped by fmaxf and it writes nothing).  OUT = 0: the max only, no
+// branches.  OUT = 1: also the statistic of every voxel (stats) and/or the
+// supra-threshold bitmask (bits), one word per (permutation, tile).
+// OUT = 2 (cluster extent, bits only, no max): lane (vt mod 32) of each half
+// keeps the bitmask word of its permutation in wb, and every 32 tiles (and
+// at the item's last tile) the half-wave stores its 32 words as one 128-byte
+// run of the permutation's row.
+template <int PP, bool FT, int OUT>
 __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, float yhi, float ylo, int vt,
-                                    int lane, int st, float (&rm)[16 / PP], bool& bad) {
+                                    int lane, int st, float (&rm)[16 / PP], bool& bad, unsigned (&wb)[16 / PP],
+                                    const TileSpan& sp) {
     constexpr int PER = 16 / PP;
     const float tyy = a.tau * yhi;  // NaN on padding: never flagged
     float u[PER];
@@ -116,9 +130,9 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
         // max(u, rm) as med3(u, rm, +inf): one v_med3_f32, where fmaxf costs
         // a second instruction to quiet the loop-carried rm; a NaN u (padding
         // voxel) yields rm, as fmaxf does (checked on the device)
-        rm[P] = __builtin_amdgcn_fmed3f(u[P], rm[P], a.pinf);
+        if constexpr (OUT != 2) rm[P] = __builtin_amdgcn_fmed3f(u[P], rm[P], a.pinf);
     }
-    if (OUT) {
+    if constexpr (OUT == 1) {
         const int hl = lane >> 5;
         const int v = vt * 32 + (lane & 31);
         const bool valid = vt >= 0 && v < a.Vc;
@@ -139,6 +153,46 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
             }
         }
     }
+    if constexpr (OUT == 2) {
+        const int v = vt * 32 + (lane & 31);
+        const unsigned long long vmask = __ballot(vt >= 0 && v < a.Vc);
+        const float scale = FT ? a.f_scale : a.sqrt_dof;
+        const int slot = vt & 31;
+        unsigned long long tmask = 0ull;
+        float stv[PER];
+#pragma unroll
+        for (int P = 0; P < PER; P++) {
+            stv[P] = u[P] * scale;
+            // the two words of the ballot into lanes slot and 32 + slot
+            const unsigned long long bal = __ballot(stv[P] > a.cdt) & vmask;
+            wb[P] = (unsigned)lane_write((int)(unsigned)bal, slot, (int)wb[P]);
+            wb[P] = (unsigned)lane_write((int)(unsigned)(bal >> 32), 32 + slot, (int)wb[P]);
+            tmask |= __ballot(stv[P] > a.cdt_lo && stv[P] <= a.cdt_hi);
+        }
+        if (tmask & vmask) {
+            // a statistic inside the error band of the threshold (rare):
+            // the map's largest cluster may differ from an exact one's
+            const int hl = lane >> 5;
+            const int kbase = st * (2 * PER) + hl * PER;
+#pragma unroll
+            for (int P = 0; P < PER; P++) {
+                const unsigned long long tb = __ballot(stv[P] > a.cdt_lo && stv[P] <= a.cdt_hi) & vmask;
+                const int k = kbase + P;
+                if ((lane & 31) == 0 && k < a.K && (hl ? (tb >> 32) : (tb & 0xFFFFFFFFull)) != 0ull) a.ties[k] = 1;
+            }
+        }
+        if (vt >= 0 && ((vt & 31) == 31 || vt == sp.last)) {
+            const int t = (vt & ~31) + (lane & 31);
+            const int kbase = st * (2 * PER) + (lane >> 5) * PER;
+            if (t >= sp.first && t <= vt) {
+#pragma unroll
+                for (int P = 0; P < PER; P++) {
+                    const int k = kbase + P;
+                    if (k < a.K) a.bits[(size_t)k * a.nwords + t] = wb[P];
+                }
+            }
+        }
+    }
 }
 
 // pipeline state: the accumulator awaiting its epilogue and its tile's norms
@@ -148,20 +202,21 @@ struct Pending {
     int vt;
 };
 
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE, int NPY>
+template <int KK, int R, int PP, bool SIGN, bool FT, int OUT, bool PIPE, int NPY>
 __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x8 (&ahi)[R][KK],
                                               const bf16x8 (&alo)[R][KK], const bf16x8 (&bh)[KK],
                                               const bf16x8 (&bm)[NPY == 3 ? KK : 1], const bf16x8 (&bl)[KK],
                                               double yv, int vt, int lane, int sub0,
                                               float (&runmax)[R][16 / PP], Pending& pd,
-                                              bool (&bad)[PIPE ? 1 : R]) {
+                                              bool (&bad)[PIPE ? 1 : R], unsigned (&wbuf)[R][16 / PP],
+                                              const TileSpan& sp) {
     const float yhi = (float)yv;  // NaN for padding voxels
     const float ylo = (float)(yv - (double)yhi);
     if constexpr (!PIPE) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const f32x16 acc = chain<KK, SIGN, NPY>(ahi[r], alo[r], bh, bm, bl);
-            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[PIPE ? 0 : r]);
+            epi<PP, FT, OUT>(a, acc, yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[PIPE ? 0 : r], wbuf[r], sp);
         }
     } else {
         constexpr int NM = KK * (SIGN ? NPY : 3);
@@ -170,8 +225,11 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const f32x16 acc = chain<KK, SIGN, NPY>(ahi[r], alo[r], bh, bm, bl);
-            if (r == 0) epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0]);
-            else epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad[0]);
+            if (r == 0)
+                epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0],
+                                 wbuf[R - 1], sp);
+            else
+                epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad[0], wbuf[r - 1], sp);
 #pragma unroll
             for (int i = 0; i < NM; i++) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA
@@ -192,7 +250,7 @@ __device__ __forceinline__ void contract_tile(const ContractArgs& a, const bf16x
 // partial maximum per (voxel chunk, permutation).
 // One work item: permutation group mg (4 waves x R subtiles) over voxel
 // chunk vc.
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+template <int KK, int R, int PP, bool SIGN, bool FT, int OUT, bool PIPE>
 __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16Launch& L, int mg, int vc) {
     constexpr int PER = 16 / PP;
     const int lane = threadIdx.x & 63;
@@ -234,6 +292,12 @@ __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16L
 
     const int vt0 = vc * L.vt_per_chunk;
     const int vt1 = min(vt0 + L.vt_per_chunk, L.n_vt);
+    const TileSpan sp{vt0, vt1 - 1};
+    unsigned wbuf[R][PER];  // OUT = 2: the bitmask words of the lane's tile slot
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int P = 0; P < PER; P++) wbuf[r][P] = 0u;
     constexpr int NPY = bf16_y_pieces(KK, SIGN);
     constexpr int NBM = NPY == 3 ? KK : 1;
     bf16x8 b0h[KK], b0l[KK], b1h[KK], b1l[KK], b0m[NBM], b1m[NBM];
@@ -276,22 +340,74 @@ __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16L
             const float yhi = (float)y0;  // NaN for padding voxels
             const float ylo = (float)(y0 - (double)yhi);
 #pragma unroll
-            for (int r = 0; r < R; r++) epi<PP, FT, OUT>(a, acc[r], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r]);
+            for (int r = 0; r < R; r++)
+                epi<PP, FT, OUT>(a, acc[r], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r], wbuf[r], sp);
             y0 = yn;
         }
     }
-    if (!ROLL && vt0 < vt1) load_tile<KK, NPY>(Y, YY, vt0, lane, b0h, b0m, b0l, y0);
-    for (int vt = vt0; vt < vt1 && !ROLL; vt += 2) {
+    // Rolling pipelined schedule (two data fragments, KK >= 5, pipelined):
+    // one data buffer; the subtiles' chains run one after another, each
+    // interleaved with the epilogue of the accumulator before it (the last
+    // subtile of the previous tile for the first), and the last chain hands
+    // each k-step's registers to the next tile's data once it has used them
+    // (every load then has KK - 1 k-steps of MFMAs to arrive).
+    constexpr bool RPIPE = PIPE && NPY == 2 && KK >= BPG_ROLL_MIN_KK;
+    if constexpr (RPIPE) {
+        constexpr int NM = KK * (SIGN ? 2 : 3);
+        constexpr int NV = (PER * (PP + 5) + NM - 1) / NM;
+        if (vt0 < vt1) load_tile<KK, NPY>(Y, YY, vt0, lane, b0h, b0m, b0l, y0);
+        for (int vt = vt0; vt < vt1; vt++) {
+            const int vn = min(vt + 1, vt1 - 1);  // the last tile reloads itself
+            const double yn = YY[vn * 32 + (lane & 31)];
+            const float yhi = (float)y0;  // NaN for padding voxels
+            const float ylo = (float)(y0 - (double)yhi);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                f32x16 acc;
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+#pragma unroll
+                for (int kk = 0; kk < KK; kk++) {
+                    if (!SIGN) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r][kk], b0h[kk], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], b0l[kk], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r][kk], b0h[kk], acc, 0, 0, 0);
+                    if (r == R - 1) {
+                        const size_t base = (((size_t)vn * KK + kk) * 2) * 64 + lane;
+                        b0h[kk] = Y[base];
+                        b0l[kk] = Y[base + 64];
+                    }
+                }
+                if (r == 0)
+                    epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0],
+                                     wbuf[R - 1], sp);
+                else
+                    epi<PP, FT, OUT>(a, pd.acc, yhi, ylo, vt, lane, sub0 + r - 1, runmax[r - 1], bad[0],
+                                     wbuf[r - 1], sp);
+#pragma unroll
+                for (int i = 0; i < NM; i++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);  // NV VALU
+                }
+                pd.acc = acc;
+            }
+            pd.yhi = yhi;
+            pd.ylo = ylo;
+            pd.vt = vt;
+            y0 = yn;
+        }
+    }
+    if (!ROLL && !RPIPE && vt0 < vt1) load_tile<KK, NPY>(Y, YY, vt0, lane, b0h, b0m, b0l, y0);
+    for (int vt = vt0; vt < vt1 && !ROLL && !RPIPE; vt += 2) {
         if (vt + 1 < vt1) load_tile<KK, NPY>(Y, YY, vt + 1, lane, b1h, b1m, b1l, y1);
         contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b0h, b0m, b0l, y0, vt, lane, sub0, runmax,
-                                                           pd, bad);
+                                                           pd, bad, wbuf, sp);
         if (vt + 1 >= vt1) break;
         if (vt + 2 < vt1) load_tile<KK, NPY>(Y, YY, vt + 2, lane, b0h, b0m, b0l, y0);
         contract_tile<KK, R, PP, SIGN, FT, OUT, PIPE, NPY>(a, ahi, alo, b1h, b1m, b1l, y1, vt + 1, lane, sub0,
-                                                           runmax, pd, bad);
+                                                           runmax, pd, bad, wbuf, sp);
     }
     if constexpr (PIPE)
-        epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0]);
+        epi<PP, FT, OUT>(a, pd.acc, pd.yhi, pd.ylo, pd.vt, lane, sub0 + R - 1, runmax[R - 1], bad[0], wbuf[R - 1], sp);
     // rows past K (zero operand rows) have z = 0 and SSE = |y*|^2: never
     // flagged.  Rare: a wave that met an ill-conditioned sample adds one to
     // the counter and flags the permutations of each (subtile, half-wave)
@@ -313,6 +429,7 @@ __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16L
     }
     if (any && lane == 0) atomicAdd(a.unsafe, 1u);
 
+    if constexpr (OUT == 2) return;  // cluster extent: no maxima
     const int hl = lane >> 5;
     const float scale = FT ? a.f_scale : a.sqrt_dof;
 #pragma unroll
@@ -331,15 +448,17 @@ __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16L
         }
 }
 
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+template <int KK, int R, int PP, bool SIGN, bool FT, int OUT, bool PIPE>
 __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16Launch& L) {
     const int lb = xcd_logical_block(L.per_xcd);
     if (lb >= L.n_mgroups * L.n_vchunks) return;
     contract_item<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L, lb % L.n_mgroups, lb / L.n_mgroups);
 }
 
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
-__global__ __launch_bounds__(256) void k_contract_bf16(ContractArgs a, Bf16Launch L) {
+// (the rolling pipelined schedule held to 256 registers, two waves per SIMD)
+template <int KK, int R, int PP, bool SIGN, bool FT, int OUT, bool PIPE>
+__global__ __launch_bounds__(256, (PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK ? 2 : 1))
+void k_contract_bf16(ContractArgs a, Bf16Launch L) {
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
 }
 
@@ -386,7 +505,7 @@ __global__ __launch_bounds__(256, (persist_min_blocks<KK, R, SIGN, PIPE>())) voi
 
 // The rolling schedule (two data fragments, KK 5..8), held to 256 registers
 // so that two waves share each SIMD
-template <int KK, int R, int PP, bool SIGN, bool FT, bool OUT, bool PIPE>
+template <int KK, int R, int PP, bool SIGN, bool FT, int OUT, bool PIPE>
 __global__ __launch_bounds__(256, R >= 3 ? 1 : 2) void k_contract_bf16_roll(ContractArgs a, Bf16Launch L) {
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
 }
@@ -397,12 +516,18 @@ __global__ __launch_bounds__(256, R >= 3 ? 1 : 2) void k_contract_bf16_roll(Cont
 template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s, int* occ) {
     constexpr bool ROLL = !PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK;
-    const bool out = occ ? *occ != 0 : (a.stats || a.bits);
+    // 0: maxima only; 1: statistics and / or bitmask words; 2: the bitmask
+    // (and ties) alone, in 128-byte runs
+    const int mode = occ ? *occ : (a.bits && a.ties && !a.stats) ? 2 : (a.stats || a.bits) ? 1 : 0;
     void (*kern)(ContractArgs, Bf16Launch);
     if constexpr (ROLL)
-        kern = out ? k_contract_bf16_roll<KK, R, PP, SIGN, FT, true, PIPE> : k_contract_bf16_roll<KK, R, PP, SIGN, FT, false, PIPE>;
+        kern = mode == 2   ? k_contract_bf16_roll<KK, R, PP, SIGN, FT, 2, PIPE>
+               : mode == 1 ? k_contract_bf16_roll<KK, R, PP, SIGN, FT, 1, PIPE>
+                           : k_contract_bf16_roll<KK, R, PP, SIGN, FT, 0, PIPE>;
     else
-        kern = out ? k_contract_bf16<KK, R, PP, SIGN, FT, true, PIPE> : k_contract_bf16<KK, R, PP, SIGN, FT, false, PIPE>;
+        kern = mode == 2   ? k_contract_bf16<KK, R, PP, SIGN, FT, 2, PIPE>
+               : mode == 1 ? k_contract_bf16<KK, R, PP, SIGN, FT, 1, PIPE>
+                           : k_contract_bf16<KK, R, PP, SIGN, FT, 0, PIPE>;
     if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, reinterpret_cast<const void*>(kern), 256, 0);
     hipLaunchKernelGGL(kern, dim3(L.per_xcd * 8), dim3(256), 0, s, a, L);
     return hipGetLastError();

@@ -118,9 +118,12 @@ __global__ __launch_bounds__(256) void k_contract_f32mx(ContractArgs a, Bf16Laun
         const double yv = a.yy[vt * 32 + (lane & 31)];
         const float yhi = (float)yv;  // NaN for padding voxels
         const float ylo = (float)(yv - (double)yhi);
+        unsigned wb_unused[16 / PP];  // bitmask words of the bits-only epilogue (not used here)
 #pragma unroll
         for (int r = 0; r < R; r++)
-            if (live[r]) epi<PP, FT, OUT>(a, acc[r][0] + acc[r][1], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r]);
+            if (live[r])
+                epi<PP, FT, OUT ? 1 : 0>(a, acc[r][0] + acc[r][1], yhi, ylo, vt, lane, sub0 + r, runmax[r], bad[r],
+                                         wb_unused, TileSpan{0, 0});
     }
     // flags of the (subtile, half-wave) pairs that met an ill-conditioned
     // sample, as in the split-BF16 kernel

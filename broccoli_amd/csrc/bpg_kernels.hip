@@ -1051,12 +1051,17 @@ __device__ __forceinline__ void lds_union(unsigned* p, unsigned a, unsigned b) {
 }
 
 // LDS capacity for active voxels when the block may use lds_bytes, the
-// scan takes 2 * 1024 words and the bitmask and its ranks 2 * nwords
+// scan takes 2 * 1024 words and the bitmask and its ranks 2 * nwords; each
+// active voxel takes 3 words (its voxel index, parent and size)
 int ccl_lds_cap(int nwords, size_t lds_bytes) {
     const long words = (long)(std::min(lds_bytes, CCL_LDS_BYTES) / 4) - 2L * CCL_LDS_THREADS - 2L * nwords;
-    return words > 0 ? (int)(words / 2) : 0;
+    return words > 0 ? (int)(words / 3) : 0;
 }
 
+// One map per block: the bitmask and the rank of each word's first voxel in
+// LDS, then the list of active voxels (compact index -> voxel), so that the
+// union and size passes split the active voxels evenly over the threads
+// whatever their layout; union-find over compact indices, sizes, max.
 __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap, unsigned* overflow,
                                                               unsigned* n_overflow) {
     extern __shared__ unsigned smem[];
@@ -1066,7 +1071,8 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     const int nw = a.nwords;
     unsigned* sbits = smem;           // [nw]
     unsigned* srank = sbits + nw;     // [nw] active voxels before each word
-    unsigned* spar = srank + nw;      // [cap]
+    unsigned* slist = srank + nw;     // [cap] voxel of each compact index
+    unsigned* spar = slist + cap;     // [cap]
     unsigned* ssz = spar + cap;       // [cap]
     const unsigned* bits = a.bits + (size_t)map * nw;
     // 8 loads in flight per thread (indices clamped, stores guarded)
@@ -1094,52 +1100,40 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
         __syncthreads();
     }
     const unsigned total = scan[cur][CCL_LDS_THREADS - 1];
-    unsigned base = scan[cur][tid] - cnt;
-    for (int w = w0; w < w1; w++) {
-        srank[w] = base;
-        base += __popc(sbits[w]);
-    }
     if (total > (unsigned)cap) {  // the whole block takes this branch
         if (tid == 0) overflow[atomicAdd(n_overflow, 1u)] = (unsigned)map;
         return;
     }
-    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS) {
-        spar[i] = i;
-        ssz[i] = 0u;
+    unsigned base = scan[cur][tid] - cnt;
+    for (int w = w0; w < w1; w++) {
+        srank[w] = base;
+        unsigned word = sbits[w];
+        while (word) {
+            const int b = __ffs(word) - 1;
+            word &= word - 1;
+            slist[base] = (unsigned)(w * 32 + b);
+            spar[base] = base;
+            ssz[base] = 0u;
+            base++;
+        }
     }
     __syncthreads();
     auto active = [&](int n) { return n >= 0 && ((sbits[n >> 5] >> (n & 31)) & 1u); };
     auto rank = [&](int n) { return srank[n >> 5] + __popc(sbits[n >> 5] & ((1u << (n & 31)) - 1u)); };
     const int nb6 = a.connectivity == 6;
-    // A thread walks the active voxels of its words (tid, tid + 1024, ...)
-    // CCL_BATCH at a time: the neighbour rows of a batch are loaded together,
-    // so one memory latency serves CCL_BATCH voxels.  Linking by the smaller
-    // root makes the result independent of the order of the unions.
-    int ww = tid;
-    unsigned word = ww < nw ? sbits[ww] : 0u;
-    bool more = true;
-    while (more) {
+    // CCL_BATCH active voxels per thread at a time: the neighbour rows of a
+    // batch are loaded together, so one memory latency serves the batch.
+    // Linking by the smaller root makes the result independent of the order
+    // of the unions.  (Bits past the last voxel are never set.)
+    for (unsigned i0 = 0; i0 < total; i0 += CCL_BATCH * CCL_LDS_THREADS) {
         int cs[CCL_BATCH];
         unsigned ls[CCL_BATCH];
 #pragma unroll
         for (int u = 0; u < CCL_BATCH; u++) {
-            cs[u] = -1;
-            ls[u] = 0u;
-            while (more && !word) {
-                ww += CCL_LDS_THREADS;
-                if (ww < nw) word = sbits[ww];
-                else more = false;
-            }
-            if (!more) continue;
-            const int b = __ffs(word) - 1;
-            word &= word - 1;
-            const int c = ww * 32 + b;
-            if (c >= a.Vc) {  // bits past the last voxel (none are set): stop
-                more = false;
-                continue;
-            }
-            cs[u] = c;
-            ls[u] = srank[ww] + __popc(sbits[ww] & ((1u << b) - 1u));
+            const unsigned i = i0 + (unsigned)u * CCL_LDS_THREADS + (unsigned)tid;
+            ls[u] = i;
+            cs[u] = i < total ? (int)slist[i] : -1;
+            if (cs[u] >= a.Vc) cs[u] = -1;
         }
         int ncs[CCL_BATCH][BPG_NBR];
 #pragma unroll
@@ -1167,19 +1161,12 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
         }
     }
     __syncthreads();
-    for (int w = tid; w < nw; w += CCL_LDS_THREADS) {
-        unsigned word = sbits[w];
-        unsigned li = srank[w];
-        while (word) {
-            const int b = __ffs(word) - 1;
-            word &= word - 1;
-            const int c = w * 32 + b;
-            if (c >= a.Vc) break;
-            // extent (KC:403-430) or mass (KC:432-458)
-            const unsigned add = a.values ? (unsigned)(a.values[(size_t)map * a.Vc + c] * 10000.0f) : 1u;
-            atomicAdd(&ssz[lds_find(spar, li)], add);
-            li++;
-        }
+    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS) {
+        const int c = (int)slist[i];
+        if (c >= a.Vc) continue;
+        // extent (KC:403-430) or mass (KC:432-458)
+        const unsigned add = a.values ? (unsigned)(a.values[(size_t)map * a.Vc + c] * 10000.0f) : 1u;
+        atomicAdd(&ssz[lds_find(spar, i)], add);
     }
     __syncthreads();
     unsigned best = 0;
@@ -1202,7 +1189,7 @@ hipError_t launch_ccl_lds(const CclArgs& a, size_t lds_bytes, unsigned* overflow
                           hipStream_t s) {
     if (a.nmaps <= 0) return hipSuccess;
     const int cap = ccl_lds_cap(a.nwords, lds_bytes);
-    const size_t dyn = 4 * ((size_t)2 * a.nwords + (size_t)2 * cap);
+    const size_t dyn = 4 * ((size_t)2 * a.nwords + (size_t)3 * cap);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ccl_lds),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     if (e != hipSuccess) return e;
