@@ -74,7 +74,7 @@ def run_batch(args):
     n_fp32 = sum(p.timing().precision_used == capi.PREC_FP32 for p in plans)
     p_fp32 = sum(p.timing().fp32_permutations for p in plans)
     t0 = time.perf_counter()
-    md_single = [p.permutation_phase(0, **d) for p, d in zip(plans, perms)]
+    md_single = [p.permutation_phase(0, **d)[0] for p, d in zip(plans, perms)]
     seq = time.perf_counter() - t0
     print(f"# stream batch: {dt*1e3:.3f} ms/step (one study after another: {seq*1e3:.3f} ms)", file=sys.stderr,
           flush=True)

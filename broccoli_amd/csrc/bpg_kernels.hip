@@ -965,8 +965,9 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
     const int slot = (int)(t / a.nwords);
     const int map = a.map_ids ? (int)a.map_ids[slot] : slot;
     const int w = (int)(t % a.nwords);
-    unsigned* parent = a.parent + (size_t)map * a.Vc;
-    unsigned* sizes = a.sizes + (size_t)map * a.Vc;
+    // union-find scratch per slot (a map of a map_ids list, or the map)
+    unsigned* parent = a.parent + (size_t)slot * a.Vc;
+    unsigned* sizes = a.sizes + (size_t)slot * a.Vc;
     const unsigned* bits = a.bits + (size_t)map * a.nwords;
     unsigned word = bits[w];
     unsigned run_root = 0xFFFFFFFFu, run_sum = 0u, best = 0u;

@@ -61,13 +61,42 @@ void build_neighbours(int W, int H, int D, const std::vector<unsigned>& comp2gri
 
 int cluster_max(bpg_ctx* ctx, ClusterScratch& s, const CclArgs& base, long K, unsigned* maxsize, hipStream_t st) {
     const int Vc = base.Vc;
+    // union-find scratch of the global path: parent and size per voxel for
+    // up to B maps at a time
     const size_t budget = (size_t)8 << 30;
     const size_t per_map = (size_t)std::max(Vc, 1) * 2 * sizeof(unsigned);
-    long B = (long)std::max<size_t>(1, budget / per_map);
-    B = std::min<long>(B, K);
+    const long B = std::min<long>((long)std::max<size_t>(1, budget / per_map), K);
+    BPG_HIP(ctx, hipMemsetAsync(maxsize, 0, K * sizeof(unsigned), st));
+    if (K > 0 && ccl_lds_fits(base.nwords, ctx->lds_per_block)) {
+        // every map in LDS, one launch; the few with too many active voxels
+        // (listed by the kernel) take the global path afterwards
+        CclArgs cc = base;
+        cc.maxsize = maxsize;
+        cc.labels = nullptr;
+        cc.map_ids = nullptr;
+        cc.nmaps = (int)K;
+        BPG_HIP(ctx, s.overflow.ensure((size_t)K + 1));
+        BPG_HIP(ctx, hipMemsetAsync(s.overflow.p + K, 0, sizeof(unsigned), st));
+        BPG_HIP(ctx, launch_ccl_lds(cc, ctx->lds_per_block, s.overflow.p, s.overflow.p + K, st));
+        unsigned n_over = 0;
+        BPG_HIP(ctx, hipMemcpyAsync(&n_over, s.overflow.p + K, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, hipStreamSynchronize(st));
+        if (n_over > 0) {
+            const long B2 = std::min<long>(B, n_over);
+            BPG_HIP(ctx, s.parent.ensure((size_t)B2 * std::max(Vc, 1)));
+            BPG_HIP(ctx, s.sizes.ensure((size_t)B2 * std::max(Vc, 1)));
+            cc.parent = s.parent.p;
+            cc.sizes = s.sizes.p;
+            for (long j0 = 0; j0 < (long)n_over; j0 += B2) {
+                cc.map_ids = s.overflow.p + j0;
+                cc.nmaps = (int)std::min<long>(B2, (long)n_over - j0);
+                BPG_HIP(ctx, launch_ccl(cc, st));
+            }
+        }
+        return BPG_OK;
+    }
     BPG_HIP(ctx, s.parent.ensure((size_t)B * std::max(Vc, 1)));
     BPG_HIP(ctx, s.sizes.ensure((size_t)B * std::max(Vc, 1)));
-    BPG_HIP(ctx, hipMemsetAsync(maxsize, 0, K * sizeof(unsigned), st));
     for (long k0 = 0; k0 < K; k0 += B) {
         CclArgs cc = base;
         cc.bits = base.bits + (size_t)k0 * base.nwords;
@@ -78,23 +107,7 @@ int cluster_max(bpg_ctx* ctx, ClusterScratch& s, const CclArgs& base, long K, un
         cc.values = base.values ? base.values + (size_t)k0 * Vc : nullptr;
         cc.map_ids = nullptr;
         cc.nmaps = (int)std::min<long>(B, K - k0);
-        if (ccl_lds_fits(base.nwords, ctx->lds_per_block)) {
-            // whole maps in LDS; the few with too many active voxels
-            // (listed by the kernel) take the global path
-            BPG_HIP(ctx, s.overflow.ensure((size_t)cc.nmaps + 1));
-            BPG_HIP(ctx, hipMemsetAsync(s.overflow.p + cc.nmaps, 0, sizeof(unsigned), st));
-            BPG_HIP(ctx, launch_ccl_lds(cc, ctx->lds_per_block, s.overflow.p, s.overflow.p + cc.nmaps, st));
-            unsigned n_over = 0;
-            BPG_HIP(ctx, hipMemcpyAsync(&n_over, s.overflow.p + cc.nmaps, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-            BPG_HIP(ctx, hipStreamSynchronize(st));
-            if (n_over > 0) {
-                cc.map_ids = s.overflow.p;
-                cc.nmaps = (int)n_over;
-                BPG_HIP(ctx, launch_ccl(cc, st));
-            }
-        } else {
-            BPG_HIP(ctx, launch_ccl(cc, st));
-        }
+        BPG_HIP(ctx, launch_ccl(cc, st));
     }
     return BPG_OK;
 }
