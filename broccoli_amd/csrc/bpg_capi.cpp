@@ -1537,8 +1537,9 @@ struct bpg_batch {
     HBuf<float> h_sign, h_max;
     HBuf<unsigned> h_unsafe;
     HBuf<unsigned char> h_flags;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
     hipEvent_t first_done = nullptr;
+    hipEvent_t fork[2] = {}, join[2] = {};
     hipGraph_t graph[2] = {};
     hipGraphExec_t exec[2] = {};
     double last_ms = 0.0, last_graph_ms = 0.0;
@@ -1554,6 +1555,11 @@ struct bpg_batch {
             if (graph[g]) (void)hipGraphDestroy(graph[g]);
         }
         if (first_done) (void)hipEventDestroy(first_done);
+        for (int g = 0; g < 2; g++) {
+            if (fork[g]) (void)hipEventDestroy(fork[g]);
+            if (join[g]) (void)hipEventDestroy(join[g]);
+        }
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto& s : st) {
             s.yy.release(); s.Yb.release(); s.Gb.release(); s.partial.release();
@@ -1712,10 +1718,15 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
         BPG_BHIP(hipMemcpy(b->jobs.p, rjobs.data(), rjobs.size() * sizeof(bpg::ReduceJob), hipMemcpyHostToDevice));
     BPG_BHIP(hipStreamCreate(&b->stream));
     BPG_BHIP(hipEventCreateWithFlags(&b->first_done, hipEventDisableTiming));
-    // Two graphs, each one chain on one stream.  The first: the upload of
-    // all permutations / signs and one fill clearing the queue heads,
-    // ill-conditioned counts and flags (a kernel, not a memset node); then,
-    // as the second, the stage's classes, the largest work first, each its
+    BPG_BHIP(hipStreamCreate(&b->side));
+    for (int g = 0; g < 2; g++) {
+        BPG_BHIP(hipEventCreateWithFlags(&b->fork[g], hipEventDisableTiming));
+        BPG_BHIP(hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming));
+    }
+    // Two graphs.  The first: the upload of all permutations / signs and one
+    // fill clearing the queue heads, ill-conditioned counts and flags (a
+    // kernel, not a memset node); then, as the second, the stage's classes,
+    // the largest work first and taken in turn by two branches, each its
     // members' nuisance transforms and operand builds and then its
     // persistent contraction; one launch of the stage's max reductions; the
     // copies of its maxima, counts and flags to the host.  Stage 0 holds
@@ -1752,10 +1763,16 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 cap(hipMemcpyAsync(b->sign_all.p, b->h_sign.p, b->n_sign * sizeof(float), hipMemcpyHostToDevice, cs));
             cap(bpg::launch_fill_u32(b->words.p, n_words, 0u, cs));
         }
+        // two branches take the stage's classes in turn, so that a class's
+        // blocks start in the tail of the one before it
+        cap(hipEventRecord(b->fork[g], cs));
+        cap(hipStreamWaitEvent(b->side, b->fork[g], 0));
+        int turn = 0;
         for (int ci : order) {
             if (rc != BPG_OK) break;
             const auto& c = b->cls[ci];
             if (c.stage != g) continue;
+            hipStream_t bs = (turn++ & 1) ? b->side : cs;
             // the members' data and operand fragments shortly before their
             // contraction, while they are still in the last-level cache
             for (int i : c.members) {
@@ -1771,7 +1788,7 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 ta.N = s.N; ta.q = m.q; ta.Vc = pl->Vc; ta.Vpad = pl->Vpad;
                 ta.KK = s.KK; ta.NS = 0; ta.npy = s.npy;
                 ta.yy = s.yy.p; ta.Yb = s.Yb.p;
-                cap(bpg::launch_transform(ta, cs));
+                cap(bpg::launch_transform(ta, bs));
                 bpg::OperandArgs oa{};
                 oa.perm = s.sign ? nullptr : b->perm_all.p + s.in_off;
                 oa.sign = s.sign ? b->sign_all.p + s.in_off : nullptr;
@@ -1779,15 +1796,17 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 oa.N = s.N; oa.pfit = m.pfit; oa.K = (int)s.K; oa.PP = s.PP; oa.KK = s.KK;
                 oa.n_sub = s.L.n_sub;
                 oa.Gb = s.Gb.p;
-                cap(bpg::launch_operand_bf16(oa, cs));
+                cap(bpg::launch_operand_bf16(oa, bs));
             }
             bpg::BatchClass bc{};
             bc.studies = b->table.p + c.first;
             bc.n_studies = (int)c.members.size();
             std::copy(c.n_items, c.n_items + bpg::BPG_XCDS, bc.n_items);
             bc.counter = b->heads() + (size_t)ci * bpg::BPG_XCDS;
-            cap(bpg::launch_contract_bf16_persist(bc, c.KK, c.PP, c.sign != 0, c.variant, c.grid, cs));
+            cap(bpg::launch_contract_bf16_persist(bc, c.KK, c.PP, c.sign != 0, c.variant, c.grid, bs));
         }
+        cap(hipEventRecord(b->join[g], b->side));
+        cap(hipStreamWaitEvent(cs, b->join[g], 0));
         if (!empty) {
             cap(bpg::launch_reduce_max_jobs(b->jobs.p + j0, b->job_end[g] - j0, b->max_K, cs));
             cap(hipMemcpyAsync(b->h_max.p + m0, b->max_all.p + m0, (b->max_end[g] - m0) * sizeof(float),

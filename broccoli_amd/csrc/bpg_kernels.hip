@@ -1051,11 +1051,11 @@ __device__ __forceinline__ void lds_union(unsigned* p, unsigned a, unsigned b) {
     }
 }
 
-// LDS capacity for active voxels when the block may use lds_bytes, the
-// scan takes 2 * 1024 words and the bitmask and its ranks 2 * nwords; each
-// active voxel takes 3 words (its voxel index, parent and size)
+// LDS capacity for active voxels when the block may use lds_bytes: the
+// scan's wave totals take 64 words, the bitmask and its ranks 2 * nwords;
+// each active voxel takes 3 words (its voxel index, parent and size)
 int ccl_lds_cap(int nwords, size_t lds_bytes) {
-    const long words = (long)(std::min(lds_bytes, CCL_LDS_BYTES) / 4) - 2L * CCL_LDS_THREADS - 2L * nwords;
+    const long words = (long)(std::min(lds_bytes, CCL_LDS_BYTES) / 4) - 64L - 2L * nwords;
     return words > 0 ? (int)(words / 3) : 0;
 }
 
@@ -1066,9 +1066,12 @@ int ccl_lds_cap(int nwords, size_t lds_bytes) {
 __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap, unsigned* overflow,
                                                               unsigned* n_overflow) {
     extern __shared__ unsigned smem[];
-    __shared__ unsigned scan[2][CCL_LDS_THREADS];
+    constexpr int NWAVES = CCL_LDS_THREADS / 64;
+    __shared__ unsigned wsum[NWAVES];
+    __shared__ unsigned scan[NWAVES];  // per-wave maxima at the end
     const int map = blockIdx.x;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int nw = a.nwords;
     unsigned* sbits = smem;           // [nw]
     unsigned* srank = sbits + nw;     // [nw] active voxels before each word
@@ -1086,37 +1089,61 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
             if (w0 + u * CCL_LDS_THREADS < nw) sbits[w0 + u * CCL_LDS_THREADS] = v[u];
     }
     __syncthreads();
-    // ranks: contiguous chunks per thread, block scan of the chunk counts
+    // ranks: contiguous chunks per thread; an in-wave scan of the chunk
+    // counts (shuffles), then the scan of the 16 wave totals
     const int per = (nw + CCL_LDS_THREADS - 1) / CCL_LDS_THREADS;
     const int w0 = min(tid * per, nw), w1 = min(w0 + per, nw);
     unsigned cnt = 0;
     for (int w = w0; w < w1; w++) cnt += __popc(sbits[w]);
-    int cur = 0;
-    scan[0][tid] = cnt;
-    __syncthreads();
-    for (int off = 1; off < CCL_LDS_THREADS; off <<= 1) {
-        const unsigned v = scan[cur][tid] + (tid >= off ? scan[cur][tid - off] : 0u);
-        scan[cur ^ 1][tid] = v;
-        cur ^= 1;
-        __syncthreads();
+    unsigned incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned v = (unsigned)__shfl_up((int)incl, off, 64);
+        if (lane >= off) incl += v;
     }
-    const unsigned total = scan[cur][CCL_LDS_THREADS - 1];
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    unsigned before = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < NWAVES; i++) {
+        const unsigned v = wsum[i];
+        before += i < wave ? v : 0u;
+        total += v;
+    }
     if (total > (unsigned)cap) {  // the whole block takes this branch
         if (tid == 0) overflow[atomicAdd(n_overflow, 1u)] = (unsigned)map;
         return;
     }
-    unsigned base = scan[cur][tid] - cnt;
+    unsigned base = before + incl - cnt;
     for (int w = w0; w < w1; w++) {
         srank[w] = base;
-        unsigned word = sbits[w];
-        while (word) {
-            const int b = __ffs(word) - 1;
-            word &= word - 1;
-            slist[base] = (unsigned)(w * 32 + b);
-            spar[base] = base;
-            ssz[base] = 0u;
-            base++;
+        base += __popc(sbits[w]);
+    }
+    __syncthreads();
+    // the list: compact index i is the voxel of the i-th set bit -- the last
+    // word whose rank is <= i (binary search), then the (i - rank)-th set
+    // bit of that word (binary select); every thread takes the same share
+    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS) {
+        int lo = 0, hi = nw - 1;  // srank[lo] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (srank[mid] <= i) lo = mid;
+            else hi = mid - 1;
         }
+        unsigned x = sbits[lo], j = i - srank[lo];
+        int b = 0;
+#pragma unroll
+        for (int sh = 16; sh > 0; sh >>= 1) {
+            const unsigned c = __popc(x & ((1u << sh) - 1u));
+            if (j >= c) {
+                j -= c;
+                x >>= sh;
+                b += sh;
+            }
+        }
+        slist[i] = (unsigned)(lo * 32 + b);
+        spar[i] = i;
+        ssz[i] = 0u;
     }
     __syncthreads();
     auto active = [&](int n) { return n >= 0 && ((sbits[n >> 5] >> (n & 31)) & 1u); };
@@ -1174,12 +1201,11 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     for (unsigned i = tid; i < total; i += CCL_LDS_THREADS)
         if (spar[i] == i) best = max(best, ssz[i]);
     for (int o = 32; o > 0; o >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, o, 64));
-    __syncthreads();
-    if ((tid & 63) == 0) scan[0][tid >> 6] = best;
+    if (lane == 0) scan[wave] = best;
     __syncthreads();
     if (tid == 0) {
         unsigned m = 0;
-        for (int i = 0; i < CCL_LDS_THREADS / 64; i++) m = max(m, scan[0][i]);
+        for (int i = 0; i < NWAVES; i++) m = max(m, scan[i]);
         a.maxsize[map] = m;
     }
 }
