@@ -314,15 +314,17 @@ struct CclArgs {
     const unsigned* map_ids;    // [nmaps] batch map of each slot (null: slot = map)
     int Vc, nwords, nmaps, connectivity;
 };
-// Whole-map CCL in LDS, one 1024-thread block per map: the map's bitmask,
-// the rank of each word (exclusive prefix of popcounts) and a union-find
-// over the map's active voxels (at most `cap`) all live in LDS.  Maps with
-// more active voxels are appended to overflow[*n_overflow] and left to the
-// global path.  Writes maxsize[map].
+// Whole-map CCL in LDS, one block per map (map_ids[b], or b, for the n_maps
+// blocks): the map's bitmask, the ranks of its words (exclusive prefix of
+// popcounts) and a union-find over the map's active voxels all live in LDS.
+// blocks = 2: 512-thread blocks, two per CU, each with half the LDS;
+// blocks = 1: 1024-thread blocks with all of it.  Maps with more active
+// voxels than the block holds are appended to overflow[*n_overflow], for
+// the next tier or the global path.  Writes maxsize[map].
 // lds_bytes: the device's shared memory per block (at most 160 KB is used)
 bool ccl_lds_fits(int nwords, size_t lds_bytes);
-hipError_t launch_ccl_lds(const CclArgs& a, size_t lds_bytes, unsigned* overflow, unsigned* n_overflow,
-                          hipStream_t s);
+hipError_t launch_ccl_lds(const CclArgs& a, size_t lds_bytes, int blocks, const unsigned* map_ids, int n_maps,
+                          unsigned* overflow, unsigned* n_overflow, hipStream_t s);
 // stage 4 (TFCE): tfce[map][c] += sqrt(size of c's component) * level^2
 hipError_t launch_ccl_tfce_level(const CclArgs& a, hipStream_t s);
 

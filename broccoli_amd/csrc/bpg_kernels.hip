@@ -1020,9 +1020,9 @@ __global__ __launch_bounds__(256) void k_ccl(CclArgs a, int stage) {
 }
 
 // ---- whole-map CCL in LDS ----
-constexpr int CCL_LDS_THREADS = 1024;
 constexpr int CCL_BATCH = 4;  // active voxels whose neighbour rows load together
 constexpr size_t CCL_LDS_BYTES = 160 * 1024;  // LDS of one CU (gfx950)
+constexpr long CCL_STATIC_WORDS = 256;        // the kernels' static LDS is kept below this
 
 // the root of x (reads only: halving the path with writes measured slower,
 // 30 against 26 us per map on the F + cluster workload)
@@ -1051,48 +1051,58 @@ __device__ __forceinline__ void lds_union(unsigned* p, unsigned a, unsigned b) {
     }
 }
 
-// LDS capacity for active voxels when the block may use lds_bytes: the
-// kernel's static LDS (wave totals, the barrier reductions) is kept under
-// 256 words, the bitmask and its ranks take 2 * nwords; each active voxel
-// takes 3 words (its voxel index, parent and size)
-int ccl_lds_cap(int nwords, size_t lds_bytes) {
-    const long words = (long)(std::min(lds_bytes, CCL_LDS_BYTES) / 4) - 256L - 2L * nwords;
+// LDS words of a map's bitmask (nwords), its 64-word group ranks and its
+// 16-bit word ranks within the group
+static long ccl_lds_fixed_words(int nwords) {
+    const long ng = (nwords + 63) / 64;
+    return (long)nwords + ng + ((long)nwords + 1) / 2;
+}
+
+// active voxels one block can hold when `blocks` blocks share a CU's LDS
+// (lds_bytes available to one block); each takes 3 words (its voxel index,
+// parent and size)
+int ccl_lds_cap(int nwords, size_t lds_bytes, int blocks) {
+    const long words = (long)(std::min(lds_bytes, CCL_LDS_BYTES) / 4 / blocks) - CCL_STATIC_WORDS - ccl_lds_fixed_words(nwords);
     return words > 0 ? (int)(words / 3) : 0;
 }
 
-// One map per block: the bitmask and the rank of each word's first voxel in
-// LDS, then the list of active voxels (compact index -> voxel), so that the
-// union and size passes split the active voxels evenly over the threads
-// whatever their layout; union-find over compact indices, sizes, max.
-__global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap, unsigned* overflow,
-                                                              unsigned* n_overflow) {
+// One map per block (map_ids[blockIdx.x], or blockIdx.x): the bitmask, the
+// rank (active voxels before) of each 64-word group and, in 16 bits, of each
+// word within its group in LDS; then the list of active voxels (compact
+// index -> voxel, built by rank and select, so every thread takes the same
+// share whatever the layout); union-find over compact indices, sizes, max.
+template <int T>
+__global__ __launch_bounds__(T) void k_ccl_lds(CclArgs a, int cap, const unsigned* map_ids, unsigned* overflow,
+                                               unsigned* n_overflow) {
     extern __shared__ unsigned smem[];
-    constexpr int NWAVES = CCL_LDS_THREADS / 64;
+    constexpr int NWAVES = T / 64;
     __shared__ unsigned wsum[NWAVES];
     __shared__ unsigned scan[NWAVES];  // per-wave maxima at the end
-    const int map = blockIdx.x;
+    const int map = map_ids ? (int)map_ids[blockIdx.x] : (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int nw = a.nwords;
-    unsigned* sbits = smem;           // [nw]
-    unsigned* srank = sbits + nw;     // [nw] active voxels before each word
-    unsigned* slist = srank + nw;     // [cap] voxel of each compact index
-    unsigned* spar = slist + cap;     // [cap]
-    unsigned* ssz = spar + cap;       // [cap]
+    const int ng = (nw + 63) / 64;
+    unsigned* sbits = smem;                                   // [nw]
+    unsigned* gbase = sbits + nw;                             // [ng] active voxels before each group
+    unsigned short* srank = reinterpret_cast<unsigned short*>(gbase + ng);  // [nw] within the group
+    unsigned* slist = gbase + ng + (nw + 1) / 2;              // [cap] voxel of each compact index
+    unsigned* spar = slist + cap;                             // [cap]
+    unsigned* ssz = spar + cap;                               // [cap]
     const unsigned* bits = a.bits + (size_t)map * nw;
     // 8 loads in flight per thread (indices clamped, stores guarded)
-    for (int w0 = tid; w0 < nw; w0 += 8 * CCL_LDS_THREADS) {
+    for (int w0 = tid; w0 < nw; w0 += 8 * T) {
         unsigned v[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = bits[min(w0 + u * CCL_LDS_THREADS, nw - 1)];
+        for (int u = 0; u < 8; u++) v[u] = bits[min(w0 + u * T, nw - 1)];
 #pragma unroll
         for (int u = 0; u < 8; u++)
-            if (w0 + u * CCL_LDS_THREADS < nw) sbits[w0 + u * CCL_LDS_THREADS] = v[u];
+            if (w0 + u * T < nw) sbits[w0 + u * T] = v[u];
     }
     __syncthreads();
     // ranks: contiguous chunks per thread; an in-wave scan of the chunk
-    // counts (shuffles), then the scan of the 16 wave totals
-    const int per = (nw + CCL_LDS_THREADS - 1) / CCL_LDS_THREADS;
+    // counts (shuffles), then the scan of the wave totals
+    const int per = (nw + T - 1) / T;
     const int w0 = min(tid * per, nw), w1 = min(w0 + per, nw);
     unsigned cnt = 0;
     for (int w = w0; w < w1; w++) cnt += __popc(sbits[w]);
@@ -1115,23 +1125,39 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
         if (tid == 0) overflow[atomicAdd(n_overflow, 1u)] = (unsigned)map;
         return;
     }
-    unsigned base = before + incl - cnt;
+    const unsigned first = before + incl - cnt;
+    unsigned base = first;
     for (int w = w0; w < w1; w++) {
-        srank[w] = base;
+        if ((w & 63) == 0) gbase[w >> 6] = base;
         base += __popc(sbits[w]);
     }
     __syncthreads();
+    base = first;
+    for (int w = w0; w < w1; w++) {
+        srank[w] = (unsigned short)(base - gbase[w >> 6]);
+        base += __popc(sbits[w]);
+    }
+    __syncthreads();
+    auto wrank = [&](int w) { return gbase[w >> 6] + (unsigned)srank[w]; };
     // the list: compact index i is the voxel of the i-th set bit -- the last
-    // word whose rank is <= i (binary search), then the (i - rank)-th set
-    // bit of that word (binary select); every thread takes the same share
-    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS) {
-        int lo = 0, hi = nw - 1;  // srank[lo] <= i
+    // group, then the last word in it, whose rank is <= i (binary searches),
+    // then the (i - rank)-th set bit of that word (binary select)
+    for (unsigned i = tid; i < total; i += T) {
+        int lo = 0, hi = ng - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (srank[mid] <= i) lo = mid;
+            if (gbase[mid] <= i) lo = mid;
             else hi = mid - 1;
         }
-        unsigned x = sbits[lo], j = i - srank[lo];
+        const int g = lo;
+        lo = g * 64;
+        hi = min(nw, g * 64 + 64) - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (wrank(mid) <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        unsigned x = sbits[lo], j = i - wrank(lo);
         int b = 0;
 #pragma unroll
         for (int sh = 16; sh > 0; sh >>= 1) {
@@ -1148,18 +1174,18 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     }
     __syncthreads();
     auto active = [&](int n) { return n >= 0 && ((sbits[n >> 5] >> (n & 31)) & 1u); };
-    auto rank = [&](int n) { return srank[n >> 5] + __popc(sbits[n >> 5] & ((1u << (n & 31)) - 1u)); };
+    auto rank = [&](int n) { return wrank(n >> 5) + __popc(sbits[n >> 5] & ((1u << (n & 31)) - 1u)); };
     const int nb6 = a.connectivity == 6;
     // CCL_BATCH active voxels per thread at a time: the neighbour rows of a
     // batch are loaded together, so one memory latency serves the batch.
     // Linking by the smaller root makes the result independent of the order
     // of the unions.  (Bits past the last voxel are never set.)
-    for (unsigned i0 = 0; i0 < total; i0 += CCL_BATCH * CCL_LDS_THREADS) {
+    for (unsigned i0 = 0; i0 < total; i0 += CCL_BATCH * T) {
         int cs[CCL_BATCH];
         unsigned ls[CCL_BATCH];
 #pragma unroll
         for (int u = 0; u < CCL_BATCH; u++) {
-            const unsigned i = i0 + (unsigned)u * CCL_LDS_THREADS + (unsigned)tid;
+            const unsigned i = i0 + (unsigned)u * T + (unsigned)tid;
             ls[u] = i;
             cs[u] = i < total ? (int)slist[i] : -1;
             if (cs[u] >= a.Vc) cs[u] = -1;
@@ -1190,7 +1216,7 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
         }
     }
     __syncthreads();
-    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS) {
+    for (unsigned i = tid; i < total; i += T) {
         const int c = (int)slist[i];
         if (c >= a.Vc) continue;
         // extent (KC:403-430) or mass (KC:432-458)
@@ -1199,7 +1225,7 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     }
     __syncthreads();
     unsigned best = 0;
-    for (unsigned i = tid; i < total; i += CCL_LDS_THREADS)
+    for (unsigned i = tid; i < total; i += T)
         if (spar[i] == i) best = max(best, ssz[i]);
     for (int o = 32; o > 0; o >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, o, 64));
     if (lane == 0) scan[wave] = best;
@@ -1211,17 +1237,26 @@ __global__ __launch_bounds__(CCL_LDS_THREADS) void k_ccl_lds(CclArgs a, int cap,
     }
 }
 
-bool ccl_lds_fits(int nwords, size_t lds_bytes) { return nwords > 0 && ccl_lds_cap(nwords, lds_bytes) >= 4096; }
+bool ccl_lds_fits(int nwords, size_t lds_bytes) { return nwords > 0 && ccl_lds_cap(nwords, lds_bytes, 1) >= 4096; }
 
-hipError_t launch_ccl_lds(const CclArgs& a, size_t lds_bytes, unsigned* overflow, unsigned* n_overflow,
-                          hipStream_t s) {
-    if (a.nmaps <= 0) return hipSuccess;
-    const int cap = ccl_lds_cap(a.nwords, lds_bytes);
-    const size_t dyn = 4 * ((size_t)2 * a.nwords + (size_t)3 * cap);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ccl_lds),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ccl_lds, dim3(a.nmaps), dim3(CCL_LDS_THREADS), dyn, s, a, cap, overflow, n_overflow);
+// blocks = 2: 512-thread blocks, two per CU; blocks = 1: 1024-thread blocks
+// with the whole LDS (for the maps the first listed as too large)
+hipError_t launch_ccl_lds(const CclArgs& a, size_t lds_bytes, int blocks, const unsigned* map_ids, int n_maps,
+                          unsigned* overflow, unsigned* n_overflow, hipStream_t s) {
+    if (n_maps <= 0) return hipSuccess;
+    const int cap = ccl_lds_cap(a.nwords, lds_bytes, blocks);
+    const size_t dyn = 4 * ((size_t)ccl_lds_fixed_words(a.nwords) + (size_t)3 * cap);
+    if (blocks == 2) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ccl_lds<512>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_ccl_lds<512>, dim3(n_maps), dim3(512), dyn, s, a, cap, map_ids, overflow, n_overflow);
+    } else {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ccl_lds<1024>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_ccl_lds<1024>, dim3(n_maps), dim3(1024), dyn, s, a, cap, map_ids, overflow, n_overflow);
+    }
     return hipGetLastError();
 }
 

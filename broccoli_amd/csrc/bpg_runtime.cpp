@@ -68,28 +68,37 @@ int cluster_max(bpg_ctx* ctx, ClusterScratch& s, const CclArgs& base, long K, un
     const long B = std::min<long>((long)std::max<size_t>(1, budget / per_map), K);
     BPG_HIP(ctx, hipMemsetAsync(maxsize, 0, K * sizeof(unsigned), st));
     if (K > 0 && ccl_lds_fits(base.nwords, ctx->lds_per_block)) {
-        // every map in LDS, one launch; the few with too many active voxels
-        // (listed by the kernel) take the global path afterwards
+        // every map in LDS, two blocks per CU, one launch; the maps with too
+        // many active voxels for that (listed by the kernel) take one block
+        // per CU, and the few still too large the global path
         CclArgs cc = base;
         cc.maxsize = maxsize;
         cc.labels = nullptr;
         cc.map_ids = nullptr;
         cc.nmaps = (int)K;
-        BPG_HIP(ctx, s.overflow.ensure((size_t)K + 1));
-        BPG_HIP(ctx, hipMemsetAsync(s.overflow.p + K, 0, sizeof(unsigned), st));
-        BPG_HIP(ctx, launch_ccl_lds(cc, ctx->lds_per_block, s.overflow.p, s.overflow.p + K, st));
-        unsigned n_over = 0;
-        BPG_HIP(ctx, hipMemcpyAsync(&n_over, s.overflow.p + K, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        BPG_HIP(ctx, s.overflow.ensure((size_t)2 * K + 2));
+        unsigned* list1 = s.overflow.p;          // [K] maps left by the first tier
+        unsigned* list2 = s.overflow.p + K;      // [K] maps left by the second
+        unsigned* counts = s.overflow.p + 2 * K; // [2]
+        BPG_HIP(ctx, hipMemsetAsync(counts, 0, 2 * sizeof(unsigned), st));
+        BPG_HIP(ctx, launch_ccl_lds(cc, ctx->lds_per_block, 2, nullptr, (int)K, list1, counts, st));
+        unsigned n_over[2] = {0, 0};
+        BPG_HIP(ctx, hipMemcpyAsync(n_over, counts, sizeof(unsigned), hipMemcpyDeviceToHost, st));
         BPG_HIP(ctx, hipStreamSynchronize(st));
-        if (n_over > 0) {
-            const long B2 = std::min<long>(B, n_over);
+        if (n_over[0] > 0) {
+            BPG_HIP(ctx, launch_ccl_lds(cc, ctx->lds_per_block, 1, list1, (int)n_over[0], list2, counts + 1, st));
+            BPG_HIP(ctx, hipMemcpyAsync(n_over + 1, counts + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            BPG_HIP(ctx, hipStreamSynchronize(st));
+        }
+        if (n_over[1] > 0) {
+            const long B2 = std::min<long>(B, n_over[1]);
             BPG_HIP(ctx, s.parent.ensure((size_t)B2 * std::max(Vc, 1)));
             BPG_HIP(ctx, s.sizes.ensure((size_t)B2 * std::max(Vc, 1)));
             cc.parent = s.parent.p;
             cc.sizes = s.sizes.p;
-            for (long j0 = 0; j0 < (long)n_over; j0 += B2) {
-                cc.map_ids = s.overflow.p + j0;
-                cc.nmaps = (int)std::min<long>(B2, (long)n_over - j0);
+            for (long j0 = 0; j0 < (long)n_over[1]; j0 += B2) {
+                cc.map_ids = list2 + j0;
+                cc.nmaps = (int)std::min<long>(B2, (long)n_over[1] - j0);
                 BPG_HIP(ctx, launch_ccl(cc, st));
             }
         }

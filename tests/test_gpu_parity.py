@@ -491,14 +491,16 @@ def test_ill_conditioned_permutations_repeat_in_fp32(ctx):
 
 
 def test_cluster_extent_large_components_take_the_global_path(ctx):
-    """Maps with more active voxels than the LDS path holds (here every
-    voxel is above a threshold of -1) are labelled by the global path; both
-    paths agree with the oracle."""
+    """The three CCL tiers against the oracle: at a threshold of 3 the maps
+    fit the two-blocks-per-CU LDS kernel; at 1.2 (about 30% of the 28k
+    voxels active, more than its 6.3k but within the 13k of the one-block
+    kernel) they go to the second; at -1 every voxel is active and the
+    global path labels them."""
     st = synth.study(3, scale=0.5)
     st["inference"] = capi.CLUSTER_EXTENT
     K = 6
     perms = synth.permutations(st, K, seed=29)
-    for cdt in (-1.0, 3.0):
+    for cdt in (-1.0, 1.2, 3.0):
         st["cdt"] = cdt
         with capi.Plan(ctx, st, "split_bf16") as plan:
             md, stats = plan.permutation_phase(0, want_stats=True, **perms)
@@ -507,6 +509,9 @@ def test_cluster_extent_large_components_take_the_global_path(ctx):
             assert md[k] == float(best), (cdt, k)
         if cdt < 0:
             assert np.all(md == st["V"])
+        if cdt == 1.2:
+            active = (stats > cdt).sum(axis=1)
+            assert ((active > 6300) & (active < 13000)).any(), active
 
 
 def _sentinel():
