@@ -86,6 +86,10 @@ struct bpg_plan {
     std::vector<unsigned> comp2grid;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
+    // the permutations' upload runs on a stream of its own, beside the
+    // nuisance transform; the operand build waits for it
+    hipStream_t copy = nullptr;
+    hipEvent_t copied = nullptr;
     bpg_timing timing{};
     bool have_obs = false;
     // cluster inference: per map, the tied maps of its last permutation phase
@@ -343,8 +347,22 @@ extern "C" int bpg_plan_create(bpg_ctx* ctx, const bpg_study_desc* desc, bpg_pla
         delete pl;
         return BPG_ERR_ARG;
     }
-    auto fail = [&](int rc) { pl->release_all(); if (pl->stream) (void)hipStreamDestroy(pl->stream); delete pl; return rc; };
-    if (hipStreamCreate(&pl->stream) != hipSuccess) { ctx->err = "hipStreamCreate failed"; return fail(BPG_ERR_HIP); }
+    auto fail = [&](int rc) {
+        pl->release_all();
+        if (pl->stream) (void)hipStreamDestroy(pl->stream);
+        if (pl->copy) (void)hipStreamDestroy(pl->copy);
+        if (pl->copied) (void)hipEventDestroy(pl->copied);
+        delete pl;
+        return rc;
+    };
+    if (hipStreamCreate(&pl->stream) != hipSuccess || hipStreamCreate(&pl->copy) != hipSuccess) {
+        ctx->err = "hipStreamCreate failed";
+        return fail(BPG_ERR_HIP);
+    }
+    if (hipEventCreateWithFlags(&pl->copied, hipEventDisableTiming) != hipSuccess) {
+        ctx->err = "hipEventCreate failed";
+        return fail(BPG_ERR_HIP);
+    }
     for (auto& e : pl->ev)
         if (hipEventCreate(&e) != hipSuccess) { ctx->err = "hipEventCreate failed"; return fail(BPG_ERR_HIP); }
     // cluster inference: compact indices of the 13 lower-index 26-neighbours
@@ -403,10 +421,13 @@ extern "C" void bpg_plan_destroy(bpg_plan* pl) {
     if (!pl) return;
     (void)hipSetDevice(pl->ctx->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    if (pl->copy) (void)hipStreamSynchronize(pl->copy);
     pl->release_all();
     for (auto& e : pl->ev)
         if (e) (void)hipEventDestroy(e);
+    if (pl->copied) (void)hipEventDestroy(pl->copied);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    if (pl->copy) (void)hipStreamDestroy(pl->copy);
     delete pl;
 }
 
@@ -583,9 +604,14 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         if (f8) BPG_HIP(ctx, bpg::launch_data_f8(pl->Y32.p, NS, N, Vpad, KS, pl->Yf8.p, pl->Ysc.p, st));
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
-    // while the transform runs: stage (and check) the permutations
+    // while the transform runs: stage (and check) the permutations, then
+    // upload them on the copy stream (the staging is free: the last phase's
+    // upload finished before its operand build, which its final stream
+    // synchronisation waited for)
     if (m.sign_operand) {
         std::memcpy(pl->h_sign.p, sign, (size_t)K * N * sizeof(float));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p, pl->h_sign.p, (size_t)K * N * sizeof(float), hipMemcpyHostToDevice,
+                                    pl->copy));
     } else {
         unsigned short* hp = pl->h_perm.p;
         unsigned short worst = 0;
@@ -598,13 +624,11 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             (void)hipStreamSynchronize(st);  // the staged model copies must finish
             BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
         }
+        BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, hp, total * sizeof(unsigned short), hipMemcpyHostToDevice, pl->copy));
     }
+    BPG_HIP(ctx, hipEventRecord(pl->copied, pl->copy));
+    BPG_HIP(ctx, hipStreamWaitEvent(st, pl->copied, 0));
     {
-        if (m.sign_operand)
-            BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p, pl->h_sign.p, (size_t)K * N * sizeof(float), hipMemcpyHostToDevice, st));
-        else
-            BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, pl->h_perm.p, (size_t)K * N * sizeof(unsigned short),
-                                        hipMemcpyHostToDevice, st));
         bpg::OperandArgs oa{};
         oa.perm = m.sign_operand ? nullptr : pl->perm.p;
         oa.sign = m.sign_operand ? pl->sign.p : nullptr;
