@@ -67,7 +67,7 @@ class Timing(ctypes.Structure):
                 ("contract_launches", c_int), ("perm_voxels", c_long),
                 ("fit_columns", c_int), ("slots_per_perm", c_int), ("precision_used", c_int),
                 ("unsafe_waves", c_int), ("fp32_permutations", c_long), ("fp64_permutations", c_long),
-                ("cdt_tie_maps", c_long), ("pvalue_ties", c_long)]
+                ("cdt_tie_maps", c_long), ("pvalue_ties", c_long), ("fp64_samples", c_long)]
 
 
 class TuneResult(ctypes.Structure):

@@ -52,7 +52,9 @@ bpg::TuneKey tune_key(int KK, int PP, bool sign, bool ftest, bool out, int Vpad)
 // The split-BF16 grid: the candidate bpg_plan_tune has in force, else the
 // tuned entry of this bucket, else the planner's default.  The alternative
 // subtile count exists for the max-T-only kernels only.
-bpg::Bf16Launch bf16_launch(const bpg_ctx* ctx, long K, int PP, int KK, int Vpad, bool sign, bool ftest, bool out) {
+// (a contraction with a sample list runs the default subtile count: excl)
+bpg::Bf16Launch bf16_launch(const bpg_ctx* ctx, long K, int PP, int KK, int Vpad, bool sign, bool ftest, bool out,
+                            bool excl = false) {
     bpg::TuneEntry e;
     if (ctx->tune_forced) {
         e = ctx->tune_force;
@@ -60,7 +62,7 @@ bpg::Bf16Launch bf16_launch(const bpg_ctx* ctx, long K, int PP, int KK, int Vpad
         const auto it = ctx->tune.find(tune_key(KK, PP, sign, ftest, out, Vpad));
         if (it != ctx->tune.end()) e = it->second;
     }
-    const int variant = out ? 0 : e.variant;
+    const int variant = out || excl ? 0 : e.variant;
     const int slots = ctx->cus / 8 * bpg::contract_bf16_blocks_per_cu(KK, PP, sign, ftest, out, variant);
     bpg::Bf16Launch L = bpg::plan_contract_bf16((int)K, PP, KK, Vpad, sign, slots, variant, e.chunk_pow);
     if (out && L.n_vt > 32) {
@@ -113,6 +115,8 @@ struct bpg_plan {
         const float* sign = nullptr;
         float* stats = nullptr;
         int pfit = 0, PP = 0, launches = 0;
+        unsigned long long sample_cap = 0;  // the sample list's per-segment capacity (0: no list)
+        long fp64_samples = 0;              // samples recomputed from the list
         std::chrono::steady_clock::time_point t0;
     } pending;
 
@@ -137,6 +141,8 @@ struct bpg_plan {
     DBuf<int> nlev;
     DBuf<unsigned> unsafe;
     DBuf<unsigned char> flags;
+    DBuf<unsigned long long> samples, n_samples;
+    HBuf<unsigned long long> h_nsamples;
     DBuf<double> oX, oXp, oC, osc;
     DBuf<float> d_obs, sorted, P, glm_out;
     DBuf<unsigned> d_pties;
@@ -161,6 +167,7 @@ struct bpg_plan {
         Yb.release(); Gb.release(); Y32.release(); G32.release();
         perm.release(); sign.release(); partial.release(); maxout.release(); stats.release(); bits.release();
         cl.release(); maxsize.release(); labels.release(); tfce.release(); sqrt_size.release(); nlev.release(); unsafe.release(); flags.release();
+        samples.release(); n_samples.release(); h_nsamples.release();
         oX.release(); oXp.release(); oC.release(); osc.release();
         d_obs.release(); sorted.release(); P.release(); glm_out.release(); d_pties.release(); mdev.release();
     }
@@ -435,6 +442,78 @@ extern "C" long bpg_plan_num_voxels(const bpg_plan* pl) { return pl ? pl->Vc : -
 
 namespace {
 
+// The regularised incomplete beta function I_x(a, b) from its continued
+// fraction (Abramowitz & Stegun 26.5.8), evaluated by the modified Lentz
+// method; on the side x > (a+1)/(a+b+2) through I_x(a,b) = 1 - I_{1-x}(b,a).
+double inc_beta(double x, double a, double b) {
+    if (!(x > 0.0)) return 0.0;
+    if (!(x < 1.0)) return 1.0;
+    const bool flip = x > (a + 1.0) / (a + b + 2.0);
+    if (flip) {
+        std::swap(a, b);
+        x = 1.0 - x;
+    }
+    const double front =
+        std::exp(std::lgamma(a + b) - std::lgamma(a) - std::lgamma(b) + a * std::log(x) + b * std::log1p(-x)) / a;
+    const double tiny = 1e-300;
+    auto clamp = [&](double v) { return std::fabs(v) < tiny ? tiny : v; };
+    double c = 1.0, d = 1.0 / clamp(1.0 - (a + b) * x / (a + 1.0)), h = d;
+    for (int m = 1; m <= 400; m++) {
+        const double m2 = 2.0 * m;
+        double t = m * (b - m) * x / ((a + m2 - 1.0) * (a + m2));
+        d = 1.0 / clamp(1.0 + t * d);
+        c = clamp(1.0 + t / c);
+        h *= d * c;
+        t = -(a + m) * (a + b + m) * x / ((a + m2) * (a + m2 + 1.0));
+        d = 1.0 / clamp(1.0 + t * d);
+        c = clamp(1.0 + t / c);
+        const double step = d * c;
+        h *= step;
+        if (std::fabs(step - 1.0) < 1e-13) break;
+    }
+    const double r = front * h;
+    return flip ? 1.0 - r : r;
+}
+
+// The chance that a permutation of a study meets an ill-conditioned sample
+// (SSE < tau |y*|^2) at one of its V voxels, were the data Gaussian noise:
+// SSE / |y*|^2 is then Beta(dof / 2, pfit / 2) at every voxel.  Orders the
+// batch's work and chooses the sample list; the results meet the accuracy
+// gate whatever it says.
+double ill_conditioned_chance(double dof, int pfit, long V, double tau) {
+    if (dof <= 0.0 || pfit <= 0 || V <= 0) return 0.0;
+    const double pv = inc_beta(tau, 0.5 * dof, 0.5 * pfit);
+    if (pv >= 1.0) return 1.0;
+    return -std::expm1((double)V * std::log1p(-pv));
+}
+
+
+// The per-segment capacity of the sample list of a split-BF16 max-T-only
+// phase (its ill-conditioned samples recomputed one by one in FP64), or 0
+// for none: a list pays when a permutation is likely to meet such a sample
+// (a chance of 1% or more; below that the few flagged permutations are
+// repeated whole) and the expected count of them, K V pv for Gaussian noise,
+// is modest.  The BPG_SAMPLE_SEGS segments hold 8 times that count, at
+// least 256 and at most 32 Ki entries each (64 MiB in all); samples past
+// their segment's end flag their permutations for a whole repeat.
+// Environment: BPG_SAMPLE_LIST=off|on forces the list off or on (on: where
+// the kernels support it), BPG_SAMPLE_CAP sets the per-segment capacity.
+unsigned long long sample_list_cap(double dof, int pfit, long V, long K, double tau) {
+    const char* mode = std::getenv("BPG_SAMPLE_LIST");
+    const char* capenv = std::getenv("BPG_SAMPLE_CAP");
+    const unsigned long long forced = capenv ? std::strtoull(capenv, nullptr, 10) : 0ull;
+    if (mode && std::strcmp(mode, "off") == 0) return 0ull;
+    const bool on = mode && std::strcmp(mode, "on") == 0;
+    if (dof <= 0.0 || pfit <= 0 || V <= 0 || K <= 0) return 0ull;
+    const double pv = inc_beta(tau, 0.5 * dof, 0.5 * pfit);
+    const double expected = pv * (double)V * (double)K;
+    const bool pays = ill_conditioned_chance(dof, pfit, V, tau) >= 0.01 && expected <= (double)(1 << 20);
+    if (!on && !pays) return 0ull;
+    if (forced) return forced;
+    const double want = std::min(8.0 * expected / bpg::BPG_SAMPLE_SEGS, 32768.0);
+    return std::max<unsigned long long>(256ull, (unsigned long long)want);
+}
+
 }  // namespace
 
 namespace {
@@ -501,8 +580,15 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const int NS = f32_rows(N);
     const int PP = bf16_slots(pfit);
     const int Vc = pl->Vc, Vpad = pl->Vpad;
+    // split-BF16 max-T of a t test whose permutations are likely to meet
+    // ill-conditioned samples: those samples go to a list for FP64
+    const unsigned long long scap =
+        bf && !want_stats && !cluster && !tfce && !m.sign_operand && d.test == BPG_TTEST && npy == 2 &&
+                KK <= bpg::BPG_EXCL_MAX_KK && pfit <= bpg::BPG_FIX_MAX_P && m.q <= 24
+            ? sample_list_cap((double)m.dof, pfit, Vc, K, bpg::BPG_TAU_BF16)
+            : 0ull;
     const bpg::Bf16Launch Lb =
-        bf ? bf16_launch(ctx, K, PP, KK, Vpad, m.sign_operand, d.test == BPG_FTEST, want_stats || cluster)
+        bf ? bf16_launch(ctx, K, PP, KK, Vpad, m.sign_operand, d.test == BPG_FTEST, want_stats || cluster, scap > 0)
            : bpg::Bf16Launch{};
     // FP32 on the matrix cores up to 16 fit columns, else on the vector ALUs
     const bool mx = form == BPG_PREC_FP32 && pfit <= 16;
@@ -569,6 +655,11 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (cluster) {
         BPG_HIP(ctx, pl->ties.ensure(K));
         BPG_HIP(ctx, pl->h_ties.ensure(K));
+    }
+    if (scap) {
+        BPG_HIP(ctx, pl->samples.ensure(scap * bpg::BPG_SAMPLE_SEGS));
+        BPG_HIP(ctx, pl->n_samples.ensure(bpg::BPG_SAMPLE_SEGS));
+        BPG_HIP(ctx, pl->h_nsamples.ensure(bpg::BPG_SAMPLE_SEGS));
     }
     if (want_stats) BPG_HIP(ctx, pl->stats.ensure((size_t)K * std::max(Vc, 1)));
     if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
@@ -684,9 +775,14 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.cdt_lo = d.cdt - band;
         ca.cdt_hi = d.cdt + band;
         ca.ties = cluster && !f64 ? pl->ties.p : nullptr;
+        ca.samples = scap ? pl->samples.p : nullptr;
+        ca.n_samples = scap ? pl->n_samples.p : nullptr;
+        ca.sample_cap = scap;
         BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));
         BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
         if (cluster) BPG_HIP(ctx, hipMemsetAsync(pl->ties.p, 0, K, st));
+        if (scap)
+            BPG_HIP(ctx, hipMemsetAsync(pl->n_samples.p, 0, bpg::BPG_SAMPLE_SEGS * sizeof(unsigned long long), st));
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
         else if (f64) BPG_HIP(ctx, bpg::launch_contract_f64(ca, Lf, st));
         else if (mx) BPG_HIP(ctx, bpg::launch_contract_f32mx(ca, Lm, PP, KG, st));
@@ -694,6 +790,17 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         else BPG_HIP(ctx, bpg::launch_contract_f32(ca, Lf, st));
         launches = 1;
         if (!cluster) BPG_HIP(ctx, bpg::launch_reduce_max(pl->partial.p, n_part, (int)K, pl->maxout.p, st));
+        if (scap) {
+            // the listed samples, recomputed in FP64, into the maxima
+            bpg::SampleFixArgs fa{};
+            fa.samples = pl->samples.p; fa.n_samples = pl->n_samples.p; fa.cap = scap;
+            fa.Yc = pl->Yc.p; fa.Qn = pl->Qn.p; fa.QB = std::max(QB, 0); fa.q = m.q;
+            fa.Qt = pl->Qt.p; fa.perm = pl->perm.p;
+            fa.N = N; fa.pfit = pfit; fa.Vc = Vc; fa.K = (int)K;
+            fa.dof = (double)m.dof;
+            fa.maxout = pl->maxout.p;
+            BPG_HIP(ctx, bpg::launch_fix_samples(fa, st));
+        }
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[3], st));
     // both results are 4-byte words: the cluster maxima are uint sizes /
@@ -778,6 +885,9 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipMemcpyAsync(pl->h_unsafe.p, pl->unsafe.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     if (!f64) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
+    if (scap)
+        BPG_HIP(ctx, hipMemcpyAsync(pl->h_nsamples.p, pl->n_samples.p,
+                                    bpg::BPG_SAMPLE_SEGS * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     if (cluster) BPG_HIP(ctx, hipMemcpyAsync(pl->h_ties.p, pl->ties.p, K, hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipEventRecord(pl->ev[5], st));
     auto& ps = pl->pending;
@@ -799,6 +909,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     ps.mx = mx || f8;
     ps.PP = PP;
     ps.launches = launches;
+    ps.sample_cap = scap;
+    ps.fp64_samples = 0;
     ps.t0 = t0;
     return BPG_OK;
 }
@@ -862,6 +974,9 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     if (!ps.active) BPG_FAIL(ctx, BPG_ERR_ARG, "no permutation phase in flight");
     ps.active = false;
     BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
+    if (ps.sample_cap)
+        for (int g = 0; g < bpg::BPG_SAMPLE_SEGS; g++)
+            ps.fp64_samples += (long)std::min(pl->h_nsamples.p[g], ps.sample_cap);
     const int unsafe_first = ps.form != BPG_PREC_FP64 ? (int)*pl->h_unsafe.p : 0;
     if (unsafe_first > 0 && pl->Vc > 0) {
         // Ill-conditioned samples: SSE = |y*|^2 - |z|^2 is too small a
@@ -941,6 +1056,7 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
     tm.unsafe_waves = unsafe_first;
     tm.fp32_permutations = ps.fp32_perms;
     tm.fp64_permutations = ps.fp64_perms;
+    tm.fp64_samples = ps.fp64_samples;
     tm.cdt_tie_maps = cluster ? ties : 0;
     return BPG_OK;
 }
@@ -1463,53 +1579,6 @@ extern "C" int bpg_tune_entries(const bpg_ctx* ctx) { return ctx ? (int)ctx->tun
 // ---------------------------------------------------------------------------
 // batches of studies: one persistent contraction per class, two hipGraphs
 // ---------------------------------------------------------------------------
-namespace {
-
-// The regularised incomplete beta function I_x(a, b) from its continued
-// fraction (Abramowitz & Stegun 26.5.8), evaluated by the modified Lentz
-// method; on the side x > (a+1)/(a+b+2) through I_x(a,b) = 1 - I_{1-x}(b,a).
-double inc_beta(double x, double a, double b) {
-    if (!(x > 0.0)) return 0.0;
-    if (!(x < 1.0)) return 1.0;
-    const bool flip = x > (a + 1.0) / (a + b + 2.0);
-    if (flip) {
-        std::swap(a, b);
-        x = 1.0 - x;
-    }
-    const double front =
-        std::exp(std::lgamma(a + b) - std::lgamma(a) - std::lgamma(b) + a * std::log(x) + b * std::log1p(-x)) / a;
-    const double tiny = 1e-300;
-    auto clamp = [&](double v) { return std::fabs(v) < tiny ? tiny : v; };
-    double c = 1.0, d = 1.0 / clamp(1.0 - (a + b) * x / (a + 1.0)), h = d;
-    for (int m = 1; m <= 400; m++) {
-        const double m2 = 2.0 * m;
-        double t = m * (b - m) * x / ((a + m2 - 1.0) * (a + m2));
-        d = 1.0 / clamp(1.0 + t * d);
-        c = clamp(1.0 + t / c);
-        h *= d * c;
-        t = -(a + m) * (a + b + m) * x / ((a + m2) * (a + m2 + 1.0));
-        d = 1.0 / clamp(1.0 + t * d);
-        c = clamp(1.0 + t / c);
-        const double step = d * c;
-        h *= step;
-        if (std::fabs(step - 1.0) < 1e-13) break;
-    }
-    const double r = front * h;
-    return flip ? 1.0 - r : r;
-}
-
-// The chance that a permutation of a study meets an ill-conditioned sample
-// (SSE < tau |y*|^2) at one of its V voxels, were the data Gaussian noise:
-// SSE / |y*|^2 is then Beta(dof / 2, pfit / 2) at every voxel.  Orders the
-// batch's work only; results never depend on it.
-double ill_conditioned_chance(double dof, int pfit, long V, double tau) {
-    if (dof <= 0.0 || pfit <= 0 || V <= 0) return 0.0;
-    const double pv = inc_beta(tau, 0.5 * dof, 0.5 * pfit);
-    if (pv >= 1.0) return 1.0;
-    return -std::expm1((double)V * std::log1p(-pv));
-}
-
-}  // namespace
 
 struct bpg_batch {
     bpg_ctx* ctx = nullptr;
@@ -1520,6 +1589,12 @@ struct bpg_batch {
         int N = 0, KK = 0, PP = 0, npy = 2;
         int stage = 1;        // 0: likely to meet ill-conditioned samples (runs first)
         bool sign = false;
+        // a sample list (its ill-conditioned samples recomputed in FP64):
+        // its per-segment capacity, its entries, its segment counters (at
+        // counters() + BPG_SAMPLE_SEGS * excl_slot)
+        unsigned long long cap = 0;
+        DBuf<unsigned long long> samples;
+        int excl_slot = -1;
         bpg::Bf16Launch L{};
         DBuf<double> yy;
         DBuf<unsigned> Yb, Gb;
@@ -1536,7 +1611,7 @@ struct bpg_batch {
     };
     std::vector<Study> st;
     struct Class {
-        int KK, PP, sign, variant, stage;
+        int KK, PP, sign, variant, stage, excl;
         std::vector<int> members;
         int n_items[bpg::BPG_XCDS] = {};  // per XCD queue
         int grid = 0;
@@ -1545,10 +1620,11 @@ struct bpg_batch {
     std::vector<Class> cls;
     DBuf<bpg::BatchStudy> table;
     DBuf<bpg::ReduceJob> jobs;
-    // one word buffer cleared by one fill: queue heads (8 per class), the
-    // studies' ill-conditioned counts, then their flag bytes
+    // one word buffer cleared by one fill: the sample lists' segment counters
+    // (64-bit, two words each), queue heads (8 per class), the studies'
+    // ill-conditioned counts, then their flag bytes
     DBuf<unsigned> words;
-    size_t n_heads = 0, flag_bytes = 0;
+    size_t n_heads = 0, flag_bytes = 0, n_excl = 0;
     DBuf<unsigned short> perm_all;
     DBuf<float> sign_all, max_all;
     size_t n_perm = 0, n_sign = 0, n_max = 0;
@@ -1569,9 +1645,10 @@ struct bpg_batch {
     double last_ms = 0.0, last_graph_ms = 0.0;
     int last_fallback = 0;
 
-    unsigned* heads() const { return words.p; }
-    unsigned* unsafe() const { return words.p + n_heads; }
-    unsigned char* flags() const { return reinterpret_cast<unsigned char*>(words.p + n_heads + st.size()); }
+    unsigned long long* counters() const { return reinterpret_cast<unsigned long long*>(words.p); }
+    unsigned* heads() const { return words.p + 2 * bpg::BPG_SAMPLE_SEGS * n_excl; }
+    unsigned* unsafe() const { return heads() + n_heads; }
+    unsigned char* flags() const { return reinterpret_cast<unsigned char*>(unsafe() + st.size()); }
 
     ~bpg_batch() {
         for (int g = 0; g < 2; g++) {
@@ -1586,7 +1663,7 @@ struct bpg_batch {
         if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto& s : st) {
-            s.yy.release(); s.Yb.release(); s.Gb.release(); s.partial.release();
+            s.yy.release(); s.Yb.release(); s.Gb.release(); s.partial.release(); s.samples.release();
         }
         table.release(); jobs.release(); words.release(); perm_all.release(); sign_all.release(); max_all.release();
         h_perm.release(); h_sign.release(); h_max.release(); h_unsafe.release(); h_flags.release();
@@ -1637,12 +1714,23 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
         s.sign = m.sign_operand;
         s.npy = bpg::bf16_y_pieces(s.KK, s.sign);
         const double tau = s.npy == 2 ? bpg::BPG_TAU_BF16 : bpg::BPG_TAU_FP32;
-        s.stage = ill_conditioned_chance((double)m.dof, m.pfit, pl->Vc, tau) >= 1e-3 ? 0 : 1;
+        // as in a single-study phase (phase_enqueue): a sample list where
+        // permutations are likely to meet ill-conditioned samples
+        if (!s.sign && pl->d.test == BPG_TTEST && s.npy == 2 && s.KK <= bpg::BPG_EXCL_MAX_KK &&
+            m.pfit <= bpg::BPG_FIX_MAX_P && m.q <= 24 && pl->Vc > 0)
+            s.cap = sample_list_cap((double)m.dof, m.pfit, pl->Vc, s.K, tau);
+        // studies likely to repeat permutations run first (a study with a
+        // sample list repeats only what overflows it)
+        s.stage = !s.cap && ill_conditioned_chance((double)m.dof, m.pfit, pl->Vc, tau) >= 1e-3 ? 0 : 1;
         s.in_off = s.sign ? b->n_sign : b->n_perm;
         (s.sign ? b->n_sign : b->n_perm) += (size_t)s.K * s.N;
         b->max_K = std::max<int>(b->max_K, (int)s.K);
         if (pl->Vc == 0) continue;
-        s.L = bf16_launch(ctx, s.K, s.PP, s.KK, pl->Vpad, s.sign, false, false);
+        if (s.cap) {
+            s.excl_slot = (int)b->n_excl++;
+            BPG_BHIP(s.samples.ensure(s.cap * bpg::BPG_SAMPLE_SEGS));
+        }
+        s.L = bf16_launch(ctx, s.K, s.PP, s.KK, pl->Vpad, s.sign, false, false, s.cap > 0);
         BPG_BHIP(s.yy.ensure(pl->Vpad));
         BPG_BHIP(s.Yb.ensure((size_t)(pl->Vpad / 32) * s.KK * s.npy * 64 * 4));
         BPG_BHIP(s.Gb.ensure((size_t)s.L.n_sub * s.KK * 2 * 64 * 4));
@@ -1668,18 +1756,19 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
         const auto& s = b->st[i];
         if (s.plan->Vc == 0) continue;
         bpg_batch::Class* c = nullptr;
+        const int excl = s.cap > 0;
         for (auto& x : b->cls)
             if (x.KK == s.KK && x.PP == s.PP && x.sign == (int)s.sign && x.variant == s.L.variant &&
-                x.stage == s.stage)
+                x.stage == s.stage && x.excl == excl)
                 c = &x;
         if (!c) {
-            b->cls.push_back(bpg_batch::Class{s.KK, s.PP, (int)s.sign, s.L.variant, s.stage, {}, {}, 0, 0});
+            b->cls.push_back(bpg_batch::Class{s.KK, s.PP, (int)s.sign, s.L.variant, s.stage, excl, {}, {}, 0, 0});
             c = &b->cls.back();
         }
         c->members.push_back(i);
     }
     b->n_heads = b->cls.size() * bpg::BPG_XCDS;
-    const size_t n_words = b->n_heads + (size_t)n + b->flag_bytes / 4;
+    const size_t n_words = 2 * bpg::BPG_SAMPLE_SEGS * b->n_excl + b->n_heads + (size_t)n + b->flag_bytes / 4;
     BPG_BHIP(b->words.ensure(n_words));
     BPG_BHIP(b->perm_all.ensure(std::max<size_t>(b->n_perm, 1)));
     BPG_BHIP(b->sign_all.ensure(std::max<size_t>(b->n_sign, 1)));
@@ -1711,6 +1800,11 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
             a.unsafe = b->unsafe() + s.slot;
             a.flags = b->flags() + s.flag_off;
             a.cdt_lo = a.cdt_hi = pl->d.cdt;
+            if (s.cap) {
+                a.samples = s.samples.p;
+                a.n_samples = b->counters() + (size_t)bpg::BPG_SAMPLE_SEGS * s.excl_slot;
+                a.sample_cap = s.cap;
+            }
             bs.L = s.L;
             for (int x = 0; x < bpg::BPG_XCDS; x++) {
                 // voxel chunks x, x + 8, ... of this study go to queue x
@@ -1722,8 +1816,8 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
         }
         int occ = 0, total = 0;
         for (int x = 0; x < bpg::BPG_XCDS; x++) total += c.n_items[x];
-        BPG_BHIP(bpg::launch_contract_bf16_persist(bpg::BatchClass{}, c.KK, c.PP, c.sign != 0, c.variant, 0, nullptr,
-                                                   &occ));
+        BPG_BHIP(bpg::launch_contract_bf16_persist(bpg::BatchClass{}, c.KK, c.PP, c.sign != 0, c.variant, c.excl != 0,
+                                                   0, nullptr, &occ));
         c.grid = std::max(1, std::min(total, ctx->cus * std::max(occ, 1)));
     }
     // the reduction jobs stage by stage
@@ -1827,12 +1921,30 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
             bc.n_studies = (int)c.members.size();
             std::copy(c.n_items, c.n_items + bpg::BPG_XCDS, bc.n_items);
             bc.counter = b->heads() + (size_t)ci * bpg::BPG_XCDS;
-            cap(bpg::launch_contract_bf16_persist(bc, c.KK, c.PP, c.sign != 0, c.variant, c.grid, bs));
+            cap(bpg::launch_contract_bf16_persist(bc, c.KK, c.PP, c.sign != 0, c.variant, c.excl != 0, c.grid, bs));
         }
         cap(hipEventRecord(b->join[g], b->side));
         cap(hipStreamWaitEvent(cs, b->join[g], 0));
         if (!empty) {
             cap(bpg::launch_reduce_max_jobs(b->jobs.p + j0, b->job_end[g] - j0, b->max_K, cs));
+            // the stage's listed samples, recomputed in FP64, into its maxima
+            for (auto& s : b->st) {
+                if (s.stage != g || !s.cap || s.plan->Vc == 0) continue;
+                const bpg::MapModel& m = s.plan->models[s.map];
+                bpg::SampleFixArgs fa{};
+                fa.samples = s.samples.p;
+                fa.n_samples = b->counters() + (size_t)bpg::BPG_SAMPLE_SEGS * s.excl_slot;
+                fa.cap = s.cap;
+                fa.Yc = s.plan->Yc.p;
+                fa.Qn = s.plan->mdev.p + s.plan->mdev_off[s.map] + m.Qt.size();
+                fa.QB = std::max(bpg::transform_width(m.q), 0); fa.q = m.q;
+                fa.Qt = s.plan->mdev.p + s.plan->mdev_off[s.map];
+                fa.perm = b->perm_all.p + s.in_off;
+                fa.N = s.N; fa.pfit = m.pfit; fa.Vc = s.plan->Vc; fa.K = (int)s.K;
+                fa.dof = (double)m.dof;
+                fa.maxout = b->max_all.p + s.max_off;
+                cap(bpg::launch_fix_samples(fa, cs));
+            }
             cap(hipMemcpyAsync(b->h_max.p + m0, b->max_all.p + m0, (b->max_end[g] - m0) * sizeof(float),
                                hipMemcpyDeviceToHost, cs));
             cap(hipMemcpyAsync(b->h_unsafe.p + s0, b->unsafe() + s0, (size_t)(b->slot_end[g] - s0) * sizeof(unsigned),

@@ -38,6 +38,7 @@ typedef const bf16x8 __attribute__((address_space(1))) gbf16x8;
 typedef const double __attribute__((address_space(1))) gdouble;
 typedef float __attribute__((address_space(1))) gfloat;
 typedef unsigned char __attribute__((address_space(1))) gbyte;
+typedef unsigned long long __attribute__((address_space(1))) gu64;
 
 // v_writelane_b32: old with lane `lane` replaced by the uniform `value` (the
 // LLVM intrinsic; this compiler exposes no builtin for it)
@@ -92,6 +93,55 @@ struct TileSpan {
 // keeps the bitmask word of its permutation in wb, and every 32 tiles (and
 // at the item's last tile) the half-wave stores its 32 words as one 128-byte
 // run of the permutation's row.
+// OUT = 3 (max only, with a sample list): an ill-conditioned sample's
+// statistic becomes NaN, which the running max drops, and the sample is
+// appended to the list (append_samples) for its FP64 recomputation.
+//
+// Appends the samples of this tile whose bit P is set in excl: one atomic
+// add per wave reserves the entries of all its samples in the wave's
+// segment, and each lane writes its own in (slot P, lane) order.  Past the
+// segment's capacity, the permutation is flagged for a whole repeat instead.
+// (a call, not inlined: the rare path then holds no registers in the
+// contraction loop around it)
+template <int PER>
+__device__ __attribute__((noinline)) void append_samples(unsigned long long* samples, unsigned long long* n_samples,
+                                                         unsigned long long cap, unsigned char* flags,
+                                                         unsigned* unsafe, int K, unsigned excl, int st, int vt,
+                                                         int lane) {
+    const int hl = lane >> 5;
+    const unsigned v = (unsigned)(vt * 32 + (lane & 31));
+    const int seg = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (unsigned)(BPG_SAMPLE_SEGS - 1));
+    unsigned long long m[PER];
+    unsigned total = 0u;
+#pragma unroll
+    for (int P = 0; P < PER; P++) {
+        m[P] = __ballot(((excl >> P) & 1u) != 0u);
+        total += (unsigned)__popcll(m[P]);
+    }
+    const int leader = __builtin_ctzll(__ballot(excl != 0u));
+    unsigned long long base = 0ull;
+    if (lane == leader) base = atomicAdd(n_samples + seg, (unsigned long long)total);
+    base = __shfl(base, leader, 64);
+    gu64* out = (gu64*)(samples + (size_t)seg * cap);
+    unsigned long long off = 0ull;
+#pragma unroll
+    for (int P = 0; P < PER; P++) {
+        const unsigned rank =
+            __builtin_amdgcn_mbcnt_hi((unsigned)(m[P] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m[P], 0u));
+        const int k = st * (2 * PER) + hl * PER + P;
+        if (((excl >> P) & 1u) != 0u && k < K) {
+            const unsigned long long idx = base + off + rank;
+            if (idx < cap) {
+                out[idx] = ((unsigned long long)k << 32) | v;
+            } else {
+                ((gbyte*)flags)[k] = 1;
+                atomicAdd(unsafe, 1u);
+            }
+        }
+        off += (unsigned long long)__popcll(m[P]);
+    }
+}
+
 template <int PP, bool FT, int OUT>
 __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, float yhi, float ylo, int vt,
                                     int lane, int st, float (&rm)[16 / PP], bool& bad, unsigned (&wb)[16 / PP],
@@ -99,6 +149,7 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
     constexpr int PER = 16 / PP;
     const float tyy = a.tau * yhi;  // NaN on padding: never flagged
     float u[PER];
+    unsigned excl = 0u;  // OUT = 3: bit P for an ill-conditioned sample P
 #pragma unroll
     for (int P = 0; P < PER; P++) {
         const float z0 = acc[P * PP];
@@ -125,12 +176,23 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
             }
             sse = d + ylo;
         }
-        bad |= sse < tyy;
+        const bool ill = sse < tyy;
         u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
+        if constexpr (OUT == 3) {
+            u[P] = ill ? __builtin_nanf("") : u[P];
+            excl |= (unsigned)ill << P;
+        } else {
+            bad |= ill;
+        }
         // max(u, rm) as med3(u, rm, +inf): one v_med3_f32, where fmaxf costs
         // a second instruction to quiet the loop-carried rm; a NaN u (padding
         // voxel) yields rm, as fmaxf does (checked on the device)
         if constexpr (OUT != 2) rm[P] = __builtin_amdgcn_fmed3f(u[P], rm[P], a.pinf);
+    }
+    if constexpr (OUT == 3) {
+        // rare: a uniform branch around the list appends
+        if (__ballot(excl != 0u) != 0ull)
+            append_samples<PER>(a.samples, a.n_samples, a.sample_cap, a.flags, a.unsafe, a.K, excl, st, vt, lane);
     }
     if constexpr (OUT == 1) {
         const int hl = lane >> 5;
@@ -412,22 +474,25 @@ __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16L
     // flagged.  Rare: a wave that met an ill-conditioned sample adds one to
     // the counter and flags the permutations of each (subtile, half-wave)
     // where it happened (of every resident subtile in the pipelined form).
-    bool any = false;
+    // With a sample list (OUT = 3) the epilogue has listed them instead.
+    if constexpr (OUT != 3) {
+        bool any = false;
 #pragma unroll
-    for (int b = 0; b < NB; b++) {
-        const unsigned long long m = __ballot(bad[b]);
-        any |= m != 0ull;
-        if (m && (lane & 31) == 0) {
-            const int hl = lane >> 5;
-            if ((hl ? (m >> 32) : (m & 0xFFFFFFFFull)) != 0ull)
-                for (int r = (PIPE ? 0 : b); r < (PIPE ? R : b + 1); r++)
-                    for (int P = 0; P < PER; P++) {
-                        const int k = (sub0 + r) * (2 * PER) + hl * PER + P;
-                        if (k < a.K) ((gbyte*)a.flags)[k] = 1;
-                    }
+        for (int b = 0; b < NB; b++) {
+            const unsigned long long m = __ballot(bad[b]);
+            any |= m != 0ull;
+            if (m && (lane & 31) == 0) {
+                const int hl = lane >> 5;
+                if ((hl ? (m >> 32) : (m & 0xFFFFFFFFull)) != 0ull)
+                    for (int r = (PIPE ? 0 : b); r < (PIPE ? R : b + 1); r++)
+                        for (int P = 0; P < PER; P++) {
+                            const int k = (sub0 + r) * (2 * PER) + hl * PER + P;
+                            if (k < a.K) ((gbyte*)a.flags)[k] = 1;
+                        }
+            }
         }
+        if (any && lane == 0) atomicAdd(a.unsafe, 1u);
     }
-    if (any && lane == 0) atomicAdd(a.unsafe, 1u);
 
     if constexpr (OUT == 2) return;  // cluster extent: no maxima
     const int hl = lane >> 5;
@@ -476,7 +541,7 @@ constexpr int persist_min_blocks() {
     return (!PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK) ? (R >= 3 ? 1 : 2) : 1;
 }
 
-template <int KK, int R, int PP, bool SIGN, bool PIPE>
+template <int KK, int R, int PP, bool SIGN, bool PIPE, int OUT>
 __global__ __launch_bounds__(256, (persist_min_blocks<KK, R, SIGN, PIPE>())) void k_contract_bf16_persist(BatchClass c) {
     __shared__ int s_item;
     // s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4): the XCD of this block
@@ -497,8 +562,8 @@ __global__ __launch_bounds__(256, (persist_min_blocks<KK, R, SIGN, PIPE>())) voi
             const ContractArgs a = c.studies[j].a;
             const Bf16Launch L = c.studies[j].L;
             const int local = item - c.studies[j].item0[x];
-            contract_item<KK, R, PP, SIGN, false, false, PIPE>(a, L, local % L.n_mgroups,
-                                                               x + BPG_XCDS * (local / L.n_mgroups));
+            contract_item<KK, R, PP, SIGN, false, OUT, PIPE>(a, L, local % L.n_mgroups,
+                                                             x + BPG_XCDS * (local / L.n_mgroups));
         }
     }
 }
@@ -517,9 +582,16 @@ template <int KK, int R, int PP, bool SIGN, bool FT, bool PIPE>
 hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s, int* occ) {
     constexpr bool ROLL = !PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK;
     // 0: maxima only; 1: statistics and / or bitmask words; 2: the bitmask
-    // (and ties) alone, in 128-byte runs
-    const int mode = occ ? *occ : (a.bits && a.ties && !a.stats) ? 2 : (a.stats || a.bits) ? 1 : 0;
-    void (*kern)(ContractArgs, Bf16Launch);
+    // (and ties) alone, in 128-byte runs; 3: maxima only, with a sample list
+    // (t statistics, permutation operands, KK <= BPG_EXCL_MAX_KK)
+    const int mode = occ ? *occ
+                         : (a.bits && a.ties && !a.stats) ? 2
+                         : (a.stats || a.bits)            ? 1
+                         : a.samples                      ? 3
+                                                          : 0;
+    constexpr bool EXCL = !SIGN && !FT && KK <= BPG_EXCL_MAX_KK && !ROLL;
+    if (mode == 3 && !EXCL) return hipErrorInvalidValue;
+    void (*kern)(ContractArgs, Bf16Launch) = nullptr;
     if constexpr (ROLL)
         kern = mode == 2   ? k_contract_bf16_roll<KK, R, PP, SIGN, FT, 2, PIPE>
                : mode == 1 ? k_contract_bf16_roll<KK, R, PP, SIGN, FT, 1, PIPE>
@@ -528,6 +600,8 @@ hipError_t run_bf16(const ContractArgs& a, const Bf16Launch& L, hipStream_t s, i
         kern = mode == 2   ? k_contract_bf16<KK, R, PP, SIGN, FT, 2, PIPE>
                : mode == 1 ? k_contract_bf16<KK, R, PP, SIGN, FT, 1, PIPE>
                            : k_contract_bf16<KK, R, PP, SIGN, FT, 0, PIPE>;
+    if constexpr (EXCL)
+        if (mode == 3) kern = k_contract_bf16<KK, R, PP, SIGN, FT, 3, PIPE>;
     if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, reinterpret_cast<const void*>(kern), 256, 0);
     hipLaunchKernelGGL(kern, dim3(L.per_xcd * 8), dim3(256), 0, s, a, L);
     return hipGetLastError();
@@ -543,7 +617,7 @@ hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool s
     constexpr bool ALT = KK <= 8;
     constexpr int RSA = bf16_rows_per_wave_alt(KK, true);
     constexpr int RA = bf16_rows_per_wave_alt(KK, false);
-    const bool out = occ ? *occ != 0 : (a.stats || a.bits);
+    const bool out = occ ? *occ != 0 : (a.stats || a.bits || a.samples);
     const bool alt = ALT && L.variant == 1 && !out;
     if (sign) {
         if (a.ftest || PP != 1) return hipErrorInvalidValue;
@@ -562,44 +636,52 @@ hipError_t dispatch_bf16_kind(const ContractArgs& a, const Bf16Launch& L, bool s
 }
 
 template <int KK, int R, int PP, bool SIGN>
-hipError_t run_bf16_persist(const BatchClass& c, int grid, hipStream_t s, int* occ) {
+hipError_t run_bf16_persist(const BatchClass& c, bool excl, int grid, hipStream_t s, int* occ) {
     constexpr bool PIPE = bf16_pipelined(KK, SIGN);
-    void (*kern)(BatchClass) = k_contract_bf16_persist<KK, R, PP, SIGN, PIPE>;
+    constexpr bool ROLL = !PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK;
+    constexpr bool EXCL = !SIGN && KK <= BPG_EXCL_MAX_KK && !ROLL;
+    if (excl && !EXCL) return hipErrorInvalidValue;
+    void (*kern)(BatchClass) = k_contract_bf16_persist<KK, R, PP, SIGN, PIPE, 0>;
+    if constexpr (EXCL)
+        if (excl) kern = k_contract_bf16_persist<KK, R, PP, SIGN, PIPE, 3>;
     if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, reinterpret_cast<const void*>(kern), 256, 0);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, c);
     return hipGetLastError();
 }
 
 template <int KK, int PP>
-hipError_t dispatch_persist_kind(const BatchClass& c, bool sign, int variant, int grid, hipStream_t s, int* occ) {
+hipError_t dispatch_persist_kind(const BatchClass& c, bool sign, int variant, bool excl, int grid, hipStream_t s,
+                                 int* occ) {
     constexpr int RS = bf16_rows_per_wave(KK, true), RSA = bf16_rows_per_wave_alt(KK, true);
     constexpr int R = bf16_rows_per_wave(KK, false), RA = bf16_rows_per_wave_alt(KK, false);
     if (sign) {
+        if (excl) return hipErrorInvalidValue;
         if constexpr (PP == 1)
-            return variant ? run_bf16_persist<KK, RSA, 1, true>(c, grid, s, occ)
-                           : run_bf16_persist<KK, RS, 1, true>(c, grid, s, occ);
+            return variant ? run_bf16_persist<KK, RSA, 1, true>(c, false, grid, s, occ)
+                           : run_bf16_persist<KK, RS, 1, true>(c, false, grid, s, occ);
         return hipErrorInvalidValue;
     }
-    return variant ? run_bf16_persist<KK, RA, PP, false>(c, grid, s, occ)
-                   : run_bf16_persist<KK, R, PP, false>(c, grid, s, occ);
+    // a class with sample lists runs the default subtile count
+    return variant && !excl ? run_bf16_persist<KK, RA, PP, false>(c, false, grid, s, occ)
+                            : run_bf16_persist<KK, R, PP, false>(c, excl, grid, s, occ);
 }
 
 // The persistent contraction of a batch class (t statistics, max-T only),
 // compiled up to 8 k-steps (N <= 128).
 template <int KK>
-hipError_t dispatch_persist_pp(const BatchClass& c, int PP, bool sign, int variant, int grid, hipStream_t s,
-                               int* occ) {
+hipError_t dispatch_persist_pp(const BatchClass& c, int PP, bool sign, int variant, bool excl, int grid,
+                               hipStream_t s, int* occ) {
     if constexpr (KK > 8) {
         return hipErrorInvalidValue;
     } else {
         switch (PP) {
-            case 1: return dispatch_persist_kind<KK, 1>(c, sign, variant, grid, s, occ);
-            case 2: return dispatch_persist_kind<KK, 2>(c, sign, variant, grid, s, occ);
-            case 3: return dispatch_persist_kind<KK, 3>(c, sign, variant, grid, s, occ);
-            case 4: return dispatch_persist_kind<KK, 4>(c, sign, variant, grid, s, occ);
-            case 5: return dispatch_persist_kind<KK, 5>(c, sign, variant, grid, s, occ);
-            case 8: return dispatch_persist_kind<KK, 8>(c, sign, variant, grid, s, occ);
-            case 16: return dispatch_persist_kind<KK, 16>(c, sign, variant, grid, s, occ);
+            case 1: return dispatch_persist_kind<KK, 1>(c, sign, variant, excl, grid, s, occ);
+            case 2: return dispatch_persist_kind<KK, 2>(c, sign, variant, excl, grid, s, occ);
+            case 3: return dispatch_persist_kind<KK, 3>(c, sign, variant, excl, grid, s, occ);
+            case 4: return dispatch_persist_kind<KK, 4>(c, sign, variant, excl, grid, s, occ);
+            case 5: return dispatch_persist_kind<KK, 5>(c, sign, variant, excl, grid, s, occ);
+            case 8: return dispatch_persist_kind<KK, 8>(c, sign, variant, excl, grid, s, occ);
+            case 16: return dispatch_persist_kind<KK, 16>(c, sign, variant, excl, grid, s, occ);
         }
         return hipErrorInvalidValue;
     }

@@ -7,8 +7,8 @@ template hipError_t dispatch_bf16_pp<5>(const ContractArgs&, const Bf16Launch&, 
 template hipError_t dispatch_bf16_pp<6>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
 template hipError_t dispatch_bf16_pp<7>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
 template hipError_t dispatch_bf16_pp<8>(const ContractArgs&, const Bf16Launch&, int, bool, hipStream_t, int*);
-template hipError_t dispatch_persist_pp<5>(const BatchClass&, int, bool, int, int, hipStream_t, int*);
-template hipError_t dispatch_persist_pp<6>(const BatchClass&, int, bool, int, int, hipStream_t, int*);
-template hipError_t dispatch_persist_pp<7>(const BatchClass&, int, bool, int, int, hipStream_t, int*);
-template hipError_t dispatch_persist_pp<8>(const BatchClass&, int, bool, int, int, hipStream_t, int*);
+template hipError_t dispatch_persist_pp<5>(const BatchClass&, int, bool, int, bool, int, hipStream_t, int*);
+template hipError_t dispatch_persist_pp<6>(const BatchClass&, int, bool, int, bool, int, hipStream_t, int*);
+template hipError_t dispatch_persist_pp<7>(const BatchClass&, int, bool, int, bool, int, hipStream_t, int*);
+template hipError_t dispatch_persist_pp<8>(const BatchClass&, int, bool, int, bool, int, hipStream_t, int*);
 }  // namespace bpg

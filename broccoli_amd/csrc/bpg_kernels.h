@@ -122,7 +122,20 @@ struct ContractArgs {
     float cdt_lo, cdt_hi;
     unsigned char* ties;   // [K] or null
     double dof;            // FP64 form: N - p
+    // split-BF16 max-T only, with a sample list (samples non-null): an
+    // ill-conditioned sample is left out of the maximum and appended as
+    // (k << 32 | v) for k_fix_samples, which recomputes it in FP64.  The list
+    // has BPG_SAMPLE_SEGS segments of sample_cap entries, segment g at
+    // samples + g * sample_cap with its own counter n_samples[g] (64-bit: it
+    // never wraps), so that appends from different waves rarely meet on one
+    // counter; a wave appends to segment (4 blockIdx + wave) mod
+    // BPG_SAMPLE_SEGS.  A sample past its segment's capacity flags its
+    // permutation and adds one to *unsafe instead, as without the list.
+    unsigned long long* samples;
+    unsigned long long* n_samples;
+    unsigned long long sample_cap;
 };
+constexpr int BPG_SAMPLE_SEGS = 256;
 // SSE / |y*|^2 below which a sample is counted as ill-conditioned: the
 // split-BF16 z carries ~1e-5 relative error per unit of |z|^2 / SSE, the
 // FP32 z ~4e-7
@@ -183,10 +196,13 @@ struct BatchClass {
     int n_items[BPG_XCDS];
     unsigned* counter;  // BPG_XCDS queue heads
 };
-// persistent launch of a class (t statistics, max-T only, KK <= 8); with
-// occ set, stores the blocks per CU of the instantiation instead
-hipError_t launch_contract_bf16_persist(const BatchClass& c, int KK, int PP, bool sign, int variant, int grid,
-                                        hipStream_t s, int* occ = nullptr);
+// persistent launch of a class (t statistics, max-T only, KK <= 8; with
+// sample lists -- excl -- permutation operands up to KK = 4); with occ set,
+// stores the blocks per CU of the instantiation instead
+hipError_t launch_contract_bf16_persist(const BatchClass& c, int KK, int PP, bool sign, int variant, bool excl,
+                                        int grid, hipStream_t s, int* occ = nullptr);
+// the largest k-step count whose max-T-only contraction keeps a sample list
+constexpr int BPG_EXCL_MAX_KK = 4;
 // with occ set, stores the blocks per CU of the instantiation instead of
 // launching (see run_bf16)
 hipError_t launch_contract_bf16(const ContractArgs& a, const Bf16Launch& L, int KK, int PP,
@@ -258,6 +274,30 @@ struct ReduceJob {
     int n_part, K;
 };
 hipError_t launch_reduce_max_jobs(const ReduceJob* jobs, int n_jobs, int max_K, hipStream_t s);
+
+// The samples a split-BF16 max-T-only contraction left out (ContractArgs::
+// samples), recomputed in FP64 as the FP64 form computes them -- y* from
+// the compact data and the nuisance basis, z = B' y* with B the permuted
+// basis rows, the residual formed itself -- and merged into maxout[k] with
+// an order-preserving atomic max (a NaN statistic is left out, as the
+// contraction's running max leaves it out).
+struct SampleFixArgs {
+    const unsigned long long* samples;    // BPG_SAMPLE_SEGS segments of cap entries
+    const unsigned long long* n_samples;  // per segment; entries past cap are not read
+    unsigned long long cap;
+    const float* Yc;              // [N][Vc]
+    const double* Qn;             // [NR][QB] nuisance basis, row-major
+    int QB, q;
+    const double* Qt;             // N x pfit, column-major
+    const unsigned short* perm;   // [K][N]
+    int N, pfit, Vc, K;
+    double dof;
+    float* maxout;                // [K]
+};
+// t statistics, up to 64 subjects, 16 fit columns and 24 nuisance columns
+constexpr int BPG_FIX_MAX_N = 64;
+constexpr int BPG_FIX_MAX_P = 16;
+hipError_t launch_fix_samples(const SampleFixArgs& a, hipStream_t s);
 
 struct ObsArgs {
     const float* Yc;       // [N][Vc]

@@ -353,19 +353,20 @@ int contract_bf16_blocks_per_cu(int KK, int PP, bool sign, bool ftest, bool out,
 }
 
 template <int KK>
-hipError_t dispatch_persist_pp(const BatchClass& c, int PP, bool sign, int variant, int grid, hipStream_t s, int* occ);
+hipError_t dispatch_persist_pp(const BatchClass& c, int PP, bool sign, int variant, bool excl, int grid,
+                               hipStream_t s, int* occ);
 
-hipError_t launch_contract_bf16_persist(const BatchClass& c, int KK, int PP, bool sign, int variant, int grid,
-                                        hipStream_t s, int* occ) {
+hipError_t launch_contract_bf16_persist(const BatchClass& c, int KK, int PP, bool sign, int variant, bool excl,
+                                        int grid, hipStream_t s, int* occ) {
     switch (KK) {
-        case 1: return dispatch_persist_pp<1>(c, PP, sign, variant, grid, s, occ);
-        case 2: return dispatch_persist_pp<2>(c, PP, sign, variant, grid, s, occ);
-        case 3: return dispatch_persist_pp<3>(c, PP, sign, variant, grid, s, occ);
-        case 4: return dispatch_persist_pp<4>(c, PP, sign, variant, grid, s, occ);
-        case 5: return dispatch_persist_pp<5>(c, PP, sign, variant, grid, s, occ);
-        case 6: return dispatch_persist_pp<6>(c, PP, sign, variant, grid, s, occ);
-        case 7: return dispatch_persist_pp<7>(c, PP, sign, variant, grid, s, occ);
-        case 8: return dispatch_persist_pp<8>(c, PP, sign, variant, grid, s, occ);
+        case 1: return dispatch_persist_pp<1>(c, PP, sign, variant, excl, grid, s, occ);
+        case 2: return dispatch_persist_pp<2>(c, PP, sign, variant, excl, grid, s, occ);
+        case 3: return dispatch_persist_pp<3>(c, PP, sign, variant, excl, grid, s, occ);
+        case 4: return dispatch_persist_pp<4>(c, PP, sign, variant, excl, grid, s, occ);
+        case 5: return dispatch_persist_pp<5>(c, PP, sign, variant, excl, grid, s, occ);
+        case 6: return dispatch_persist_pp<6>(c, PP, sign, variant, excl, grid, s, occ);
+        case 7: return dispatch_persist_pp<7>(c, PP, sign, variant, excl, grid, s, occ);
+        case 8: return dispatch_persist_pp<8>(c, PP, sign, variant, excl, grid, s, occ);
     }
     return hipErrorInvalidValue;
 }
@@ -448,6 +449,86 @@ __global__ __launch_bounds__(1024) void k_reduce_max_jobs(const ReduceJob* __res
 hipError_t launch_reduce_max_jobs(const ReduceJob* jobs, int n_jobs, int max_K, hipStream_t s) {
     if (n_jobs <= 0 || max_K <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_max_jobs, dim3((max_K + 31) / 32, n_jobs), dim3(1024), 0, s, jobs);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 recomputation of listed samples (one thread per sample).  y* = y -
+// Qn (Qn' y) and the fit of y* on the permuted basis rows B[s][i] =
+// Qt[perm[s] + i N] are both orthogonal projections, applied one column at
+// a time (modified Gram-Schmidt: z_i = B_i' r, r -= B_i z_i), so the
+// residual is formed itself, as in the FP64 form, and SSE = |r|^2.
+// ---------------------------------------------------------------------------
+// max(*addr, v) for floats: non-negative floats order as signed integers,
+// negative ones in reverse as unsigned integers
+__device__ __forceinline__ void atomic_max_f32(float* addr, float v) {
+    v = v + 0.0f;  // -0 -> +0
+    if (v >= 0.0f) atomicMax(reinterpret_cast<int*>(addr), __float_as_int(v));
+    else atomicMin(reinterpret_cast<unsigned*>(addr), __float_as_uint(v));
+}
+
+// The block's threads share the basis in LDS (the nuisance columns read as
+// broadcasts, the permuted fit columns gathered by row); MN, the subjects
+// held in registers, is N rounded up to 16.
+template <int MN>
+__global__ __launch_bounds__(256) void k_fix_samples(SampleFixArgs a) {
+    __shared__ double sQt[BPG_FIX_MAX_N * BPG_FIX_MAX_P];
+    __shared__ double sQn[BPG_FIX_MAX_N * 24];
+    const int N = a.N, q = a.q, pfit = a.pfit;
+    for (int t = threadIdx.x; t < N * pfit; t += 256) sQt[t] = a.Qt[t];
+    for (int t = threadIdx.x; t < N * q; t += 256) sQn[t] = a.Qn[(size_t)(t / q) * a.QB + t % q];
+    __syncthreads();
+    // block b takes segments b, b + gridDim.x, ...
+    for (int g = blockIdx.x; g < BPG_SAMPLE_SEGS; g += gridDim.x) {
+    const unsigned long long n = min(a.n_samples[g], a.cap);
+    for (unsigned long long i = threadIdx.x; i < n; i += 256ull) {
+        const unsigned long long e = a.samples[(size_t)g * a.cap + i];
+        const int k = (int)(e >> 32), v = (int)(e & 0xFFFFFFFFull);
+        if (k >= a.K || v >= a.Vc) continue;  // the contraction lists in-range samples only
+        double r[MN];
+        int row[MN];
+#pragma unroll
+        for (int s = 0; s < MN; s++) {
+            r[s] = s < N ? (double)a.Yc[(size_t)s * a.Vc + v] : 0.0;
+            row[s] = s < N ? (int)a.perm[(size_t)k * N + s] : 0;
+        }
+        for (int j = 0; j < q; j++) {
+            double w = 0.0;
+#pragma unroll
+            for (int s = 0; s < MN; s++) w = fma(s < N ? sQn[s * q + j] : 0.0, r[s], w);
+#pragma unroll
+            for (int s = 0; s < MN; s++)
+                if (s < N) r[s] = fma(-sQn[s * q + j], w, r[s]);
+        }
+        double z0 = 0.0;
+        for (int c = 0; c < pfit; c++) {
+            const double* col = sQt + c * N;
+            double b[MN];
+#pragma unroll
+            for (int s = 0; s < MN; s++) b[s] = s < N ? col[row[s]] : 0.0;
+            double z = 0.0;
+#pragma unroll
+            for (int s = 0; s < MN; s++) z = fma(b[s], r[s], z);
+#pragma unroll
+            for (int s = 0; s < MN; s++) r[s] = fma(-b[s], z, r[s]);
+            if (c == 0) z0 = z;
+        }
+        double sse = 0.0;
+#pragma unroll
+        for (int s = 0; s < MN; s++) sse = fma(r[s], r[s], sse);
+        const float t = (float)(z0 * sqrt(a.dof / sse));
+        if (t == t) atomic_max_f32(a.maxout + k, t);
+    }
+    }
+}
+
+hipError_t launch_fix_samples(const SampleFixArgs& a, hipStream_t s) {
+    if (a.N > BPG_FIX_MAX_N || a.pfit > BPG_FIX_MAX_P || a.q > 24 || !a.perm) return hipErrorInvalidValue;
+    const unsigned blocks = BPG_SAMPLE_SEGS;
+    if (a.N <= 16) hipLaunchKernelGGL(k_fix_samples<16>, dim3(blocks), dim3(256), 0, s, a);
+    else if (a.N <= 32) hipLaunchKernelGGL(k_fix_samples<32>, dim3(blocks), dim3(256), 0, s, a);
+    else if (a.N <= 48) hipLaunchKernelGGL(k_fix_samples<48>, dim3(blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_fix_samples<64>, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -121,6 +121,15 @@ typedef struct {
     long pvalue_ties;     /* set by bpg_plan_pvalues: the (voxel, permutation)
                              comparisons inside the error band (the total of
                              its `ties` output)                              */
+    long fp64_samples;    /* split-BF16 max-T of a t test with few residual
+                             degrees of freedom (up to 64 subjects): the
+                             ill-conditioned samples left out of the maxima
+                             and recomputed one by one in FP64 (a list sized
+                             from the expected count; samples past it send
+                             their permutations to the whole repeat above;
+                             BPG_SAMPLE_LIST=off|on and BPG_SAMPLE_CAP, the
+                             entries per each of 256 segments, override the
+                             choice and the size)                            */
 } bpg_timing;
 
 /* ---- context: replaces BROCCOLI_LIB(platform, device, ...) +

@@ -55,6 +55,113 @@ def check_maxT(md_gpu, ref, stats):
     return int(d.sum())
 
 
+def check_maxT_gate(md_gpu, ref):
+    """Quantised max-T against the FP64 oracle within the accuracy gate
+    itself: a null value that differs from the FP64 one must lie between
+    the quantised values of m64 -/+ max_v 1e-4 (1 + |t64|).  For max-T-only
+    runs, whose per-sample errors are not returned."""
+    m64 = ref.max(axis=1)
+    q64 = oracle.quantised_max(m64)
+    band = (TOL * (1.0 + np.abs(ref))).max(axis=1)
+    d = md_gpu != q64
+    lo = oracle.quantised_max(m64 - band)
+    hi = oracle.quantised_max(m64 + band)
+    assert np.all((md_gpu[d] >= lo[d]) & (md_gpu[d] <= hi[d])), "max-T differs outside the gate's band"
+    return int(d.sum())
+
+
+@pytest.fixture
+def sample_env():
+    """Restores the sample-list environment switches after a test."""
+    import os
+    keep = {k: os.environ.get(k) for k in ("BPG_SAMPLE_LIST", "BPG_SAMPLE_CAP")}
+    yield os.environ
+    for k, v in keep.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+def low_dof_study(N, p, scale, seed):
+    """A t test on the last of p columns (intercept + p - 1 covariates) with
+    N - p residual degrees of freedom, on Gaussian data."""
+    rng = np.random.default_rng(seed)
+    st = synth.study(2, scale=scale)
+    st["N"], st["p"], st["nc"] = N, p, 1
+    st["X"] = np.c_[np.ones(N), rng.standard_normal((N, p - 1))].astype(np.float32)
+    C = np.zeros((1, p), np.float32)
+    C[0, p - 1] = 1.0
+    st["C"] = C
+    st["Y"] = rng.standard_normal((N,) + st["mask"].shape).astype(np.float32)
+    st["inference"] = capi.VOXEL
+    return st
+
+
+@pytest.mark.parametrize("N,p", [(14, 6), (10, 9), (24, 17)])
+def test_sample_list_recomputes_ill_conditioned_samples_in_fp64(ctx, sample_env, N, p):
+    """Max-T only, few residual degrees of freedom: most permutations meet
+    ill-conditioned samples (SSE < |y*|^2 / 8).  They are left out of the
+    split-BF16 maxima and recomputed one by one in FP64 (no permutation is
+    repeated), and the null meets the FP64 oracle within the gate; the
+    whole-permutation repeat (list switched off) does too."""
+    st = low_dof_study(N, p, 0.12, 40 + N)
+    K = 96
+    perms = dict(perm=capi.gen_perm_correlation(N, K, compat=False, seed=3))
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    print(f"sample list N={N} p={p} V={st['V']} K={K}", flush=True)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md = plan.permutation_phase(0, **perms)[0]
+        tm = plan.timing()
+    print(f"  listed samples {tm.fp64_samples}, fp32 perms {tm.fp32_permutations}, unsafe {tm.unsafe_waves}, "
+          f"ties {check_maxT_gate(md, ref)}", flush=True)
+    assert tm.fp64_samples > 0 and tm.fp32_permutations == 0 and tm.unsafe_waves == 0
+    sample_env["BPG_SAMPLE_LIST"] = "off"
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md_off = plan.permutation_phase(0, **perms)[0]
+        tm_off = plan.timing()
+    print(f"  list off: fp32 perms {tm_off.fp32_permutations}, fp64 perms {tm_off.fp64_permutations}, "
+          f"ties {check_maxT_gate(md_off, ref)}", flush=True)
+    # repeated whole: in FP32, and in FP64 where FP32 flags them again
+    assert tm_off.fp64_samples == 0 and tm_off.fp32_permutations + tm_off.fp64_permutations > 0
+
+
+def test_sample_list_overflow_repeats_whole_permutations(ctx, sample_env):
+    """One residual degree of freedom and lists of 16 entries per segment
+    (about 17,000 samples over 256 segments): the samples past a segment's
+    end flag their permutations, which are repeated whole in FP32, and in
+    FP64 where FP32 flags them again; the null still meets the FP64 oracle
+    within the gate."""
+    st = low_dof_study(10, 9, 0.12, 54)
+    K = 64
+    perms = dict(perm=capi.gen_perm_correlation(10, K, compat=False, seed=4))
+    ref = oracle.perm_stats(st, perms, K)[:, vox(st)]
+    sample_env["BPG_SAMPLE_CAP"] = "16"
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        md = plan.permutation_phase(0, **perms)[0]
+        tm = plan.timing()
+    print(f"overflow: listed {tm.fp64_samples}, unsafe {tm.unsafe_waves}, fp32 perms {tm.fp32_permutations}, "
+          f"fp64 perms {tm.fp64_permutations}, ties {check_maxT_gate(md, ref)}", flush=True)
+    assert 0 < tm.fp64_samples <= 16 * 256 and tm.unsafe_waves > 0
+    assert tm.fp32_permutations + tm.fp64_permutations > 0
+
+
+def test_sample_list_forced_on_a_well_conditioned_design(ctx, sample_env):
+    """The list switched on where no sample is ill-conditioned: nothing is
+    listed and the null equals the one without the list bit for bit."""
+    st = synth.study(2, scale=0.15)
+    st["inference"] = capi.VOXEL
+    K = 64
+    perms = synth.permutations(st, K, seed=12)
+    with capi.Plan(ctx, st, "split_bf16") as plan:
+        base = plan.permutation_phase(0, **perms)[0]
+        sample_env["BPG_SAMPLE_LIST"] = "on"
+        on = plan.permutation_phase(0, **perms)[0]
+        tm = plan.timing()
+    assert tm.fp64_samples == 0 and tm.unsafe_waves == 0
+    assert np.array_equal(base, on)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "split_bf16", "fp64", "fp32_valu"])
 @pytest.mark.parametrize("wl,scale,K,con", CASES)
 def test_permuted_statistics_match_fp64(ctx, wl, scale, K, con, prec):
@@ -687,13 +794,17 @@ def test_graph_batch_of_64_studies(ctx):
                 ms, gms, repeated, classes = batch.last_timing()
                 print(f"  replay {rep}: {ms:.2f} ms (graph {gms:.2f}), {classes} classes, {repeated} studies with "
                       f"repeated permutations", flush=True)
+                listed = 0
                 for i, (pl, d) in enumerate(zip(plans, perms)):
-                    single, stats = pl.permutation_phase(0, want_stats=True, **d)
+                    single = pl.permutation_phase(0, **d)[0]
+                    listed += pl.timing().fp64_samples > 0
                     assert np.array_equal(got[i], single), i
-                    if rep == 0 and i % 8 == 0:
+                    if rep == 0 and (i % 8 == 0 or pl.timing().fp64_samples > 0):
                         ref = oracle.perm_stats(studies[i], d, Ks[i])[:, vox(studies[i])]
+                        _, stats = pl.permutation_phase(0, want_stats=True, **d)
                         assert (np.abs(stats - ref) / (1.0 + np.abs(ref))).max() <= TOL
-                        check_maxT(single, ref, stats)
+                        check_maxT_gate(single, ref)
+                print(f"  {listed} studies with listed samples", flush=True)
     finally:
         for pl in plans:
             pl.close()
