@@ -19,6 +19,12 @@
 #include "bpg_kernels.h"
 #include "bpg_runtime.h"
 
+// the least chance of an ill-conditioned sample per permutation for which a
+// max-T-only phase keeps a sample list
+#ifndef BPG_LIST_MIN_CHANCE
+#define BPG_LIST_MIN_CHANCE 0.01
+#endif
+
 namespace {
 
 using bpg::DBuf;
@@ -507,7 +513,7 @@ unsigned long long sample_list_cap(double dof, int pfit, long V, long K, double 
     if (dof <= 0.0 || pfit <= 0 || V <= 0 || K <= 0) return 0ull;
     const double pv = inc_beta(tau, 0.5 * dof, 0.5 * pfit);
     const double expected = pv * (double)V * (double)K;
-    const bool pays = ill_conditioned_chance(dof, pfit, V, tau) >= 0.01 && expected <= (double)(1 << 20);
+    const bool pays = ill_conditioned_chance(dof, pfit, V, tau) >= BPG_LIST_MIN_CHANCE && expected <= (double)(1 << 20);
     if (!on && !pays) return 0ull;
     if (forced) return forced;
     const double want = std::min(8.0 * expected / bpg::BPG_SAMPLE_SEGS, 32768.0);
@@ -1637,9 +1643,12 @@ struct bpg_batch {
     HBuf<float> h_sign, h_max;
     HBuf<unsigned> h_unsafe;
     HBuf<unsigned char> h_flags;
-    hipStream_t stream = nullptr, side = nullptr;
+    hipStream_t stream = nullptr;
+    // the branches of the graphs: stream and side[0]
+    static constexpr int NBR = 2;
+    hipStream_t side[NBR - 1] = {};
     hipEvent_t first_done = nullptr;
-    hipEvent_t fork[2] = {}, join[2] = {};
+    hipEvent_t fork[2] = {}, join[2][NBR] = {}, prep[2][NBR] = {};
     hipGraph_t graph[2] = {};
     hipGraphExec_t exec[2] = {};
     double last_ms = 0.0, last_graph_ms = 0.0;
@@ -1658,9 +1667,13 @@ struct bpg_batch {
         if (first_done) (void)hipEventDestroy(first_done);
         for (int g = 0; g < 2; g++) {
             if (fork[g]) (void)hipEventDestroy(fork[g]);
-            if (join[g]) (void)hipEventDestroy(join[g]);
+            for (int h = 0; h < NBR; h++) {
+                if (join[g][h]) (void)hipEventDestroy(join[g][h]);
+                if (prep[g][h]) (void)hipEventDestroy(prep[g][h]);
+            }
         }
-        if (side) (void)hipStreamDestroy(side);
+        for (auto& x : side)
+            if (x) (void)hipStreamDestroy(x);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto& s : st) {
             s.yy.release(); s.Yb.release(); s.Gb.release(); s.partial.release(); s.samples.release();
@@ -1836,20 +1849,24 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
         BPG_BHIP(hipMemcpy(b->jobs.p, rjobs.data(), rjobs.size() * sizeof(bpg::ReduceJob), hipMemcpyHostToDevice));
     BPG_BHIP(hipStreamCreate(&b->stream));
     BPG_BHIP(hipEventCreateWithFlags(&b->first_done, hipEventDisableTiming));
-    BPG_BHIP(hipStreamCreate(&b->side));
+    for (auto& x : b->side) BPG_BHIP(hipStreamCreate(&x));
     for (int g = 0; g < 2; g++) {
         BPG_BHIP(hipEventCreateWithFlags(&b->fork[g], hipEventDisableTiming));
-        BPG_BHIP(hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming));
+        for (int h = 0; h < bpg_batch::NBR; h++) {
+            BPG_BHIP(hipEventCreateWithFlags(&b->join[g][h], hipEventDisableTiming));
+            BPG_BHIP(hipEventCreateWithFlags(&b->prep[g][h], hipEventDisableTiming));
+        }
     }
     // Two graphs.  The first: the upload of all permutations / signs and one
-    // fill clearing the queue heads, ill-conditioned counts and flags (a
-    // kernel, not a memset node); then, as the second, the stage's classes,
-    // the largest work first and taken in turn by two branches, each its
-    // members' nuisance transforms and operand builds and then its
-    // persistent contraction; one launch of the stage's max reductions; the
-    // copies of its maxima, counts and flags to the host.  Stage 0 holds
-    // the studies likely to meet ill-conditioned samples, so that their
-    // repeats run beside the second graph.
+    // fill clearing the sample-list counters, queue heads, ill-conditioned
+    // counts and flags (a kernel, not a memset node); then, as the second,
+    // the stage's classes, the largest work first and taken in turn by two
+    // branches: all members' nuisance transforms and operand builds, then
+    // (both branches done) the persistent contractions; one launch of the
+    // stage's max reductions; the FP64 recomputation of listed samples; the
+    // copies of its maxima, counts and flags to the host.  Stage 0 holds the
+    // studies likely to repeat permutations, so that their repeats run
+    // beside the second graph.
     std::vector<double> work(b->cls.size(), 0.0);
     for (size_t ci = 0; ci < b->cls.size(); ci++)
         for (int i : b->cls[ci].members) {
@@ -1881,18 +1898,21 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 cap(hipMemcpyAsync(b->sign_all.p, b->h_sign.p, b->n_sign * sizeof(float), hipMemcpyHostToDevice, cs));
             cap(bpg::launch_fill_u32(b->words.p, n_words, 0u, cs));
         }
-        // two branches take the stage's classes in turn, so that a class's
-        // blocks start in the tail of the one before it
+        // the branches take the stage's classes in turn: first their
+        // members' data and operand fragments (bandwidth-bound kernels that
+        // would wait for CUs behind a persistent contraction), then, once
+        // every branch has built its share, the contractions, so that a
+        // class's blocks start in the tail of the one before it
+        constexpr int NBR = bpg_batch::NBR;
+        auto branch = [&](int t) { return t % NBR == 0 ? cs : b->side[t % NBR - 1]; };
         cap(hipEventRecord(b->fork[g], cs));
-        cap(hipStreamWaitEvent(b->side, b->fork[g], 0));
+        for (int h = 1; h < NBR; h++) cap(hipStreamWaitEvent(b->side[h - 1], b->fork[g], 0));
         int turn = 0;
         for (int ci : order) {
             if (rc != BPG_OK) break;
             const auto& c = b->cls[ci];
             if (c.stage != g) continue;
-            hipStream_t bs = (turn++ & 1) ? b->side : cs;
-            // the members' data and operand fragments shortly before their
-            // contraction, while they are still in the last-level cache
+            hipStream_t bs = branch(turn++);
             for (int i : c.members) {
                 auto& s = b->st[i];
                 bpg_plan* pl = s.plan;
@@ -1916,6 +1936,17 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 oa.Gb = s.Gb.p;
                 cap(bpg::launch_operand_bf16(oa, bs));
             }
+        }
+        for (int h = 0; h < NBR; h++) cap(hipEventRecord(b->prep[g][h], branch(h)));
+        for (int h = 0; h < NBR; h++)
+            for (int o = 0; o < NBR; o++)
+                if (o != h) cap(hipStreamWaitEvent(branch(h), b->prep[g][o], 0));
+        turn = 0;
+        for (int ci : order) {
+            if (rc != BPG_OK) break;
+            const auto& c = b->cls[ci];
+            if (c.stage != g) continue;
+            hipStream_t bs = branch(turn++);
             bpg::BatchClass bc{};
             bc.studies = b->table.p + c.first;
             bc.n_studies = (int)c.members.size();
@@ -1923,8 +1954,10 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
             bc.counter = b->heads() + (size_t)ci * bpg::BPG_XCDS;
             cap(bpg::launch_contract_bf16_persist(bc, c.KK, c.PP, c.sign != 0, c.variant, c.excl != 0, c.grid, bs));
         }
-        cap(hipEventRecord(b->join[g], b->side));
-        cap(hipStreamWaitEvent(cs, b->join[g], 0));
+        for (int h = 1; h < NBR; h++) {
+            cap(hipEventRecord(b->join[g][h], b->side[h - 1]));
+            cap(hipStreamWaitEvent(cs, b->join[g][h], 0));
+        }
         if (!empty) {
             cap(bpg::launch_reduce_max_jobs(b->jobs.p + j0, b->job_end[g] - j0, b->max_K, cs));
             // the stage's listed samples, recomputed in FP64, into its maxima

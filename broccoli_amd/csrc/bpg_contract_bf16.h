@@ -40,6 +40,22 @@ typedef float __attribute__((address_space(1))) gfloat;
 typedef unsigned char __attribute__((address_space(1))) gbyte;
 typedef unsigned long long __attribute__((address_space(1))) gu64;
 
+// *p, read by the whole wave from one address, as scalar-register values:
+// each word goes through v_readfirstlane (read through a generic pointer, a
+// structure of kernel arguments would otherwise stay in vector registers)
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    constexpr int W = (int)(sizeof(T) / 4);
+    const unsigned* src = reinterpret_cast<const unsigned*>(p);
+    unsigned w[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // v_writelane_b32: old with lane `lane` replaced by the uniform `value` (the
 // LLVM intrinsic; this compiler exposes no builtin for it)
 __device__ int lane_write(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
@@ -557,10 +573,10 @@ __global__ __launch_bounds__(256, (persist_min_blocks<KK, R, SIGN, PIPE>())) voi
             if (item >= c.n_items[x]) break;
             int j = 0;
             while (j + 1 < c.n_studies && c.studies[j + 1].item0[x] <= item) j++;
-            // the study's arguments as values (uniform: scalar registers),
-            // not references into memory the item's stores might alias
-            const ContractArgs a = c.studies[j].a;
-            const Bf16Launch L = c.studies[j].L;
+            // the study's arguments as values, not references into memory
+            // the item's stores might alias, and in scalar registers
+            const ContractArgs a = uniform_load(&c.studies[j].a);
+            const Bf16Launch L = uniform_load(&c.studies[j].L);
             const int local = item - c.studies[j].item0[x];
             contract_item<KK, R, PP, SIGN, false, OUT, PIPE>(a, L, local % L.n_mgroups,
                                                              x + BPG_XCDS * (local / L.n_mgroups));
