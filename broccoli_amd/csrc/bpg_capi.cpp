@@ -1626,6 +1626,16 @@ struct bpg_batch {
     std::vector<Class> cls;
     DBuf<bpg::BatchStudy> table;
     DBuf<bpg::ReduceJob> jobs;
+    // the nuisance transforms, one launch per (stage, nuisance width): the
+    // group's studies' arguments at ttab + tab_off, their first blocks at
+    // tfirst + tab_off
+    struct TGroup {
+        int stage, QB, n, blocks;
+        size_t tab_off;
+    };
+    std::vector<TGroup> tgroups;
+    DBuf<bpg::TransformArgs> ttab;
+    DBuf<int> tfirst;
     // one word buffer cleared by one fill: the sample lists' segment counters
     // (64-bit, two words each), queue heads (8 per class), the studies'
     // ill-conditioned counts, then their flag bytes
@@ -1678,7 +1688,7 @@ struct bpg_batch {
         for (auto& s : st) {
             s.yy.release(); s.Yb.release(); s.Gb.release(); s.partial.release(); s.samples.release();
         }
-        table.release(); jobs.release(); words.release(); perm_all.release(); sign_all.release(); max_all.release();
+        table.release(); jobs.release(); ttab.release(); tfirst.release(); words.release(); perm_all.release(); sign_all.release(); max_all.release();
         h_perm.release(); h_sign.release(); h_max.release(); h_unsafe.release(); h_flags.release();
     }
 };
@@ -1841,6 +1851,46 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 rjobs.push_back(bpg::ReduceJob{s.partial.p, b->max_all.p + s.max_off, s.L.n_vchunks, (int)s.K});
         b->job_end[g] = (int)rjobs.size();
     }
+    // the transform groups
+    {
+        std::vector<bpg::TransformArgs> targs;
+        std::vector<int> tfirst;
+        // (each width's studies in two halves, one per branch)
+        for (int g = 0; g < 2; g++)
+            for (int QB : {24, 16, 8, 4, 0})
+                for (int half = 0; half < 2; half++) {
+                bpg_batch::TGroup tg{g, QB, 0, 0, targs.size()};
+                int seen = 0;
+                for (const auto& s : b->st) {
+                    const bpg_plan* pl = s.plan;
+                    if (s.stage != g || pl->Vc == 0) continue;
+                    const bpg::MapModel& m = pl->models[s.map];
+                    if (bpg::transform_width(m.q) != QB) continue;
+                    if ((seen++ & 1) != half) continue;
+                    const double* mq = pl->mdev.p + pl->mdev_off[s.map];
+                    const int NR = (s.N + 15) / 16 * 16;
+                    bpg::TransformArgs ta{};
+                    ta.Yc = pl->Yc.p;
+                    ta.Qn = mq + m.Qt.size();
+                    ta.dscale = mq + m.Qt.size() + (size_t)NR * std::max(QB, 1);
+                    ta.N = s.N; ta.q = m.q; ta.Vc = pl->Vc; ta.Vpad = pl->Vpad;
+                    ta.KK = s.KK; ta.NS = 0; ta.npy = s.npy;
+                    ta.yy = s.yy.p; ta.Yb = s.Yb.p;
+                    targs.push_back(ta);
+                    tfirst.push_back(tg.blocks);
+                    tg.blocks += (pl->Vpad + 255) / 256;
+                    tg.n++;
+                }
+                if (tg.n) b->tgroups.push_back(tg);
+                }
+        BPG_BHIP(b->ttab.ensure(std::max<size_t>(targs.size(), 1)));
+        BPG_BHIP(b->tfirst.ensure(std::max<size_t>(tfirst.size(), 1)));
+        if (!targs.empty()) {
+            BPG_BHIP(hipMemcpy(b->ttab.p, targs.data(), targs.size() * sizeof(bpg::TransformArgs),
+                               hipMemcpyHostToDevice));
+            BPG_BHIP(hipMemcpy(b->tfirst.p, tfirst.data(), tfirst.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
     BPG_BHIP(b->table.ensure(std::max<size_t>(host.size(), 1)));
     if (!host.empty())
         BPG_BHIP(hipMemcpy(b->table.p, host.data(), host.size() * sizeof(bpg::BatchStudy), hipMemcpyHostToDevice));
@@ -1898,16 +1948,22 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
                 cap(hipMemcpyAsync(b->sign_all.p, b->h_sign.p, b->n_sign * sizeof(float), hipMemcpyHostToDevice, cs));
             cap(bpg::launch_fill_u32(b->words.p, n_words, 0u, cs));
         }
-        // the branches take the stage's classes in turn: first their
-        // members' data and operand fragments (bandwidth-bound kernels that
-        // would wait for CUs behind a persistent contraction), then, once
-        // every branch has built its share, the contractions, so that a
-        // class's blocks start in the tail of the one before it
+        // the branches take the stage's work in turn: first the data
+        // fragments (one transform launch per nuisance width, so that small
+        // studies fill the chip together) and operand fragments of its
+        // studies (bandwidth-bound kernels that would wait for CUs behind a
+        // persistent contraction), then, once every branch has built its
+        // share, the contractions, class by class, so that a class's blocks
+        // start in the tail of the one before it
         constexpr int NBR = bpg_batch::NBR;
         auto branch = [&](int t) { return t % NBR == 0 ? cs : b->side[t % NBR - 1]; };
         cap(hipEventRecord(b->fork[g], cs));
         for (int h = 1; h < NBR; h++) cap(hipStreamWaitEvent(b->side[h - 1], b->fork[g], 0));
         int turn = 0;
+        for (const auto& tg : b->tgroups)
+            if (tg.stage == g && rc == BPG_OK)
+                cap(bpg::launch_transform_batch(b->ttab.p + tg.tab_off, b->tfirst.p + tg.tab_off, tg.n, tg.blocks,
+                                                tg.QB, branch(turn++)));
         for (int ci : order) {
             if (rc != BPG_OK) break;
             const auto& c = b->cls[ci];
@@ -1915,22 +1971,11 @@ extern "C" int bpg_batch_create(bpg_plan* const* plans, int n, const int* maps, 
             hipStream_t bs = branch(turn++);
             for (int i : c.members) {
                 auto& s = b->st[i];
-                bpg_plan* pl = s.plan;
-                const bpg::MapModel& m = pl->models[s.map];
-                const double* mq = pl->mdev.p + pl->mdev_off[s.map];
-                const int QB = bpg::transform_width(m.q), NR = (s.N + 15) / 16 * 16;
-                bpg::TransformArgs ta{};
-                ta.Yc = pl->Yc.p;
-                ta.Qn = mq + m.Qt.size();
-                ta.dscale = mq + m.Qt.size() + (size_t)NR * std::max(QB, 1);
-                ta.N = s.N; ta.q = m.q; ta.Vc = pl->Vc; ta.Vpad = pl->Vpad;
-                ta.KK = s.KK; ta.NS = 0; ta.npy = s.npy;
-                ta.yy = s.yy.p; ta.Yb = s.Yb.p;
-                cap(bpg::launch_transform(ta, bs));
+                const bpg::MapModel& m = s.plan->models[s.map];
                 bpg::OperandArgs oa{};
                 oa.perm = s.sign ? nullptr : b->perm_all.p + s.in_off;
                 oa.sign = s.sign ? b->sign_all.p + s.in_off : nullptr;
-                oa.Qt = mq;
+                oa.Qt = s.plan->mdev.p + s.plan->mdev_off[s.map];
                 oa.N = s.N; oa.pfit = m.pfit; oa.K = (int)s.K; oa.PP = s.PP; oa.KK = s.KK;
                 oa.n_sub = s.L.n_sub;
                 oa.Gb = s.Gb.p;

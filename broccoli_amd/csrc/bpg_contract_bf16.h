@@ -40,21 +40,6 @@ typedef float __attribute__((address_space(1))) gfloat;
 typedef unsigned char __attribute__((address_space(1))) gbyte;
 typedef unsigned long long __attribute__((address_space(1))) gu64;
 
-// *p, read by the whole wave from one address, as scalar-register values:
-// each word goes through v_readfirstlane (read through a generic pointer, a
-// structure of kernel arguments would otherwise stay in vector registers)
-template <typename T>
-__device__ __forceinline__ T uniform_load(const T* p) {
-    static_assert(sizeof(T) % 4 == 0, "whole words");
-    constexpr int W = (int)(sizeof(T) / 4);
-    const unsigned* src = reinterpret_cast<const unsigned*>(p);
-    unsigned w[W];
-#pragma unroll
-    for (int i = 0; i < W; i++) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
-    T v;
-    __builtin_memcpy(&v, w, sizeof(T));
-    return v;
-}
 
 // v_writelane_b32: old with lane `lane` replaced by the uniform `value` (the
 // LLVM intrinsic; this compiler exposes no builtin for it)

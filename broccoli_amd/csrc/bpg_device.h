@@ -33,6 +33,22 @@ __device__ __forceinline__ float dec_max(unsigned e) {
 }
 
 
+// *p, read by the whole wave from one address, as scalar-register values:
+// each word goes through v_readfirstlane (read through a generic pointer, a
+// structure of kernel arguments would otherwise stay in vector registers)
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    constexpr int W = (int)(sizeof(T) / 4);
+    const unsigned* src = reinterpret_cast<const unsigned*>(p);
+    unsigned w[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // Blocks are dealt round-robin to the 8 XCDs.  Renumber them so that the
 // blocks resident on one XCD walk consecutive logical ids, i.e. share the
 // same voxel chunk (and its lines in that XCD's L2).

@@ -62,6 +62,11 @@ struct TransformArgs {
 // padded width of the nuisance basis for q columns (0 for none, -1 if q > 24)
 int transform_width(int q);
 hipError_t launch_transform(const TransformArgs& a, hipStream_t s);
+// the transforms of n studies of one nuisance width QB in one launch: study
+// j takes blocks first[j] .. first[j + 1] - 1 (first[j + 1] - first[j] =
+// its Vpad / 256), `blocks` in all; tab and first in device memory
+hipError_t launch_transform_batch(const TransformArgs* tab, const int* first, int n, int blocks, int QB,
+                                  hipStream_t s);
 
 struct OperandArgs {
     const unsigned short* perm;  // [K][N] (null for signs)

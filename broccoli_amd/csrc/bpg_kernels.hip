@@ -51,9 +51,7 @@ __device__ __forceinline__ void load8(const TransformArgs& a, int v, bool in, in
 }
 
 template <int QB>
-__global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= a.Vpad) return;
+__device__ __forceinline__ void transform_voxel(const TransformArgs& a, int v) {
     const bool in = v < a.Vc;
     double w[QB > 0 ? QB : 1];
 #pragma unroll
@@ -149,6 +147,32 @@ __global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
     a.yy[v] = in ? yy : __builtin_nan("");
 }
 
+template <int QB>
+__global__ __launch_bounds__(256) void k_transform(TransformArgs a) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.Vpad) return;
+    transform_voxel<QB>(a, v);
+}
+
+// Several studies' transforms (one nuisance width) in one launch, so that
+// the small grids of small studies fill the chip together: block b belongs
+// to study j with first[j] <= b < first[j + 1] (a binary search of the
+// uniform table), whose arguments it reads as scalar values.
+template <int QB>
+__global__ __launch_bounds__(256) void k_transform_batch(const TransformArgs* tab, const int* first, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (__builtin_amdgcn_readfirstlane(first[mid]) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TransformArgs a = uniform_load(tab + lo);
+    const int v = (b - __builtin_amdgcn_readfirstlane(first[lo])) * 256 + (int)threadIdx.x;
+    if (v >= a.Vpad) return;
+    transform_voxel<QB>(a, v);
+}
+
 int transform_width(int q) {
     if (q <= 0) return 0;
     if (q <= 4) return 4;
@@ -165,6 +189,20 @@ hipError_t launch_transform(const TransformArgs& a, hipStream_t s) {
         case 8: hipLaunchKernelGGL(k_transform<8>, dim3(blocks), dim3(256), 0, s, a); break;
         case 16: hipLaunchKernelGGL(k_transform<16>, dim3(blocks), dim3(256), 0, s, a); break;
         case 24: hipLaunchKernelGGL(k_transform<24>, dim3(blocks), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_transform_batch(const TransformArgs* tab, const int* first, int n, int blocks, int QB,
+                                  hipStream_t s) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    switch (QB) {
+        case 0: hipLaunchKernelGGL(k_transform_batch<0>, dim3(blocks), dim3(256), 0, s, tab, first, n); break;
+        case 4: hipLaunchKernelGGL(k_transform_batch<4>, dim3(blocks), dim3(256), 0, s, tab, first, n); break;
+        case 8: hipLaunchKernelGGL(k_transform_batch<8>, dim3(blocks), dim3(256), 0, s, tab, first, n); break;
+        case 16: hipLaunchKernelGGL(k_transform_batch<16>, dim3(blocks), dim3(256), 0, s, tab, first, n); break;
+        case 24: hipLaunchKernelGGL(k_transform_batch<24>, dim3(blocks), dim3(256), 0, s, tab, first, n); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
