@@ -41,6 +41,23 @@ def test_binding_argument_counts_match_declarations():
         assert len(getattr(lib, name).argtypes) == n, name
 
 
+@pytest.mark.parametrize("cname,pyname", [("bpg_timing", "Timing"), ("bpg_study_desc", "StudyDesc")])
+def test_binding_structures_mirror_declarations(cname, pyname):
+    """The ctypes structures name the C structure's fields in its order (a
+    field added in C and not in the binding would shift every later one)."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "bpg.h").read_text(), flags=re.S)
+    body = re.search(r"typedef struct\s*\w*\s*\{([^{}]*)\}\s*" + cname + r"\s*;", text).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if decl:
+            # "type a, b", "type* p", "type a[3]": the names after the type
+            for part in decl.split(","):
+                names.append(re.sub(r"\[.*\]", "", part).strip().split()[-1].lstrip("*"))
+    fields = [f[0] for f in getattr(capi, pyname)._fields_]
+    assert fields == names
+
+
 def test_dropin_class_symbols_exported():
     out = subprocess.run(["nm", "-DC", "--defined-only", str(capi.LIB_PATH)], capture_output=True,
                          text=True).stdout
