@@ -150,6 +150,12 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
     constexpr int PER = 16 / PP;
     const float tyy = a.tau * yhi;  // NaN on padding: never flagged
     float u[PER];
+    // F, bits only (OUT = 2): the statistic's numerator num f_scale and SSE,
+    // compared with the thresholds as products (F > c <=> num f_scale > c SSE
+    // for SSE > 0; an SSE <= 0 is ill-conditioned and its permutation
+    // repeated), which spares the reciprocal
+    constexpr bool FPROD = FT && OUT == 2 && BPG_F_BITS_PRODUCT;
+    float fnum[PER], fsse[PER];
     unsigned excl = 0u;  // OUT = 3: bit P for an ill-conditioned sample P
 #pragma unroll
     for (int P = 0; P < PER; P++) {
@@ -178,7 +184,12 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
             sse = d + ylo;
         }
         const bool ill = sse < tyy;
-        u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
+        if constexpr (FPROD) {
+            fnum[P] = num * a.f_scale;
+            fsse[P] = sse;
+        } else {
+            u[P] = FT ? num * __builtin_amdgcn_rcpf(sse) : z0 * __builtin_amdgcn_rsqf(sse);
+        }
         if constexpr (OUT == 3) {
             u[P] = ill ? __builtin_nanf("") : u[P];
             excl |= (unsigned)ill << P;
@@ -223,14 +234,23 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
         const int slot = vt & 31;
         unsigned long long tmask = 0ull;
         float stv[PER];
+        bool tie[PER];
 #pragma unroll
         for (int P = 0; P < PER; P++) {
-            stv[P] = u[P] * scale;
+            bool over;
+            if constexpr (FPROD) {
+                over = fnum[P] > a.cdt * fsse[P];
+                tie[P] = fnum[P] > a.cdt_lo * fsse[P] && fnum[P] <= a.cdt_hi * fsse[P];
+            } else {
+                stv[P] = u[P] * scale;
+                over = stv[P] > a.cdt;
+                tie[P] = stv[P] > a.cdt_lo && stv[P] <= a.cdt_hi;
+            }
             // the two words of the ballot into lanes slot and 32 + slot
-            const unsigned long long bal = __ballot(stv[P] > a.cdt) & vmask;
+            const unsigned long long bal = __ballot(over) & vmask;
             wb[P] = (unsigned)lane_write((int)(unsigned)bal, slot, (int)wb[P]);
             wb[P] = (unsigned)lane_write((int)(unsigned)(bal >> 32), 32 + slot, (int)wb[P]);
-            tmask |= __ballot(stv[P] > a.cdt_lo && stv[P] <= a.cdt_hi);
+            tmask |= __ballot(tie[P]);
         }
         if (tmask & vmask) {
             // a statistic inside the error band of the threshold (rare):
@@ -239,7 +259,7 @@ __device__ __forceinline__ void epi(const ContractArgs& a, const f32x16& acc, fl
             const int kbase = st * (2 * PER) + hl * PER;
 #pragma unroll
             for (int P = 0; P < PER; P++) {
-                const unsigned long long tb = __ballot(stv[P] > a.cdt_lo && stv[P] <= a.cdt_hi) & vmask;
+                const unsigned long long tb = __ballot(tie[P]) & vmask;
                 const int k = kbase + P;
                 if ((lane & 31) == 0 && k < a.K && (hl ? (tb >> 32) : (tb & 0xFFFFFFFFull)) != 0ull) a.ties[k] = 1;
             }

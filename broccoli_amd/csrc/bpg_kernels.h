@@ -33,6 +33,10 @@
 #ifndef BPG_PIPE_LARGE
 #define BPG_PIPE_LARGE 1
 #endif
+// F statistics with the bitmask alone: thresholds compared as products
+#ifndef BPG_F_BITS_PRODUCT
+#define BPG_F_BITS_PRODUCT 1
+#endif
 #ifndef BPG_ROLL_MIN_KK
 #define BPG_ROLL_MIN_KK 5
 #endif
