@@ -702,26 +702,34 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     }
     BPG_HIP(ctx, hipEventRecord(pl->ev[1], st));
     // while the transform runs: stage (and check) the permutations, then
-    // upload them on the copy stream (the staging is free: the last phase's
-    // upload finished before its operand build, which its final stream
-    // synchronisation waited for)
-    if (m.sign_operand) {
-        std::memcpy(pl->h_sign.p, sign, (size_t)K * N * sizeof(float));
-        BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p, pl->h_sign.p, (size_t)K * N * sizeof(float), hipMemcpyHostToDevice,
-                                    pl->copy));
-    } else {
+    // upload them on the copy stream, in chunks, each uploaded while the
+    // next is staged (the staging is free: the last phase's upload finished
+    // before its operand build, which its final stream synchronisation
+    // waited for)
+    const size_t total = (size_t)K * N;
+    const size_t chunk = std::max<size_t>((total + 3) / 4, (size_t)1 << 16);
+    for (size_t c0 = 0; c0 < total; c0 += chunk) {
+        const size_t c1 = std::min(total, c0 + chunk);
+        if (m.sign_operand) {
+            std::memcpy(pl->h_sign.p + c0, sign + c0, (c1 - c0) * sizeof(float));
+            BPG_HIP(ctx, hipMemcpyAsync(pl->sign.p + c0, pl->h_sign.p + c0, (c1 - c0) * sizeof(float),
+                                        hipMemcpyHostToDevice, pl->copy));
+            continue;
+        }
         unsigned short* hp = pl->h_perm.p;
         unsigned short worst = 0;
-        const size_t total = (size_t)K * N;
-        for (size_t i = 0; i < total; i++) {
+        for (size_t i = c0; i < c1; i++) {
             hp[i] = perm[i];
             worst = std::max(worst, perm[i]);
         }
         if (worst >= N) {
-            (void)hipStreamSynchronize(st);  // the staged model copies must finish
+            // the staged model copies and the chunks in flight must finish
+            (void)hipStreamSynchronize(pl->copy);
+            (void)hipStreamSynchronize(st);
             BPG_FAIL(ctx, BPG_ERR_ARG, "permutation index out of range");
         }
-        BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p, hp, total * sizeof(unsigned short), hipMemcpyHostToDevice, pl->copy));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->perm.p + c0, hp + c0, (c1 - c0) * sizeof(unsigned short),
+                                    hipMemcpyHostToDevice, pl->copy));
     }
     BPG_HIP(ctx, hipEventRecord(pl->copied, pl->copy));
     BPG_HIP(ctx, hipStreamWaitEvent(st, pl->copied, 0));
