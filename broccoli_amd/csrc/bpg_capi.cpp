@@ -625,13 +625,15 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
 
     // buffers
     BPG_HIP(ctx, pl->yy.ensure(Vpad));
-    BPG_HIP(ctx, pl->Qt.ensure((size_t)N * pfit));
     // nuisance basis row-major, zero-padded to QB columns and NR rows
     const int QB = bpg::transform_width(m.q);
     const int NR = (N + 15) / 16 * 16;
     if (QB < 0) BPG_FAIL(ctx, BPG_ERR_DESIGN, "more than 24 nuisance regressors");
-    BPG_HIP(ctx, pl->Qn.ensure((size_t)NR * std::max(QB, 1)));
-    BPG_HIP(ctx, pl->dscale.ensure(NR));
+    if (f64) {
+        BPG_HIP(ctx, pl->Qt.ensure((size_t)N * pfit));
+        BPG_HIP(ctx, pl->Qn.ensure((size_t)NR * std::max(QB, 1)));
+        BPG_HIP(ctx, pl->dscale.ensure(NR));
+    }
     if (bf) {
         BPG_HIP(ctx, pl->Yb.ensure((size_t)(Vpad / 32) * KK * npy * 64 * 4));
         BPG_HIP(ctx, pl->Gb.ensure((size_t)n_sub * KK * 2 * 64 * 4));
@@ -671,27 +673,36 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (cluster || tfce) BPG_HIP(ctx, pl->bits.ensure((size_t)K * pl->nwords));
 
     // pinned staging (free at entry: every phase ends with a stream sync)
-    const size_t nQt = m.Qt.size(), nQn = (size_t)NR * QB;
-    BPG_HIP(ctx, pl->h_model.ensure(nQt + nQn + NR));
     BPG_HIP(ctx, pl->h_max.ensure(K));
     if (m.sign_operand) BPG_HIP(ctx, pl->h_sign.ensure((size_t)K * N));
     else BPG_HIP(ctx, pl->h_perm.ensure((size_t)K * N));
-    double* hm = pl->h_model.p;
-    std::memcpy(hm, m.Qt.data(), nQt * sizeof(double));
-    std::fill(hm + nQt, hm + nQt + nQn + NR, 0.0);
-    for (int j = 0; j < m.q; j++)
-        for (int s = 0; s < N; s++) hm[nQt + (size_t)s * QB + j] = m.Qn[(size_t)j * N + s];
-    // the FP64 form reads y* itself (its operand carries the scale)
-    if (f64) std::fill(hm + nQt + nQn, hm + nQt + nQn + N, 1.0);
-    else std::memcpy(hm + nQt + nQn, m.dscale.data(), N * sizeof(double));
-
+    // the map's model [Qt | Qn | dscale]: the copy uploaded at creation; the
+    // FP64 form reads y* itself (its operand carries the scale), so it takes
+    // one with unit scales, uploaded here
+    const size_t nQt = m.Qt.size();
+    const double* dQt = pl->mdev.p + pl->mdev_off[map];
+    const double* dQn = dQt + nQt;
+    const double* dds = dQn + (size_t)NR * std::max(QB, 1);
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
-    BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, hm, nQt * sizeof(double), hipMemcpyHostToDevice, st));
-    if (nQn) BPG_HIP(ctx, hipMemcpyAsync(pl->Qn.p, hm + nQt, nQn * sizeof(double), hipMemcpyHostToDevice, st));
-    BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, hm + nQt + nQn, NR * sizeof(double), hipMemcpyHostToDevice, st));
+    if (f64) {
+        const size_t nQn = (size_t)NR * QB;
+        BPG_HIP(ctx, pl->h_model.ensure(nQt + nQn + NR));
+        double* hm = pl->h_model.p;
+        std::memcpy(hm, m.Qt.data(), nQt * sizeof(double));
+        std::fill(hm + nQt, hm + nQt + nQn + NR, 0.0);
+        for (int j = 0; j < m.q; j++)
+            for (int s = 0; s < N; s++) hm[nQt + (size_t)s * QB + j] = m.Qn[(size_t)j * N + s];
+        std::fill(hm + nQt + nQn, hm + nQt + nQn + N, 1.0);
+        BPG_HIP(ctx, hipMemcpyAsync(pl->Qt.p, hm, nQt * sizeof(double), hipMemcpyHostToDevice, st));
+        if (nQn) BPG_HIP(ctx, hipMemcpyAsync(pl->Qn.p, hm + nQt, nQn * sizeof(double), hipMemcpyHostToDevice, st));
+        BPG_HIP(ctx, hipMemcpyAsync(pl->dscale.p, hm + nQt + nQn, NR * sizeof(double), hipMemcpyHostToDevice, st));
+        dQt = pl->Qt.p;
+        dQn = pl->Qn.p;
+        dds = pl->dscale.p;
+    }
     {
         bpg::TransformArgs ta{};
-        ta.Yc = pl->Yc.p; ta.Qn = pl->Qn.p; ta.dscale = pl->dscale.p;
+        ta.Yc = pl->Yc.p; ta.Qn = dQn; ta.dscale = dds;
         ta.N = N; ta.q = m.q; ta.Vc = Vc; ta.Vpad = Vpad;
         ta.KK = bf ? KK : 0; ta.NS = bf ? 0 : f64 ? NS64 : NS; ta.npy = npy;
         ta.yy = pl->yy.p; ta.Yb = bf ? pl->Yb.p : nullptr; ta.Y32 = bf || f64 || mx ? nullptr : pl->Y32.p;
@@ -737,7 +748,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         bpg::OperandArgs oa{};
         oa.perm = m.sign_operand ? nullptr : pl->perm.p;
         oa.sign = m.sign_operand ? pl->sign.p : nullptr;
-        oa.Qt = pl->Qt.p;
+        oa.Qt = dQt;
         oa.N = N; oa.pfit = pfit; oa.K = (int)K; oa.PP = PP; oa.KK = KK; oa.NS = NS;
         if (bf) {
             oa.n_sub = n_sub;
@@ -767,7 +778,7 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.Gf8 = pl->Gf8.p; ca.Gsc = pl->Gsc.p; ca.Yf8 = pl->Yf8.p; ca.Ysc = pl->Ysc.p;
         ca.perm = m.sign_operand ? nullptr : pl->perm.p;
         ca.sign = m.sign_operand ? pl->sign.p : nullptr;
-        ca.Qt = pl->Qt.p;
+        ca.Qt = dQt;
         ca.dof = (double)m.dof;
         ca.yy = pl->yy.p; ca.partial = pl->partial.p;
         ca.stats = want_stats ? pl->stats.p : nullptr;
@@ -808,8 +819,8 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
             // the listed samples, recomputed in FP64, into the maxima
             bpg::SampleFixArgs fa{};
             fa.samples = pl->samples.p; fa.n_samples = pl->n_samples.p; fa.cap = scap;
-            fa.Yc = pl->Yc.p; fa.Qn = pl->Qn.p; fa.QB = std::max(QB, 0); fa.q = m.q;
-            fa.Qt = pl->Qt.p; fa.perm = pl->perm.p;
+            fa.Yc = pl->Yc.p; fa.Qn = dQn; fa.QB = std::max(QB, 1); fa.q = m.q;
+            fa.Qt = dQt; fa.perm = pl->perm.p;
             fa.N = N; fa.pfit = pfit; fa.Vc = Vc; fa.K = (int)K;
             fa.dof = (double)m.dof;
             fa.maxout = pl->maxout.p;
