@@ -684,6 +684,13 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     const double* dQn = dQt + nQt;
     const double* dds = dQn + (size_t)NR * std::max(QB, 1);
     BPG_HIP(ctx, hipEventRecord(pl->ev[0], st));
+    // the contraction's counters and flags, cleared ahead of the transform
+    // (beside the permutations' staging rather than before the contraction)
+    BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));
+    BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
+    if (cluster) BPG_HIP(ctx, hipMemsetAsync(pl->ties.p, 0, K, st));
+    if (scap)
+        BPG_HIP(ctx, hipMemsetAsync(pl->n_samples.p, 0, bpg::BPG_SAMPLE_SEGS * sizeof(unsigned long long), st));
     if (f64) {
         const size_t nQn = (size_t)NR * QB;
         BPG_HIP(ctx, pl->h_model.ensure(nQt + nQn + NR));
@@ -803,11 +810,6 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
         ca.samples = scap ? pl->samples.p : nullptr;
         ca.n_samples = scap ? pl->n_samples.p : nullptr;
         ca.sample_cap = scap;
-        BPG_HIP(ctx, hipMemsetAsync(pl->unsafe.p, 0, sizeof(unsigned), st));
-        BPG_HIP(ctx, hipMemsetAsync(pl->flags.p, 0, K, st));
-        if (cluster) BPG_HIP(ctx, hipMemsetAsync(pl->ties.p, 0, K, st));
-        if (scap)
-            BPG_HIP(ctx, hipMemsetAsync(pl->n_samples.p, 0, bpg::BPG_SAMPLE_SEGS * sizeof(unsigned long long), st));
         if (bf) BPG_HIP(ctx, bpg::launch_contract_bf16(ca, Lb, KK, PP, m.sign_operand, st));
         else if (f64) BPG_HIP(ctx, bpg::launch_contract_f64(ca, Lf, st));
         else if (mx) BPG_HIP(ctx, bpg::launch_contract_f32mx(ca, Lm, PP, KG, st));
@@ -909,7 +911,6 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     if (stats)
         BPG_HIP(ctx, hipMemcpyAsync(stats, pl->stats.p, (size_t)K * Vc * sizeof(float), hipMemcpyDeviceToHost, st));
     BPG_HIP(ctx, hipMemcpyAsync(pl->h_unsafe.p, pl->unsafe.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    if (!f64) BPG_HIP(ctx, hipMemcpyAsync(pl->h_flags.p, pl->flags.p, K, hipMemcpyDeviceToHost, st));
     if (scap)
         BPG_HIP(ctx, hipMemcpyAsync(pl->h_nsamples.p, pl->n_samples.p,
                                     bpg::BPG_SAMPLE_SEGS * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -937,6 +938,13 @@ int phase_enqueue(bpg_plan* pl, int map, const unsigned short* perm, const float
     ps.sample_cap = scap;
     ps.fp64_samples = 0;
     ps.t0 = t0;
+    return BPG_OK;
+}
+
+// The flags of the finished phase ([K] bytes), copied when they are needed:
+// only a phase that met ill-conditioned samples reads them.
+int fetch_flags(bpg_plan* pl, long K) {
+    BPG_HIP(pl->ctx, hipMemcpy(pl->h_flags.p, pl->flags.p, (size_t)K, hipMemcpyDeviceToHost));
     return BPG_OK;
 }
 
@@ -975,6 +983,7 @@ int repeat_subset(bpg_plan* pl, std::vector<long>& S, int form, std::vector<sign
     if (rc != BPG_OK) return rc;
     BPG_HIP(ctx, hipStreamSynchronize(pl->stream));
     const bool again = form != BPG_PREC_FP64 && *pl->h_unsafe.p > 0;
+    if (again && fetch_flags(pl, Ks) != BPG_OK) return BPG_ERR_HIP;
     const unsigned* sub = reinterpret_cast<const unsigned*>(pl->h_max.p);
     std::vector<long> S2;
     for (long i = 0; i < Ks; i++) {
@@ -1019,6 +1028,7 @@ int phase_finish(bpg_plan* pl, float* maxdist) {
         }
         std::vector<signed char> final_form(ps.K, (signed char)ps.form);
         std::vector<long> S;
+        if (fetch_flags(pl, ps.K) != BPG_OK) return BPG_ERR_HIP;
         for (long k = 0; k < ps.K; k++)
             if (pl->h_flags.p[k]) S.push_back(k);
         int form = ps.form;
@@ -1282,6 +1292,10 @@ extern "C" int bpg_plans_permutation_phase(bpg_plan* const* plans, int n, const 
         if (hipStreamSynchronize(pl->stream) != hipSuccess || ps.form != BPG_PREC_SPLIT_BF16 || ps.exact ||
             *pl->h_unsafe.p == 0)
             continue;
+        if (fetch_flags(pl, ps.K) != BPG_OK) {
+            rc = BPG_ERR_HIP;
+            continue;
+        }
         long nf = 0;
         for (long k = 0; k < ps.K; k++) nf += pl->h_flags.p[k] != 0;
         if (nf * 2 <= ps.K) continue;
