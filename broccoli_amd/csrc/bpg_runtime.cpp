@@ -8,14 +8,6 @@
 
 namespace bpg {
 
-int quantize(float t) {
-    const float x = t * 10000.0f;
-    if (std::isnan(x)) return INT_MIN;
-    if (x >= 2147483648.0f) return INT_MAX;
-    if (x <= -2147483648.0f) return INT_MIN;
-    return (int)x;
-}
-
 float empty_max() {
     const float init = -1000000.0f;
     int m;

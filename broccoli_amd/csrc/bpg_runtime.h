@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
+#include <cmath>
 #include <map>
 #include <string>
 #include <tuple>
@@ -91,8 +93,15 @@ struct HBuf {
     }
 };
 
-// (int)(T * 10000.0f) as KM:299, made total (saturating, NaN -> INT_MIN).
-int quantize(float t);
+// (int)(T * 10000.0f) as KM:299, made total (saturating, NaN -> INT_MIN);
+// inline, so that the loops over K maxima vectorise
+inline int quantize(float t) {
+    const float x = t * 10000.0f;
+    if (std::isnan(x)) return INT_MIN;
+    if (x >= 2147483648.0f) return INT_MAX;
+    if (x <= -2147483648.0f) return INT_MIN;
+    return (int)x;
+}
 // The empty-mask value of LIB:10971: the int buffer starts from the bit
 // pattern of -1e6f and is returned as (float)max / 10000.0f.
 float empty_max();
