@@ -332,6 +332,35 @@ def accuracy(ctx, st, perms, nsample, forms=("split_bf16", "fp32")):
     return out
 
 
+def accuracy_cluster(ctx, st, perms, nsample, forms=("split_bf16", "fp32")):
+    """Cluster inference: sampled permutations' statistics of each form
+    against the FP64 oracle at the full grid, and their largest clusters
+    against the oracle's labelling of the FP64 statistics; a size that
+    differs counts as a tie when some FP64 statistic lies within
+    1e-4 (1 + |cdt|) of the threshold."""
+    from oracle import oracle
+
+    sub = {k: v[:nsample] for k, v in perms.items() if k in ("perm", "sign")}
+    inmask = st["mask"].ravel() == 1.0
+    ref = oracle.perm_stats(st, sub, nsample)[:, inmask]
+    band = 1e-4 * (1.0 + abs(st["cdt"]))
+    best64, near = [], []
+    for k in range(nsample):
+        g = np.zeros(inmask.size, np.float32)
+        g[inmask] = ref[k]
+        best64.append(oracle.ccl(g.reshape(st["mask"].shape), st["mask"], st["cdt"], st.get("connectivity", 26))[2])
+        near.append(bool((np.abs(ref[k] - st["cdt"]) <= band).any()))
+    out = {}
+    for prec in forms:
+        with capi.Plan(ctx, st, prec) as plan:
+            md, stats = plan.permutation_phase(0, want_stats=True, **sub)
+        err = np.abs(stats - ref) / (1.0 + np.abs(ref))
+        differ = [k for k in range(nsample) if md[k] != float(best64[k])]
+        out[prec] = dict(max_err=float(err.max()), sampled_perms=nsample, cluster_sizes_differing=len(differ),
+                         differing_outside_ties=sum(1 for k in differ if not near[k]))
+    return out
+
+
 def full_k_check(st, perms, K, md, md32, obs, P, ties, ncores):
     """Every permutation's quantised max-T against the FP64 oracle's, and the
     p-value counts of the observed map against those of the FP64 null.
@@ -449,7 +478,6 @@ def main():
     if args.workload != 2:
         # extra data points: no CPU baseline or oracle sample at these sizes
         args.no_cpu = True
-        args.no_accuracy = args.no_accuracy or args.workload == 3
     V = st["V"]
     N, p = st["N"], st["p"]
     ctx = capi.Context(0)
@@ -538,7 +566,10 @@ def main():
             res["speedup_vs_fp8"] = dt8 / dt
     if not args.no_accuracy:
         forms = ("split_bf16", "fp32", "split_fp8") if args.workload == 5 else ("split_bf16", "fp32")
-        res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8, forms)
+        if st.get("inference", capi.VOXEL) in (capi.CLUSTER_EXTENT, capi.CLUSTER_MASS):
+            res["accuracy_vs_fp64"] = accuracy_cluster(ctx, st, perms, 8, forms)
+        else:
+            res["accuracy_vs_fp64"] = accuracy(ctx, st, perms, 8, forms)
         if args.workload in (1, 2):
             full = full_k_check(st, perms, K, md, md32, obs, Pv, pties, host_cores())
             res["accuracy_vs_fp64"]["full_K"] = full
