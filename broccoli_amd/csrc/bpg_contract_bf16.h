@@ -85,6 +85,13 @@ struct TileSpan {
     int first, last;
 };
 
+// The rolling pipelined schedule: pipelined general operands (two data
+// fragments) at 5..8 k-steps.  Above 8 the ping-pong pipelined schedule
+// runs, whose accumulators need the AGPRs of a single wave per SIMD.
+constexpr bool rolling_pipelined(int KK, bool sign, bool pipe) {
+    return pipe && bf16_y_pieces(KK, sign) == 2 && KK >= BPG_ROLL_MIN_KK && KK <= 8;
+}
+
 // Statistic and running max of global subtile st over voxel tile vt; vt < 0
 // marks the empty item that opens the pipelined schedule (its NaN norm is
 // dropped by fmaxf and it writes nothing).  OUT = 0: the max only, no
@@ -428,13 +435,13 @@ __device__ __forceinline__ void contract_item(const ContractArgs& a, const Bf16L
             y0 = yn;
         }
     }
-    // Rolling pipelined schedule (two data fragments, KK >= 5, pipelined):
+    // Rolling pipelined schedule (two data fragments, KK 5..8, pipelined):
     // one data buffer; the subtiles' chains run one after another, each
     // interleaved with the epilogue of the accumulator before it (the last
     // subtile of the previous tile for the first), and the last chain hands
     // each k-step's registers to the next tile's data once it has used them
     // (every load then has KK - 1 k-steps of MFMAs to arrive).
-    constexpr bool RPIPE = PIPE && NPY == 2 && KK >= BPG_ROLL_MIN_KK;
+    constexpr bool RPIPE = rolling_pipelined(KK, SIGN, PIPE);
     if constexpr (RPIPE) {
         constexpr int NM = KK * (SIGN ? 2 : 3);
         constexpr int NV = (PER * (PP + 5) + NM - 1) / NM;
@@ -541,9 +548,11 @@ __device__ __forceinline__ void contract_body(const ContractArgs& a, const Bf16L
     contract_item<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L, lb % L.n_mgroups, lb / L.n_mgroups);
 }
 
-// (the rolling pipelined schedule held to 256 registers, two waves per SIMD)
+// (the rolling pipelined schedule held to 256 registers, two waves per SIMD;
+// the ping-pong schedule above 8 k-steps keeps its 256 VGPRs and AGPRs at
+// one wave per SIMD)
 template <int KK, int R, int PP, bool SIGN, bool FT, int OUT, bool PIPE>
-__global__ __launch_bounds__(256, (PIPE && bf16_y_pieces(KK, SIGN) == 2 && KK >= BPG_ROLL_MIN_KK ? 2 : 1))
+__global__ __launch_bounds__(256, (rolling_pipelined(KK, SIGN, PIPE) ? 2 : 1))
 void k_contract_bf16(ContractArgs a, Bf16Launch L) {
     contract_body<KK, R, PP, SIGN, FT, OUT, PIPE>(a, L);
 }
