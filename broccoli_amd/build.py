@@ -32,6 +32,25 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build broccoli_amd)")
 
 
+# per-kernel register use of the device code, as the compiler reports it
+# (name, VGPRs, AGPRs, scratch bytes per lane, spilled VGPRs): the tests check
+# that no contraction kernel spills
+RESOURCES = BUILD / "kernel_resources.tsv"
+
+
+def _resources(stderr: str) -> list[tuple[str, int, int, int, int]]:
+    import re
+    rows = []
+    for part in stderr.split("Function Name: ")[1:]:
+        name = part.split()[0]
+
+        def get(key):
+            m = re.search(key + r": (\d+)", part)
+            return int(m.group(1)) if m else -1
+        rows.append((name, get("VGPRs"), get("AGPRs"), get(r"ScratchSize \[bytes/lane\]"), get("VGPRs Spill")))
+    return rows
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
     extra = []
@@ -39,14 +58,19 @@ def _compile(src: Path) -> Path:
         # keep MFMA accumulators in architectural VGPRs: the epilogue reads
         # them directly instead of through v_accvgpr_read
         extra = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
-    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra,
+           "-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [hipcc(), *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-x", "c++",
                "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    _compile.resources[src.name] = _resources(res.stderr) if src.suffix == ".hip" else []
     return obj
+
+
+_compile.resources = {}
 
 
 def build(verbose: bool = False) -> Path:
@@ -54,6 +78,11 @@ def build(verbose: bool = False) -> Path:
     sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         objs = list(ex.map(_compile, sources))
+    with open(RESOURCES, "w") as f:
+        f.write("source\tkernel\tvgprs\tagprs\tscratch_bytes\tvgpr_spills\n")
+        for name in sorted(_compile.resources):
+            for row in _compile.resources[name]:
+                f.write("\t".join([name, *map(str, row)]) + "\n")
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -7,7 +7,8 @@
 //  - operand subtiles kept per wave: BPG_R_SIGN for exact +-1 operands with
 //    KK <= 4 (N <= 64), BPG_R_SMALL for general operands with KK <= 4, 2
 //    up to KK = BPG_R2_KK, else 1 (R = 2 at KK = 13 needs 256 VGPRs and
-//    200 AGPRs, one wave per SIMD, and halves the data-tile loads);
+//    about 220 AGPRs, one wave per SIMD, and halves the data-tile loads; at
+//    14..16 it spills, and R = 1 ran 8% faster at KK = 15);
 //  - BPG_PIPE_SMALL / _MID / _LARGE: software-pipelined epilogue for
 //    general operands at KK <= 4, KK <= 8 and above;
 //  - the number of blocks the voxel range is split over.
@@ -18,7 +19,7 @@
 #define BPG_R_SMALL 3
 #endif
 #ifndef BPG_R2_KK
-#define BPG_R2_KK 16
+#define BPG_R2_KK 13
 #endif
 // operand subtiles per wave at KK 5..8 (the rolling schedule)
 #ifndef BPG_R_MID

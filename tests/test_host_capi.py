@@ -271,3 +271,37 @@ def test_validate_study_refuses_bad_designs():
     assert capi.validate_study(st)[0] == -1
     st = _small_study(None)
     assert capi.validate_study(st, precision=7)[0] == -1
+
+
+def test_max_t_contraction_kernels_do_not_spill():
+    """The build's kernel report (broccoli_amd/_build/kernel_resources.tsv,
+    the compiler's resource remarks): no split-BF16 contraction kernel of the
+    max-T paths (output modes 0 and 3, and the persistent form) spills
+    vector registers to scratch.  A launch bound that capped a 13-k-step
+    kernel at 256 registers once made it spill and run 3x slower; this
+    catches that at build time.  (The bitmask and statistics modes are not
+    held to it: several of their instantiations spill a little by design,
+    e.g. the rolling schedule's bits-only kernels at 8 k-steps, held to 256
+    registers for two waves per SIMD.)"""
+    report = ROOT / "broccoli_amd" / "_build" / "kernel_resources.tsv"
+    if not report.exists():
+        pytest.skip("no build report (run the build first)")
+    rows = [line.split("\t") for line in report.read_text().splitlines()[1:]]
+    names = subprocess.run(["c++filt"], input="\n".join(r[1] for r in rows), capture_output=True,
+                           text=True).stdout.splitlines()
+    checked, spilling = 0, []
+    for r, name in zip(rows, names):
+        m = re.search(r"k_contract_bf16(?:_roll)?<(\d+), \d+, \d+, \w+, \w+, (\d+), \w+>", name)
+        p = re.search(r"k_contract_bf16_persist<\d+, \d+, \d+, \w+, \w+, (\d+)>", name)
+        if m:
+            hot = int(m.group(2)) in (0, 3)
+        elif p:
+            hot = True
+        else:
+            continue
+        if hot:
+            checked += 1
+            if int(r[5]) > 0:
+                spilling.append(name)
+    assert checked > 100
+    assert not spilling, spilling[:5]
